@@ -1,0 +1,222 @@
+// hg_cell.hpp — per-cell arithmetic of the erosion passes, shared by the
+// per-pass kernels and the fused kernels so that both produce bit-identical
+// results for the same inputs.
+//
+// Semantics follow the reference GLSL shaders (ger0/hydro-gen, glsl/*.glsl);
+// each function cites the lines it restates.  Where the code is arranged for
+// fewer instructions than the literal shader, the arithmetic differs from the
+// shader's FP32 rounding only (never in meaning):
+//  * slope: sin_a = sqrt((dx^2+dz^2)/(dx^2+dz^2+4)) is the closed form of
+//    |cross((2,dx,0),(0,dz,2))| normalisation (hydro_erosion.glsl:23-35,57);
+//  * talus test: D/d > tan(Kalpha) replaces atan(D/d) > Kalpha, and the
+//    sharpness 1 + atan(max marked D/d) - Kalpha takes one atan per layer
+//    instead of one per direction (thermal_erosion.glsl:63-91), since atan is
+//    monotonic;
+//  * thermal outflow S*D_k/bk is formed as (S/bk)*D_k, so a cell's outflows
+//    are described by one scalar g = S/bk and its height differences;
+//  * mix(a,b,t) = a*(1-t)+b*t is evaluated as a + t*(b-a).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hgk {
+
+// Out-of-grid total height, FP32 literal 999999999999.0
+// (hydro_flux.glsl:33-39, thermal_erosion.glsl:20-26).
+#define HG_OOB_HEIGHT 999999999999.0f
+
+// Kernel-side copy of the erosion coefficients with the per-layer thermal
+// thresholds precomputed on the host.
+struct KParams {
+    float Kc, Kconv, Ke, E, G, dt;
+    float Ks[2], Kd[2], Kalpha[2], Kspeed[2];
+    float tan_alpha[2];       // tan(Kalpha[l])
+    float thr_diag[2];        // sqrt(2) * tan(Kalpha[l])
+    float smooth_m;           // clamp(Kspeed[1]*dt, 0, 1)
+    float evap;               // 1 - Ke*dt
+    float rain_amount;
+    int rain_mode;            // HG_RAIN_*
+    uint32_t seed_lo, seed_hi;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                               uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = uint32_t(p1);
+        uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = uint32_t(p0);
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__host__ __device__ inline float u24(uint32_t w) { return float(w >> 8) * (1.0f / 16777216.0f); }
+
+// ---------------------------------------------------------------------------
+// hydro_flux (glsl/hydro_flux.glsl:77-166) for one cell.
+//  f[4]      : own old flux (L,R,T,B), replaced by the new flux
+//  d1        : own water (after rain)
+//  w         : own total height, wn[4] : neighbour totals (OOB = HG_OOB_HEIGHT)
+//  fin[4]    : neighbours' old flux pointing at this cell
+//              (F(p+L).R, F(p+R).L, F(p+T).B, F(p+B).T), 0 out of grid
+//  wall      : bit0 x==0, bit1 x==N-1, bit2 y==0, bit3 y==N-1
+// Returns the new water d2; writes velocity (vx, vy, vz).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float flux_cell(float f[4], float d1, float w, const float wn[4],
+                                           const float fin[4], unsigned wall, const KParams& k,
+                                           float& vx, float& vy, float& vz) {
+    const float fo0 = f[0], fo1 = f[1], fo2 = f[2], fo3 = f[3];
+    const float c = k.dt * k.G;  // dt*A*G/L with A = L = 1
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f[i] = fmaxf(0.0f, k.E * f[i] + c * (w - wn[i]));   // :100-107
+    if (wall & 1u) f[0] = 0.0f; else if (wall & 2u) f[1] = 0.0f;                      // :110-119
+    if (wall & 4u) f[3] = 0.0f; else if (wall & 8u) f[2] = 0.0f;
+    const float sum_in = (fin[0] + fin[1]) + (fin[2] + fin[3]);
+    float sum_out = (f[0] + f[1]) + (f[2] + f[3]);
+    // K = min(1, b/(sum_out*dt)); sum_out == 0 gives K = 1 (:125)
+    const float q = d1 / (sum_out * k.dt);
+    const float K = (q < 1.0f) ? q : 1.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f[i] *= K;
+    sum_out *= K;
+    const float d2 = fmaxf(0.0f, d1 + k.dt * (sum_in - sum_out));                      // :133-134
+    vz = d1 + d2;                                                                      // :144
+    if (vz > 0.0f) {                                                                   // :146-161
+        const float inv = 1.0f / vz;
+        vx = ((fin[0] - fo0) + (fo1 - fin[1])) * inv;
+        vy = ((fin[3] - fo3) + (fo2 - fin[2])) * inv;
+    } else {
+        vx = 0.0f;
+        vy = 0.0f;
+    }
+    return d2;
+}
+
+// ---------------------------------------------------------------------------
+// hydro_erosion (glsl/hydro_erosion.glsl:37-92) for one cell.
+// hl, hr, ht, hb: rock+dirt of the L, R, T, B neighbours (0 out of grid).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void erosion_cell(float& r, float& d, float& s0, float& s1, float vx,
+                                             float vy, float vz, float hl, float hr, float ht,
+                                             float hb, const KParams& k) {
+    float ero = sqrtf(vx * vx + vy * vy);                                              // :45-52
+    if (vz < 1e-3f) {
+        const float dd = fmaxf(5e-4f, vz);
+        float t = (dd - 1e-3f) * (1.0f / (5e-4f - 1e-3f));
+        t = fminf(fmaxf(t, 0.0f), 1.0f);
+        ero = ero * (1.0f - t * t * (3.0f - 2.0f * t));
+    }
+    const float dx = hr - hl, dz = ht - hb;                                            // :23-35
+    const float g2 = dx * dx + dz * dz;
+    const float sin_a = sqrtf(g2 / (g2 + 4.0f));                                       // :57
+    const float cap0 = k.Kc * fmaxf(0.02f, sin_a) * ero;
+    // layer loop, dirt (1) then rock (0), :58-83
+    float cap = 0.0f;
+    bool stop = false;
+    {
+        const float c = fmaxf(0.0f, cap0 - cap);
+        if (c > s1) {
+            const float old = d;
+            const float delta = (k.dt * k.Ks[1]) * (c - s1);
+            d -= delta;
+            s1 += delta;
+            if (d < 0.0f) { s1 += d; d = 0.0f; cap += old; }
+            else stop = true;
+        } else {
+            const float delta = (k.dt * k.Kd[1]) * (s1 - c);
+            d += delta;
+            s1 -= delta;
+        }
+    }
+    if (!stop) {
+        const float c = fmaxf(0.0f, cap0 - cap);
+        if (c > s0) {
+            const float delta = (k.dt * k.Ks[0]) * (c - s0);
+            r -= delta;
+            s0 += delta;
+            if (r < 0.0f) { s0 += r; r = 0.0f; }
+        } else {
+            const float delta = (k.dt * k.Kd[0]) * (s0 - c);
+            r += delta;
+            s0 -= delta;
+        }
+    }
+    const float conv = s0 * (k.Kconv * k.dt);                                          // :84-88
+    s1 += conv;
+    s0 -= conv;
+}
+
+// Semi-Lagrangian back-trace (sediment_transport.glsl:70 and :26-30): returns
+// the clamped sample position.
+__device__ __forceinline__ void back_trace(int x, int y, float vx, float vy, float dt, int nx, int ny,
+                                           float& bx, float& by) {
+    bx = fminf(fmaxf(float(x) - vx * dt, 0.0f), float(nx - 1));
+    by = fminf(fmaxf(float(y) - vy * dt, 0.0f), float(ny - 1));
+}
+
+__device__ __forceinline__ float lerp1(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+// ---------------------------------------------------------------------------
+// Thermal erosion (glsl/thermal_erosion.glsl:28-109) for one cell and layer.
+// D[8]: cumulative layer-top differences to the 8 neighbours in the order
+// L, R, T, B, LT, RT, LB, RB (OOB neighbours give a very negative D).
+// Returns g = S/bk (0 when no direction is marked); out_k = marked_k ? g*D_k : 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool talus_marked(float D, int k, int layer, const KParams& p) {
+    return D > (k < 4 ? p.tan_alpha[layer] : p.thr_diag[layer]);
+}
+
+__device__ __forceinline__ float thermal_g(const float D[8], float h_layer, int layer,
+                                           const KParams& p) {
+    float H = 0.0f;                                                                    // :46-58
+#pragma unroll
+    for (int k = 0; k < 8; ++k) H = fmaxf(H, D[k]);
+    H = fminf(h_layer, H);
+    float bk = 0.0f, rmax = 0.0f;
+    const float ta = p.tan_alpha[layer], td = p.thr_diag[layer];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (D[k] > ta) { bk += D[k]; rmax = fmaxf(rmax, D[k]); }
+#pragma unroll
+    for (int k = 4; k < 8; ++k)
+        if (D[k] > td) { bk += D[k]; rmax = fmaxf(rmax, D[k] * 0.70710678118654752f); }
+    if (!(bk > 0.0f)) return 0.0f;
+    float sharp = 1.0f + atanf(rmax) - p.Kalpha[layer];                                // :75-83
+    sharp = fmaxf(sharp, 1.0f);
+    sharp = sharp * (sharp * sharp);                                                   // :92
+    const float S = p.dt * p.Kspeed[layer] * sharp * H * 0.5f;                         // :95
+    return S / bk;
+}
+
+// ---------------------------------------------------------------------------
+// Smoothing, grid branch (glsl/smoothing.glsl:36-74,97-103), interior cell.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void smooth_cell(float& r, float& d, float lr, float lg, float rr,
+                                            float rg, float tr, float tg, float br, float bg,
+                                            float m) {
+    const float dlr = r - lr, dlg = (d - lg) + dlr;
+    const float drr = r - rr, drg = (d - rg) + drr;
+    const float dtr = r - tr, dtg = (d - tg) + dtr;
+    const float dbr = r - br, dbg = (d - bg) + dbr;
+    const float gh = fabsf(((dlg + drg) + (dtg + dbg)) * 0.25f);
+    const float rh = fabsf(((dlr + drr) + (dtr + dbr)) * 0.25f);
+    float nr = r, ng = d;
+    if (((-dlr > rh || -drr > rh) && dlr * drr > 0.0f) || ((-dtr > rh || -dbr > rh) && dtr * dbr > 0.0f))
+        nr = (r + lr + rr + tr + br) * 0.2f;
+    if (((-dlg > gh || -drg > gh) && dlg * drg > 0.0f) || ((-dtg > gh || -dbg > gh) && dtg * dbg > 0.0f))
+        ng = (d + lg + rg + tg + bg) * 0.2f;
+    r = fmaf(m, nr - r, r);
+    d = fmaf(m, ng - d, d);
+}
+
+}  // namespace hgk

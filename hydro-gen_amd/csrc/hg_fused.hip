@@ -1,0 +1,423 @@
+// hg_fused.hip — the GRID step in two fused kernels.
+//
+//  hydro   : rain -> hydro_flux -> hydro_erosion -> sediment_transport
+//            (glsl/rain.glsl, hydro_flux.glsl, hydro_erosion.glsl,
+//            sediment_transport.glsl).  Reads the 9 state planes once and
+//            writes them once (72 B/cell).
+//  thermal : thermal_erosion + thermal_transport for layer 0 then layer 1,
+//            then smoothing (glsl/thermal_erosion.glsl, thermal_transport.glsl,
+//            smoothing.glsl).  Reads rock+dirt once, writes them once (16 B/cell).
+//
+// Both kernels work on a TX x TY output tile per workgroup.  The tile plus a
+// halo of its dependency radius is staged in LDS, every intermediate stage is
+// evaluated on a region that shrinks by one cell per stage, and only the tile
+// is written back.  The per-cell arithmetic is the same inline code as the
+// per-pass kernels (hg_cell.hpp), so the fused and per-pass paths agree bit
+// for bit.
+//
+// The semi-Lagrangian gather of sediment_transport has no bounded radius
+// (|v|*dt is unbounded).  The hydro kernel evaluates eroded sediment on the
+// tile plus an HA-cell halo; a back-trace whose bilinear taps fall outside it
+// recomputes the eroded sediment of those taps from the step's input planes
+// (still intact: every pass is ping-pong), which is exact.
+#include "hg_kernels.hpp"
+
+namespace hgk {
+
+namespace {
+
+__device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * size_t(nx) + size_t(x); }
+
+// ---------------------------------------------------------------------------
+// Eroded sediment of one cell, recomputed from the step's input planes
+// (rain -> flux -> erosion at q).  Used for back-trace taps outside the halo.
+// ---------------------------------------------------------------------------
+__device__ void eroded_sediment_global(const Planes& in, int qx, int qy, int nx, int ny,
+                                       uint32_t step, bool rain, const KParams& k, float& s0o,
+                                       float& s1o) {
+    const float* R = in.p[HG_P_ROCK];
+    const float* D = in.p[HG_P_DIRT];
+    const float* B = in.p[HG_P_WATER];
+    auto water = [&](int x, int y) -> float {
+        float b = B[at(x, y, nx)];
+        if (rain) b = b + rain_incr(x, y, step, 0, k);
+        return b;
+    };
+    auto total = [&](int x, int y) -> float {
+        if (x < 0 || y < 0 || x >= nx || y >= ny) return HG_OOB_HEIGHT;
+        const size_t j = at(x, y, nx);
+        return (R[j] + D[j]) + water(x, y);
+    };
+    auto rd = [&](int x, int y) -> float {
+        if (x < 0 || y < 0 || x >= nx || y >= ny) return 0.0f;
+        const size_t j = at(x, y, nx);
+        return R[j] + D[j];
+    };
+    const size_t i = at(qx, qy, nx);
+    float f[4] = {in.p[HG_P_FLUX_L][i], in.p[HG_P_FLUX_R][i], in.p[HG_P_FLUX_T][i], in.p[HG_P_FLUX_B][i]};
+    const float b = water(qx, qy);
+    const float w = (R[i] + D[i]) + b;
+    const float wn[4] = {total(qx - 1, qy), total(qx + 1, qy), total(qx, qy + 1), total(qx, qy - 1)};
+    const float fin[4] = {qx > 0 ? in.p[HG_P_FLUX_R][i - 1] : 0.0f,
+                          qx < nx - 1 ? in.p[HG_P_FLUX_L][i + 1] : 0.0f,
+                          qy < ny - 1 ? in.p[HG_P_FLUX_B][i + nx] : 0.0f,
+                          qy > 0 ? in.p[HG_P_FLUX_T][i - nx] : 0.0f};
+    const unsigned wall = (qx == 0 ? 1u : 0u) | (qx == nx - 1 ? 2u : 0u) | (qy == 0 ? 4u : 0u) |
+                          (qy == ny - 1 ? 8u : 0u);
+    float vx, vy, vz;
+    (void)flux_cell(f, b, w, wn, fin, wall, k, vx, vy, vz);
+    float r = R[i], d = D[i];
+    s0o = in.p[HG_P_SED_ROCK][i];
+    s1o = in.p[HG_P_SED_DIRT][i];
+    erosion_cell(r, d, s0o, s1o, vx, vy, vz, rd(qx - 1, qy), rd(qx + 1, qy), rd(qx, qy + 1),
+                 rd(qx, qy - 1), k);
+}
+
+// ---------------------------------------------------------------------------
+// hydro kernel
+// ---------------------------------------------------------------------------
+template <int TX, int TY, int HA, int NT>
+struct HydroTile {
+    // input region: tile +- (HA+1) cells, loaded in aligned quads, so the LDS
+    // region spans x in [x0 - QH*4, x0 + TX + QH*4)
+    static constexpr int QH = (HA + 1 + 3) / 4;           // halo quads per side
+    static constexpr int WI = TX + 8 * QH;                 // LDS row width (cells)
+    static constexpr int OX = 4 * QH;                      // LDS x of tile column 0
+    static constexpr int HI = TY + 2 * (HA + 1);           // LDS rows
+    static constexpr int OY = HA + 1;                      // LDS y of tile row 0
+    static constexpr int WE = TX + 2 * HA, HE = TY + 2 * HA;  // eroded-sediment region
+    static constexpr int NE = WE * HE;
+    static constexpr int ITER_E = (NE + NT - 1) / NT;
+    static constexpr int NTILE = TX * TY;
+    static constexpr int ITER_T = (NTILE + NT - 1) / NT;
+    static constexpr int NLDS = WI * HI;
+    static_assert(2 * NE + 2 * NTILE <= 6 * NLDS, "aliased outputs must fit the input arrays");
+    static_assert(TX % 4 == 0, "tile width must be a multiple of 4");
+};
+
+template <int TX, int TY, int HA, int NT>
+__global__ __launch_bounds__(NT) void k_fused_hydro(Planes in, Planes out, int nx, int ny,
+                                                    uint32_t step, int rain, int keep_vel, KParams k) {
+    using T = HydroTile<TX, TY, HA, NT>;
+    // LDS planes (pitch WI): rd = rock+dirt, b = water after rain, fl..fb = old flux.
+    __shared__ float lds[6 * T::NLDS];
+    float* s_rd = lds;
+    float* s_b = lds + T::NLDS;
+    float* s_fl = lds + 2 * T::NLDS;
+    float* s_fr = lds + 3 * T::NLDS;
+    float* s_ft = lds + 4 * T::NLDS;
+    float* s_fb = lds + 5 * T::NLDS;
+
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int lx0 = x0 - T::OX, ly0 = y0 - T::OY;   // grid coords of LDS (0,0)
+
+    // ---- stage 0: stage inputs in aligned quads; rain is added here -------
+    {
+        constexpr int QW = T::WI / 4;
+        constexpr int NQ = QW * T::HI;
+        for (int q = tid; q < NQ; q += NT) {
+            const int qy = q / QW, qxq = q - qy * QW;
+            const int gx = lx0 + 4 * qxq, gy = ly0 + qy;
+            const int li = qy * T::WI + 4 * qxq;
+            float rd[4], b[4], fl[4], fr[4], ft[4], fb[4];
+            if (gy >= 0 && gy < ny && gx >= 0 && gx + 3 < nx) {
+                const size_t g = at(gx, gy, nx);
+                const float4 r4 = *reinterpret_cast<const float4*>(in.p[HG_P_ROCK] + g);
+                const float4 d4 = *reinterpret_cast<const float4*>(in.p[HG_P_DIRT] + g);
+                const float4 b4 = *reinterpret_cast<const float4*>(in.p[HG_P_WATER] + g);
+                const float4 l4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_L] + g);
+                const float4 rr4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_R] + g);
+                const float4 t4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_T] + g);
+                const float4 bb4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_B] + g);
+                rd[0] = r4.x + d4.x; rd[1] = r4.y + d4.y; rd[2] = r4.z + d4.z; rd[3] = r4.w + d4.w;
+                b[0] = b4.x; b[1] = b4.y; b[2] = b4.z; b[3] = b4.w;
+                fl[0] = l4.x; fl[1] = l4.y; fl[2] = l4.z; fl[3] = l4.w;
+                fr[0] = rr4.x; fr[1] = rr4.y; fr[2] = rr4.z; fr[3] = rr4.w;
+                ft[0] = t4.x; ft[1] = t4.y; ft[2] = t4.z; ft[3] = t4.w;
+                fb[0] = bb4.x; fb[1] = bb4.y; fb[2] = bb4.z; fb[3] = bb4.w;
+                if (rain) {
+                    float inc[4];
+                    if (k.rain_mode == HG_RAIN_UNIFORM) {
+                        uint32_t o[4];
+                        philox4x32_10(uint32_t(gx) >> 2, uint32_t(gy), step, 0u, k.seed_lo, k.seed_hi, o);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) inc[j] = k.rain_amount * u24(o[j]);
+                    } else {
+                        const float a = k.rain_mode == HG_RAIN_CONSTANT ? k.rain_amount : 0.0f;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) inc[j] = a;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) b[j] = b[j] + inc[j];
+                }
+            } else {
+                // whole quad outside the grid (n is a multiple of 8, so quads
+                // never straddle the border): flux reads a total of
+                // HG_OOB_HEIGHT and zero flux, erosion reads rock+dirt 0.
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    rd[j] = 0.0f; b[j] = HG_OOB_HEIGHT;
+                    fl[j] = 0.0f; fr[j] = 0.0f; ft[j] = 0.0f; fb[j] = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s_rd[li + j] = rd[j]; s_b[li + j] = b[j];
+                s_fl[li + j] = fl[j]; s_fr[li + j] = fr[j]; s_ft[li + j] = ft[j]; s_fb[li + j] = fb[j];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 1: flux + erosion on the tile +- HA region -----------------
+    float es0[T::ITER_E], es1[T::ITER_E], evx[T::ITER_E], evy[T::ITER_E];
+#pragma unroll
+    for (int it = 0; it < T::ITER_E; ++it) {
+        es0[it] = 0.0f; es1[it] = 0.0f; evx[it] = 0.0f; evy[it] = 0.0f;
+        const int e = tid + it * NT;
+        if (e >= T::NE) continue;
+        const int ey = e / T::WE, ex = e - ey * T::WE;
+        const int gx = x0 - HA + ex, gy = y0 - HA + ey;
+        if (gx < 0 || gy < 0 || gx >= nx || gy >= ny) continue;
+        const int li = (ey + 1) * T::WI + (ex + T::OX - HA);   // LDS index of this cell
+        float f[4] = {s_fl[li], s_fr[li], s_ft[li], s_fb[li]};
+        const float b = s_b[li];
+        const float w = s_rd[li] + b;
+        const float wn[4] = {s_rd[li - 1] + s_b[li - 1], s_rd[li + 1] + s_b[li + 1],
+                             s_rd[li + T::WI] + s_b[li + T::WI], s_rd[li - T::WI] + s_b[li - T::WI]};
+        const float fin[4] = {s_fr[li - 1], s_fl[li + 1], s_fb[li + T::WI], s_ft[li - T::WI]};
+        const unsigned wall = (gx == 0 ? 1u : 0u) | (gx == nx - 1 ? 2u : 0u) | (gy == 0 ? 4u : 0u) |
+                              (gy == ny - 1 ? 8u : 0u);
+        float vx, vy, vz;
+        const float d2 = flux_cell(f, b, w, wn, fin, wall, k, vx, vy, vz);
+        const size_t g = at(gx, gy, nx);
+        float r = in.p[HG_P_ROCK][g], d = in.p[HG_P_DIRT][g];
+        float s0 = in.p[HG_P_SED_ROCK][g], s1 = in.p[HG_P_SED_DIRT][g];
+        erosion_cell(r, d, s0, s1, vx, vy, vz, s_rd[li - 1], s_rd[li + 1], s_rd[li + T::WI],
+                     s_rd[li - T::WI], k);
+        es0[it] = s0; es1[it] = s1; evx[it] = vx; evy[it] = vy;
+        const bool tile = ex >= HA && ex < HA + TX && ey >= HA && ey < HA + TY;
+        if (tile) {
+            out.p[HG_P_ROCK][g] = r;
+            out.p[HG_P_DIRT][g] = d;
+            out.p[HG_P_WATER][g] = d2 * k.evap;     // sediment_transport.glsl:75
+            out.p[HG_P_FLUX_L][g] = f[0];
+            out.p[HG_P_FLUX_R][g] = f[1];
+            out.p[HG_P_FLUX_T][g] = f[2];
+            out.p[HG_P_FLUX_B][g] = f[3];
+            if (keep_vel) {
+                out.p[HG_P_VEL_U][g] = vx;
+                out.p[HG_P_VEL_V][g] = vy;
+                out.p[HG_P_VEL_DEPTH][g] = vz;
+            }
+        }
+    }
+    __syncthreads();   // all reads of the staged inputs are done
+
+    // eroded sediment of the region and velocity of the tile, aliased over LDS
+    float* s_s0 = lds;
+    float* s_s1 = lds + T::NE;
+    float* s_vx = lds + 2 * T::NE;
+    float* s_vy = lds + 2 * T::NE + T::NTILE;
+#pragma unroll
+    for (int it = 0; it < T::ITER_E; ++it) {
+        const int e = tid + it * NT;
+        if (e >= T::NE) continue;
+        s_s0[e] = es0[it];
+        s_s1[e] = es1[it];
+        const int ey = e / T::WE, ex = e - ey * T::WE;
+        if (ex >= HA && ex < HA + TX && ey >= HA && ey < HA + TY) {
+            const int t = (ey - HA) * TX + (ex - HA);
+            s_vx[t] = evx[it];
+            s_vy[t] = evy[it];
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 2: back-trace + bilinear gather on the tile -----------------
+    for (int t = tid; t < T::NTILE; t += NT) {
+        const int ty = t / TX, tx = t - ty * TX;
+        const int gx = x0 + tx, gy = y0 + ty;
+        if (gx >= nx || gy >= ny) continue;
+        float bx, by;
+        back_trace(gx, gy, s_vx[t], s_vy[t], k.dt, nx, ny, bx, by);
+        const int ix = int(bx), iy = int(by);
+        const float fx = bx - float(ix), fy = by - float(iy);
+        const int ix1 = min(ix + 1, nx - 1), iy1 = min(iy + 1, ny - 1);
+        // region-local coordinates of the taps
+        const int ex0 = ix - (x0 - HA), ey0 = iy - (y0 - HA);
+        const int ex1 = ix1 - (x0 - HA), ey1 = iy1 - (y0 - HA);
+        float a0[4], a1[4];
+        if (ex0 >= 0 && ey0 >= 0 && ex1 < T::WE && ey1 < T::HE) {
+            const int e00 = ey0 * T::WE + ex0, e10 = ey0 * T::WE + ex1;
+            const int e01 = ey1 * T::WE + ex0, e11 = ey1 * T::WE + ex1;
+            a0[0] = s_s0[e00]; a0[1] = s_s0[e10]; a0[2] = s_s0[e01]; a0[3] = s_s0[e11];
+            a1[0] = s_s1[e00]; a1[1] = s_s1[e10]; a1[2] = s_s1[e01]; a1[3] = s_s1[e11];
+        } else {
+            eroded_sediment_global(in, ix, iy, nx, ny, step, rain != 0, k, a0[0], a1[0]);
+            eroded_sediment_global(in, ix1, iy, nx, ny, step, rain != 0, k, a0[1], a1[1]);
+            eroded_sediment_global(in, ix, iy1, nx, ny, step, rain != 0, k, a0[2], a1[2]);
+            eroded_sediment_global(in, ix1, iy1, nx, ny, step, rain != 0, k, a0[3], a1[3]);
+        }
+        const size_t g = at(gx, gy, nx);
+        out.p[HG_P_SED_ROCK][g] = lerp1(lerp1(a0[0], a0[1], fx), lerp1(a0[2], a0[3], fx), fy);
+        out.p[HG_P_SED_DIRT][g] = lerp1(lerp1(a1[0], a1[1], fx), lerp1(a1[2], a1[3], fx), fy);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// thermal kernel: thermal layer 0, thermal layer 1, smoothing
+// ---------------------------------------------------------------------------
+template <int TX, int TY, int NT>
+struct ThermalTile {
+    static constexpr int H = 5;                     // dependency radius
+    static constexpr int W0 = TX + 2 * H, H0 = TY + 2 * H;
+    static constexpr int N0 = W0 * H0;
+};
+
+__device__ __forceinline__ int region_count(int r, int TXv, int TYv) { return (TXv + 2 * r) * (TYv + 2 * r); }
+
+template <int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_fused_thermal(Planes in, Planes out, int nx, int ny,
+                                                      int do_thermal, int do_smooth, KParams k) {
+    using T = ThermalTile<TX, TY, NT>;
+    constexpr int W0 = T::W0, N0 = T::N0;
+    // s_a: rock (stages A,B), then new dirt (stages D,E)
+    // s_d: dirt; s_g: layer-0 then layer-1 outflow scale g; s_r2: rock after layer 0
+    __shared__ float s_a[N0], s_d[N0], s_g[N0], s_r2[N0];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TX - T::H, y0 = blockIdx.y * TY - T::H;   // grid coords of LDS (0,0)
+    constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+    constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+
+    // ---- load rock, dirt (out of grid: HG_OOB_HEIGHT, thermal_erosion.glsl:20-26)
+    for (int i = tid; i < N0; i += NT) {
+        const int ly = i / W0, lx = i - ly * W0;
+        const int gx = x0 + lx, gy = y0 + ly;
+        float r = HG_OOB_HEIGHT, d = HG_OOB_HEIGHT;
+        if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
+            const size_t g = at(gx, gy, nx);
+            r = in.p[HG_P_ROCK][g];
+            d = in.p[HG_P_DIRT][g];
+        }
+        s_a[i] = r;
+        s_d[i] = d;
+        s_r2[i] = r;
+    }
+    __syncthreads();
+
+    // Iterate the cells of the region of radius `rad` around the tile.
+    auto for_region = [&](int rad, auto&& fn) {
+        const int w = TX + 2 * rad, h = TY + 2 * rad;
+        const int n = w * h;
+        for (int i = tid; i < n; i += NT) {
+            const int ry = i / w, rx = i - ry * w;
+            const int lx = rx + (T::H - rad), ly = ry + (T::H - rad);
+            const int gx = x0 + lx, gy = y0 + ly;
+            fn(ly * W0 + lx, gx, gy);
+        }
+    };
+    auto in_grid = [&](int gx, int gy) { return gx >= 0 && gy >= 0 && gx < nx && gy < ny; };
+
+    if (do_thermal) {
+        // ---- A: layer-0 outflow scale on radius 4
+        for_region(4, [&](int li, int gx, int gy) {
+            float g = 0.0f;
+            if (in_grid(gx, gy)) {
+                const float r = s_a[li];
+                float D[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) D[kk] = r - s_a[li + DY[kk] * W0 + DX[kk]];
+                g = thermal_g(D, r, 0, k);
+            }
+            s_g[li] = g;
+        });
+        __syncthreads();
+        // ---- B: layer-0 transport on radius 3 -> s_r2
+        for_region(3, [&](int li, int gx, int gy) {
+            if (!in_grid(gx, gy)) return;
+            const float r = s_a[li], g = s_g[li];
+            float fin[8], fout[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const int lj = li + DY[kk] * W0 + DX[kk];
+                const float Dk = r - s_a[lj];
+                fout[kk] = talus_marked(Dk, kk, 0, k) ? g * Dk : 0.0f;
+                fin[kk] = talus_marked(-Dk, kk, 0, k) ? s_g[lj] * (-Dk) : 0.0f;
+            }
+            s_r2[li] = r + thermal_delta(fin, fout);
+        });
+        __syncthreads();
+        // ---- C: layer-1 outflow scale on radius 2 (rock after layer 0)
+        for_region(2, [&](int li, int gx, int gy) {
+            float g = 0.0f;
+            if (in_grid(gx, gy)) {
+                const float r = s_r2[li], d = s_d[li];
+                float D[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const int lj = li + DY[kk] * W0 + DX[kk];
+                    D[kk] = (r - s_r2[lj]) + (d - s_d[lj]);
+                }
+                g = thermal_g(D, d, 1, k);
+            }
+            s_g[li] = g;
+        });
+        __syncthreads();
+        // ---- D: layer-1 transport on radius 1 -> s_a (new dirt)
+        for_region(1, [&](int li, int gx, int gy) {
+            if (!in_grid(gx, gy)) return;
+            const float r = s_r2[li], d = s_d[li], g = s_g[li];
+            float fin[8], fout[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const int lj = li + DY[kk] * W0 + DX[kk];
+                const float Dk = (r - s_r2[lj]) + (d - s_d[lj]);
+                fout[kk] = talus_marked(Dk, kk, 1, k) ? g * Dk : 0.0f;
+                fin[kk] = talus_marked(-Dk, kk, 1, k) ? s_g[lj] * (-Dk) : 0.0f;
+            }
+            s_a[li] = d + thermal_delta(fin, fout);
+        });
+        __syncthreads();
+    } else {
+        // no thermal: new dirt = dirt
+        for (int i = tid; i < N0; i += NT) s_a[i] = s_d[i];
+        __syncthreads();
+    }
+
+    // ---- E: smoothing on the tile, write back
+    for_region(0, [&](int li, int gx, int gy) {
+        if (!in_grid(gx, gy)) return;
+        float r = s_r2[li], d = s_a[li];
+        if (do_smooth && gx > 0 && gy > 0 && gx < nx - 1 && gy < ny - 1)
+            smooth_cell(r, d, s_r2[li - 1], s_a[li - 1], s_r2[li + 1], s_a[li + 1], s_r2[li + W0],
+                        s_a[li + W0], s_r2[li - W0], s_a[li - W0], k.smooth_m);
+        const size_t g = at(gx, gy, nx);
+        out.p[HG_P_ROCK][g] = r;
+        out.p[HG_P_DIRT][g] = d;
+    });
+}
+
+constexpr int HYDRO_TX = 64, HYDRO_TY = 16, HYDRO_HA = 2, HYDRO_NT = 256;
+constexpr int THERM_TX = 64, THERM_TY = 16, THERM_NT = 256;
+
+}  // namespace
+
+hipError_t launch_fused_hydro(const Planes& in, const Planes& out, int nx, int ny, uint32_t step,
+                              bool rain, bool keep_velocity, const KParams& k, hipStream_t s) {
+    const dim3 grid((nx + HYDRO_TX - 1) / HYDRO_TX, (ny + HYDRO_TY - 1) / HYDRO_TY);
+    k_fused_hydro<HYDRO_TX, HYDRO_TY, HYDRO_HA, HYDRO_NT><<<grid, HYDRO_NT, 0, s>>>(
+        in, out, nx, ny, step, rain ? 1 : 0, keep_velocity ? 1 : 0, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_thermal(const Planes& in, const Planes& out, int nx, int ny, bool thermal,
+                                bool smooth, const KParams& k, hipStream_t s) {
+    const dim3 grid((nx + THERM_TX - 1) / THERM_TX, (ny + THERM_TY - 1) / THERM_TY);
+    k_fused_thermal<THERM_TX, THERM_TY, THERM_NT><<<grid, THERM_NT, 0, s>>>(
+        in, out, nx, ny, thermal ? 1 : 0, smooth ? 1 : 0, k);
+    return hipGetLastError();
+}
+
+}  // namespace hgk

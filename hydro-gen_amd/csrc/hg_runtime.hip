@@ -1,0 +1,572 @@
+// hg_runtime.hip — host runtime behind the C ABI of include/hg.h.
+//
+// Replaces the reference's GL resource layer and erosion dispatch
+// (src/state.cpp:3-55, src/erosion.cpp:21-200): a context owns the SoA state
+// planes of one n x n terrain on one HIP device, a stream, the settings and
+// the step counter.  Every pass reads the current buffer of each plane and
+// writes the alternate buffer; the planes a pass produced are then swapped,
+// which is the Tex_pair::swap of the reference (src/shaderprogram.cpp:51-72)
+// per plane.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hg_kernels.hpp"
+
+using hgk::KParams;
+using hgk::Planes;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct KernelStat {
+    std::string name;
+    uint64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+struct PendingTiming {
+    hipEvent_t start, end;
+    int stat;
+};
+
+}  // namespace
+
+struct hg_ctx {
+    int device = 0;
+    int n = 0;
+    hipStream_t stream = nullptr;
+    void* block = nullptr;           // one allocation for all plane buffers
+    float* cur[HG_NUM_PLANES] = {};
+    float* alt[HG_NUM_PLANES] = {};
+    unsigned* minmax = nullptr;      // 2 words for the init reduction
+    hg_erosion_settings es{};
+    hg_rain_settings rs{};
+    int rain_mode = HG_RAIN_OFF;
+    uint64_t seed = 0;
+    uint64_t step = 0;
+    KParams kp{};
+    std::string err;
+    // timing
+    bool timing = false;
+    std::vector<KernelStat> stats;
+    std::vector<PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<float> host_tmp;
+};
+
+namespace {
+
+int fail(hg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg; else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(hg_ctx* c, hipError_t e, const char* what) {
+    return fail(c, HG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HG_HIP(ctx, call)                                       \
+    do {                                                        \
+        hipError_t e_ = (call);                                 \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+size_t plane_n(const hg_ctx* c) { return size_t(c->n) * size_t(c->n); }
+
+Planes planes_of(float* const* p) {
+    Planes P;
+    for (int i = 0; i < HG_NUM_PLANES; ++i) P.p[i] = p[i];
+    return P;
+}
+
+hg_erosion_settings reference_grid_defaults() {
+    // src/state.cpp:81-92
+    hg_erosion_settings s{};
+    s.particle_count = 0;
+    s.Kc = 0.2f;
+    s.Kalpha[0] = 1.3f; s.Kalpha[1] = 0.6f;
+    s.Kconv = 0.001f;
+    s.Ks[0] = 0.03f; s.Ks[1] = 0.09f;
+    s.Kd[0] = 0.01f; s.Kd[1] = 0.03f;
+    s.Ke = 0.03f;
+    s.energy_kept = 1.0f;
+    s.Kspeed[0] = 0.5f; s.Kspeed[1] = 2.0f;
+    s.G = 1.0f;
+    s.d_t = 0.001f;
+    return s;
+}
+
+hg_rain_settings reference_rain_defaults() {
+    // src/settings.hpp:13-25
+    hg_rain_settings r{};
+    r.amount = 0.01f;
+    r.mountain_thresh = 0.55f;
+    r.mountain_multip = 0.05f;
+    r.period = 512;
+    r.drops = 0.02f;
+    return r;
+}
+
+void update_kparams(hg_ctx* c) {
+    const hg_erosion_settings& s = c->es;
+    KParams& k = c->kp;
+    k.Kc = s.Kc; k.Kconv = s.Kconv; k.Ke = s.Ke; k.E = s.energy_kept; k.G = s.G; k.dt = s.d_t;
+    for (int l = 0; l < 2; ++l) {
+        k.Ks[l] = s.Ks[l]; k.Kd[l] = s.Kd[l]; k.Kalpha[l] = s.Kalpha[l]; k.Kspeed[l] = s.Kspeed[l];
+        k.tan_alpha[l] = std::tan(s.Kalpha[l]);
+        k.thr_diag[l] = std::sqrt(2.0f) * k.tan_alpha[l];
+    }
+    const float m = s.Kspeed[1] * s.d_t;                      // glsl/smoothing.glsl:75
+    k.smooth_m = m < 0.0f ? 0.0f : (m > 1.0f ? 1.0f : m);
+    k.evap = 1.0f - s.Ke * s.d_t;                             // sediment_transport.glsl:75
+    k.rain_amount = c->rs.amount;
+    k.rain_mode = c->rain_mode;
+    k.seed_lo = uint32_t(c->seed);
+    k.seed_hi = uint32_t(c->seed >> 32);
+}
+
+int stat_index(hg_ctx* c, const char* name) {
+    for (size_t i = 0; i < c->stats.size(); ++i)
+        if (c->stats[i].name == name) return int(i);
+    c->stats.push_back(KernelStat{name, 0, 0.0});
+    return int(c->stats.size()) - 1;
+}
+
+hipEvent_t take_event(hg_ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Resolve all pending kernel timings (synchronises on their end events).
+int resolve_timing(hg_ctx* c) {
+    for (auto& p : c->pending) {
+        HG_HIP(c, hipEventSynchronize(p.end));
+        float ms = 0.0f;
+        HG_HIP(c, hipEventElapsedTime(&ms, p.start, p.end));
+        c->stats[p.stat].launches += 1;
+        c->stats[p.stat].total_ms += double(ms);
+        c->event_pool.push_back(p.start);
+        c->event_pool.push_back(p.end);
+    }
+    c->pending.clear();
+    return HG_OK;
+}
+
+// Launch helper: brackets the launch with events when timing is on.
+template <class F>
+int timed_launch(hg_ctx* c, const char* name, F&& launch) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (!e0 || !e1) return fail(c, HG_ERR_HIP, "hipEventCreate failed");
+        HG_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    const hipError_t e = launch();
+    if (e != hipSuccess) return hip_fail(c, e, name);
+    if (c->timing) {
+        HG_HIP(c, hipEventRecord(e1, c->stream));
+        c->pending.push_back(PendingTiming{e0, e1, stat_index(c, name)});
+        if (c->pending.size() > 4096) {
+            const int rc = resolve_timing(c);
+            if (rc != HG_OK) return rc;
+        }
+    }
+    return HG_OK;
+}
+
+void swap_planes(hg_ctx* c, std::initializer_list<int> ps) {
+    for (int p : ps) std::swap(c->cur[p], c->alt[p]);
+}
+
+const char* pass_name(int pass) {
+    static const char* names[HG_NUM_PASSES] = {
+        "pass_rain", "pass_hydro_flux", "pass_hydro_erosion", "pass_sediment_transport",
+        "pass_thermal_flux_0", "pass_thermal_transport_0", "pass_thermal_flux_1",
+        "pass_thermal_transport_1", "pass_smoothing"};
+    return names[pass];
+}
+
+int run_pass(hg_ctx* c, int pass) {
+    const Planes in = planes_of(c->cur), out = planes_of(c->alt);
+    const uint32_t step = uint32_t(c->step);
+    int rc = timed_launch(c, pass_name(pass), [&] {
+        return hgk::launch_pass(pass, in, out, c->n, c->n, step, c->kp, c->stream);
+    });
+    if (rc != HG_OK) return rc;
+    switch (pass) {
+        case HG_PASS_RAIN: swap_planes(c, {HG_P_WATER}); break;
+        case HG_PASS_FLUX:
+            swap_planes(c, {HG_P_WATER, HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B, HG_P_VEL_U,
+                            HG_P_VEL_V, HG_P_VEL_DEPTH});
+            break;
+        case HG_PASS_EROSION: swap_planes(c, {HG_P_ROCK, HG_P_DIRT, HG_P_SED_ROCK, HG_P_SED_DIRT}); break;
+        case HG_PASS_TRANSPORT: swap_planes(c, {HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER}); break;
+        case HG_PASS_THERMAL_FLUX_0:
+        case HG_PASS_THERMAL_FLUX_1:
+            swap_planes(c, {HG_P_THERM_L, HG_P_THERM_R, HG_P_THERM_T, HG_P_THERM_B, HG_P_THERM_LT,
+                            HG_P_THERM_RT, HG_P_THERM_LB, HG_P_THERM_RB});
+            break;
+        case HG_PASS_THERMAL_TRANSPORT_0: swap_planes(c, {HG_P_ROCK}); break;
+        case HG_PASS_THERMAL_TRANSPORT_1: swap_planes(c, {HG_P_DIRT}); break;
+        case HG_PASS_SMOOTH: swap_planes(c, {HG_P_ROCK, HG_P_DIRT}); break;
+        default: break;
+    }
+    return HG_OK;
+}
+
+bool finite_settings(const hg_erosion_settings* s) {
+    const float v[] = {s->Kc, s->Kalpha[0], s->Kalpha[1], s->Kconv, s->Ks[0], s->Ks[1], s->Kd[0],
+                       s->Kd[1], s->Ke, s->energy_kept, s->Kspeed[0], s->Kspeed[1], s->G, s->d_t,
+                       s->density, s->init_volume, s->friction, s->inertia, s->min_volume,
+                       s->min_velocity};
+    for (float x : v)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
+    if (!out) return fail(nullptr, HG_ERR_INVALID, "hg_create: out is NULL");
+    *out = nullptr;
+    if (n < 8 || n > 32768 || (n % 8) != 0)
+        return fail(nullptr, HG_ERR_INVALID,
+                    "hg_create: n must be a multiple of 8 in [8, 32768] (the reference dispatches n/8 "
+                    "workgroups of 8x8)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, HG_ERR_HIP, "hg_create: no HIP device");
+    if (device < 0 || device >= ndev) return fail(nullptr, HG_ERR_INVALID, "hg_create: bad device");
+    (void)particle_count;
+    DeviceGuard guard(device);
+    hg_ctx* c = new hg_ctx();
+    c->device = device;
+    c->n = int(n);
+    const size_t pn = size_t(n) * n;
+    const size_t pitch = (pn * sizeof(float) + 255) / 256 * 256;
+    const size_t bytes = pitch * 2 * HG_NUM_PLANES + 256;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, HG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    e = hipMalloc(&c->block, bytes);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(nullptr, HG_ERR_OOM, std::string("hipMalloc state planes: ") + hipGetErrorString(e));
+    }
+    char* base = static_cast<char*>(c->block);
+    for (int p = 0; p < HG_NUM_PLANES; ++p) {
+        c->cur[p] = reinterpret_cast<float*>(base + pitch * (2 * p));
+        c->alt[p] = reinterpret_cast<float*>(base + pitch * (2 * p + 1));
+    }
+    c->minmax = reinterpret_cast<unsigned*>(base + pitch * 2 * HG_NUM_PLANES);
+    e = hipMemsetAsync(c->block, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(c->block);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(nullptr, HG_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
+    c->es = reference_grid_defaults();
+    c->rs = reference_rain_defaults();
+    update_kparams(c);
+    *out = c;
+    return HG_OK;
+}
+
+int hg_destroy(hg_ctx* c) {
+    if (!c) return HG_OK;
+    DeviceGuard guard(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) { (void)hipEventDestroy(p.start); (void)hipEventDestroy(p.end); }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    (void)hipFree(c->block);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HG_OK;
+}
+
+const char* hg_last_error(const hg_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int hg_size(const hg_ctx* c, uint32_t* n) {
+    if (!c || !n) return HG_ERR_INVALID;
+    *n = uint32_t(c->n);
+    return HG_OK;
+}
+
+int hg_set_erosion(hg_ctx* c, const hg_erosion_settings* s) {
+    if (!c) return HG_ERR_INVALID;
+    if (!s) return fail(c, HG_ERR_INVALID, "hg_set_erosion: settings is NULL");
+    if (!finite_settings(s)) return fail(c, HG_ERR_INVALID, "hg_set_erosion: non-finite setting");
+    if (!(s->d_t > 0.0f)) return fail(c, HG_ERR_INVALID, "hg_set_erosion: d_t must be > 0");
+    const float nonneg[] = {s->Kc, s->Kconv, s->Ks[0], s->Ks[1], s->Kd[0], s->Kd[1], s->Ke,
+                            s->energy_kept, s->Kspeed[0], s->Kspeed[1], s->G};
+    for (float x : nonneg)
+        if (x < 0.0f) return fail(c, HG_ERR_INVALID, "hg_set_erosion: negative coefficient");
+    for (int l = 0; l < 2; ++l)
+        if (!(s->Kalpha[l] >= 0.0f && s->Kalpha[l] < 1.5707963f))
+            return fail(c, HG_ERR_INVALID, "hg_set_erosion: Kalpha must be in [0, pi/2) radians");
+    c->es = *s;
+    update_kparams(c);
+    return HG_OK;
+}
+
+int hg_get_erosion(const hg_ctx* c, hg_erosion_settings* s) {
+    if (!c || !s) return HG_ERR_INVALID;
+    *s = c->es;
+    return HG_OK;
+}
+
+int hg_set_rain(hg_ctx* c, const hg_rain_settings* s, int mode, uint64_t seed) {
+    if (!c) return HG_ERR_INVALID;
+    if (!s) return fail(c, HG_ERR_INVALID, "hg_set_rain: settings is NULL");
+    if (mode < HG_RAIN_OFF || mode > HG_RAIN_UNIFORM) return fail(c, HG_ERR_INVALID, "hg_set_rain: bad mode");
+    if (!std::isfinite(s->amount) || s->amount < 0.0f)
+        return fail(c, HG_ERR_INVALID, "hg_set_rain: amount must be finite and >= 0");
+    c->rs = *s;
+    c->rain_mode = mode;
+    c->seed = seed;
+    update_kparams(c);
+    return HG_OK;
+}
+
+int hg_init_fbm(hg_ctx* c, const hg_fbm_params* p) {
+    if (!c) return HG_ERR_INVALID;
+    if (!p) return fail(c, HG_ERR_INVALID, "hg_init_fbm: params is NULL");
+    if (p->octaves < 1 || p->octaves > 32 || !std::isfinite(p->base_freq) || !std::isfinite(p->gain) ||
+        !std::isfinite(p->lacunarity) || !std::isfinite(p->height_lo) || !std::isfinite(p->height_hi) ||
+        !(p->height_hi >= p->height_lo))
+        return fail(c, HG_ERR_INVALID, "hg_init_fbm: bad parameters");
+    DeviceGuard guard(c->device);
+    uint8_t perm[256];
+    hgk::fbm_permutation(p->seed, perm);
+    const size_t pn = plane_n(c);
+    for (int pl = 0; pl < HG_NUM_PLANES; ++pl)
+        if (pl != HG_P_ROCK) HG_HIP(c, hgk::launch_fill(c->cur[pl], pn, 0.0f, c->stream));
+    int rc = timed_launch(c, "init_fbm", [&] {
+        return hgk::launch_fbm(c->cur[HG_P_ROCK], c->n, c->n, perm, p->octaves, p->lacunarity, p->gain,
+                               p->base_freq, c->stream);
+    });
+    if (rc != HG_OK) return rc;
+    HG_HIP(c, hgk::launch_minmax(c->cur[HG_P_ROCK], pn, c->minmax, c->stream));
+    HG_HIP(c, hgk::launch_rescale(c->cur[HG_P_ROCK], pn, c->minmax, p->height_lo, p->height_hi, c->stream));
+    c->step = 0;
+    return HG_OK;
+}
+
+int hg_rain(hg_ctx* c) {
+    if (!c) return HG_ERR_INVALID;
+    DeviceGuard guard(c->device);
+    return run_pass(c, HG_PASS_RAIN);
+}
+
+int hg_run_pass(hg_ctx* c, int pass) {
+    if (!c) return HG_ERR_INVALID;
+    if (pass < 0 || pass >= HG_NUM_PASSES) return fail(c, HG_ERR_INVALID, "hg_run_pass: bad pass");
+    DeviceGuard guard(c->device);
+    return run_pass(c, pass);
+}
+
+int hg_step_grid(hg_ctx* c, uint32_t nsteps, uint32_t flags) {
+    if (!c) return HG_ERR_INVALID;
+    const uint32_t known = HG_STEP_RAIN | HG_STEP_THERMAL | HG_STEP_SMOOTH | HG_STEP_PER_PASS |
+                           HG_STEP_KEEP_VELOCITY;
+    if (flags & ~known) return fail(c, HG_ERR_INVALID, "hg_step_grid: unknown flag bits");
+    DeviceGuard guard(c->device);
+    const bool rain = (flags & HG_STEP_RAIN) && c->rain_mode != HG_RAIN_OFF;
+    const bool thermal = (flags & HG_STEP_THERMAL) != 0;
+    const bool smooth = (flags & HG_STEP_SMOOTH) != 0;
+    for (uint32_t s = 0; s < nsteps; ++s) {
+        int rc = HG_OK;
+        if (flags & HG_STEP_PER_PASS) {
+            if (rain) rc = run_pass(c, HG_PASS_RAIN);
+            for (int p = HG_PASS_FLUX; rc == HG_OK && p <= HG_PASS_TRANSPORT; ++p) rc = run_pass(c, p);
+            if (thermal)
+                for (int p = HG_PASS_THERMAL_FLUX_0; rc == HG_OK && p <= HG_PASS_THERMAL_TRANSPORT_1; ++p)
+                    rc = run_pass(c, p);
+            if (rc == HG_OK && smooth) rc = run_pass(c, HG_PASS_SMOOTH);
+        } else {
+            const bool kv = (flags & HG_STEP_KEEP_VELOCITY) != 0;
+            const Planes in = planes_of(c->cur), out = planes_of(c->alt);
+            const uint32_t step = uint32_t(c->step);
+            rc = timed_launch(c, "fused_hydro", [&] {
+                return hgk::launch_fused_hydro(in, out, c->n, c->n, step, rain, kv, c->kp, c->stream);
+            });
+            if (rc == HG_OK) {
+                swap_planes(c, {HG_P_ROCK, HG_P_DIRT, HG_P_WATER, HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T,
+                                HG_P_FLUX_B, HG_P_SED_ROCK, HG_P_SED_DIRT});
+                if (kv) swap_planes(c, {HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH});
+                if (thermal || smooth) {
+                    const Planes in2 = planes_of(c->cur), out2 = planes_of(c->alt);
+                    rc = timed_launch(c, "fused_thermal", [&] {
+                        return hgk::launch_fused_thermal(in2, out2, c->n, c->n, thermal, smooth, c->kp,
+                                                         c->stream);
+                    });
+                    if (rc == HG_OK) swap_planes(c, {HG_P_ROCK, HG_P_DIRT});
+                }
+            }
+        }
+        if (rc != HG_OK) return rc;
+        c->step += 1;
+    }
+    return HG_OK;
+}
+
+int hg_get_step(const hg_ctx* c, uint64_t* step) {
+    if (!c || !step) return HG_ERR_INVALID;
+    *step = c->step;
+    return HG_OK;
+}
+
+int hg_set_step(hg_ctx* c, uint64_t step) {
+    if (!c) return HG_ERR_INVALID;
+    c->step = step;
+    return HG_OK;
+}
+
+int hg_read_plane(hg_ctx* c, int plane, float* dst, size_t n_floats) {
+    if (!c) return HG_ERR_INVALID;
+    if (plane < 0 || plane >= HG_NUM_PLANES) return fail(c, HG_ERR_INVALID, "hg_read_plane: bad plane");
+    if (!dst || n_floats != plane_n(c)) return fail(c, HG_ERR_INVALID, "hg_read_plane: bad buffer size");
+    DeviceGuard guard(c->device);
+    HG_HIP(c, hipMemcpyAsync(dst, c->cur[plane], n_floats * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    return HG_OK;
+}
+
+int hg_write_plane(hg_ctx* c, int plane, const float* src, size_t n_floats) {
+    if (!c) return HG_ERR_INVALID;
+    if (plane < 0 || plane >= HG_NUM_PLANES) return fail(c, HG_ERR_INVALID, "hg_write_plane: bad plane");
+    if (!src || n_floats != plane_n(c)) return fail(c, HG_ERR_INVALID, "hg_write_plane: bad buffer size");
+    DeviceGuard guard(c->device);
+    HG_HIP(c, hipMemcpyAsync(c->cur[plane], src, n_floats * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    return HG_OK;
+}
+
+int hg_read(hg_ctx* c, int channel, float* rgba, size_t n_floats) {
+    if (!c) return HG_ERR_INVALID;
+    const size_t pn = plane_n(c);
+    if (!rgba || n_floats != 4 * pn) return fail(c, HG_ERR_INVALID, "hg_read: bad buffer size");
+    int planes[4] = {-1, -1, -1, -1};
+    switch (channel) {
+        case HG_CH_HEIGHT: planes[0] = HG_P_ROCK; planes[1] = HG_P_DIRT; planes[2] = HG_P_WATER; break;
+        case HG_CH_FLUX: planes[0] = HG_P_FLUX_L; planes[1] = HG_P_FLUX_R; planes[2] = HG_P_FLUX_T; planes[3] = HG_P_FLUX_B; break;
+        case HG_CH_VELOCITY: planes[0] = HG_P_VEL_U; planes[1] = HG_P_VEL_V; planes[2] = HG_P_VEL_DEPTH; break;
+        case HG_CH_SEDIMENT: planes[0] = HG_P_SED_ROCK; planes[1] = HG_P_SED_DIRT; break;
+        default: return fail(c, HG_ERR_INVALID, "hg_read: bad channel");
+    }
+    c->host_tmp.resize(4 * pn);
+    for (int k = 0; k < 4; ++k) {
+        if (planes[k] < 0) continue;
+        const int rc = hg_read_plane(c, planes[k], c->host_tmp.data() + k * pn, pn);
+        if (rc != HG_OK) return rc;
+    }
+    for (size_t i = 0; i < pn; ++i)
+        for (int k = 0; k < 4; ++k) rgba[4 * i + k] = planes[k] < 0 ? 0.0f : c->host_tmp[k * pn + i];
+    if (channel == HG_CH_HEIGHT)
+        for (size_t i = 0; i < pn; ++i) rgba[4 * i + 3] = rgba[4 * i] + rgba[4 * i + 1] + rgba[4 * i + 2];
+    return HG_OK;
+}
+
+int hg_write(hg_ctx* c, int channel, const float* rgba, size_t n_floats) {
+    if (!c) return HG_ERR_INVALID;
+    const size_t pn = plane_n(c);
+    if (!rgba || n_floats != 4 * pn) return fail(c, HG_ERR_INVALID, "hg_write: bad buffer size");
+    int planes[4] = {-1, -1, -1, -1};
+    switch (channel) {
+        case HG_CH_HEIGHT: planes[0] = HG_P_ROCK; planes[1] = HG_P_DIRT; planes[2] = HG_P_WATER; break;
+        case HG_CH_FLUX: planes[0] = HG_P_FLUX_L; planes[1] = HG_P_FLUX_R; planes[2] = HG_P_FLUX_T; planes[3] = HG_P_FLUX_B; break;
+        case HG_CH_VELOCITY: planes[0] = HG_P_VEL_U; planes[1] = HG_P_VEL_V; planes[2] = HG_P_VEL_DEPTH; break;
+        case HG_CH_SEDIMENT: planes[0] = HG_P_SED_ROCK; planes[1] = HG_P_SED_DIRT; break;
+        default: return fail(c, HG_ERR_INVALID, "hg_write: bad channel");
+    }
+    c->host_tmp.resize(pn);
+    for (int k = 0; k < 4; ++k) {
+        if (planes[k] < 0) continue;
+        for (size_t i = 0; i < pn; ++i) c->host_tmp[i] = rgba[4 * i + k];
+        const int rc = hg_write_plane(c, planes[k], c->host_tmp.data(), pn);
+        if (rc != HG_OK) return rc;
+    }
+    return HG_OK;
+}
+
+int hg_plane_device_ptr(hg_ctx* c, int plane, void** ptr) {
+    if (!c) return HG_ERR_INVALID;
+    if (plane < 0 || plane >= HG_NUM_PLANES || !ptr) return fail(c, HG_ERR_INVALID, "hg_plane_device_ptr: bad argument");
+    *ptr = c->cur[plane];
+    return HG_OK;
+}
+
+int hg_sync(hg_ctx* c) {
+    if (!c) return HG_ERR_INVALID;
+    DeviceGuard guard(c->device);
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    return HG_OK;
+}
+
+int hg_get_stream(hg_ctx* c, void** stream) {
+    if (!c || !stream) return HG_ERR_INVALID;
+    *stream = static_cast<void*>(c->stream);
+    return HG_OK;
+}
+
+int hg_set_timing(hg_ctx* c, int enable) {
+    if (!c) return HG_ERR_INVALID;
+    c->timing = enable != 0;
+    return HG_OK;
+}
+
+int hg_kernel_stats(hg_ctx* c, int i, const char** name, uint64_t* launches, double* total_ms) {
+    if (!c) return HG_ERR_INVALID;
+    DeviceGuard guard(c->device);
+    const int rc = resolve_timing(c);
+    if (rc != HG_OK) return rc;
+    if (i < 0 || size_t(i) >= c->stats.size()) return HG_ERR_INVALID;
+    if (name) *name = c->stats[i].name.c_str();
+    if (launches) *launches = c->stats[i].launches;
+    if (total_ms) *total_ms = c->stats[i].total_ms;
+    return HG_OK;
+}
+
+int hg_reset_timing(hg_ctx* c) {
+    if (!c) return HG_ERR_INVALID;
+    DeviceGuard guard(c->device);
+    const int rc = resolve_timing(c);
+    if (rc != HG_OK) return rc;
+    c->stats.clear();
+    return HG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,293 @@
+"""ctypes binding of libhydrogen.so (the C ABI declared in include/hg.h).
+
+The library is the product; this module only marshals arguments.  Loading
+fails loudly when the built library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))   # hydro-gen_amd/
+LIB_PATH = os.path.join(PKG_ROOT, "libhydrogen.so")
+
+# ---- enums mirrored from include/hg.h -------------------------------------
+HG_OK, HG_ERR_INVALID, HG_ERR_HIP, HG_ERR_OOM, HG_ERR_STATE, HG_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+RAIN_OFF, RAIN_CONSTANT, RAIN_UNIFORM = 0, 1, 2
+CH_HEIGHT, CH_FLUX, CH_VELOCITY, CH_SEDIMENT = 0, 1, 2, 3
+(P_ROCK, P_DIRT, P_WATER, P_FLUX_L, P_FLUX_R, P_FLUX_T, P_FLUX_B, P_SED_ROCK, P_SED_DIRT,
+ P_VEL_U, P_VEL_V, P_VEL_DEPTH, P_THERM_L, P_THERM_R, P_THERM_T, P_THERM_B, P_THERM_LT,
+ P_THERM_RT, P_THERM_LB, P_THERM_RB) = range(20)
+NUM_PLANES = 20
+NUM_STATE_PLANES = 9
+STEP_RAIN, STEP_THERMAL, STEP_SMOOTH, STEP_PER_PASS, STEP_KEEP_VELOCITY = 0x1, 0x2, 0x4, 0x8, 0x10
+STEP_DEFAULT = STEP_RAIN | STEP_THERMAL | STEP_SMOOTH
+(PASS_RAIN, PASS_FLUX, PASS_EROSION, PASS_TRANSPORT, PASS_THERMAL_FLUX_0, PASS_THERMAL_TRANSPORT_0,
+ PASS_THERMAL_FLUX_1, PASS_THERMAL_TRANSPORT_1, PASS_SMOOTH) = range(9)
+NUM_PASSES = 9
+
+# Every exported symbol of include/hg.h (checked by the ABI test).
+EXPORTS = [
+    "hg_create", "hg_destroy", "hg_last_error", "hg_size", "hg_set_erosion", "hg_get_erosion",
+    "hg_set_rain", "hg_init_fbm", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
+    "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
+    "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
+]
+
+
+class ErosionSettings(ctypes.Structure):
+    """hg_erosion_settings = Erosion_data (glsl/bindings.glsl:39-60), 96 bytes."""
+    _fields_ = [
+        ("particle_count", ctypes.c_uint32), ("Kc", ctypes.c_float),
+        ("Kalpha", ctypes.c_float * 2), ("Kconv", ctypes.c_float), ("_pad0", ctypes.c_float),
+        ("Ks", ctypes.c_float * 2), ("Kd", ctypes.c_float * 2), ("Ke", ctypes.c_float),
+        ("energy_kept", ctypes.c_float), ("Kspeed", ctypes.c_float * 2), ("G", ctypes.c_float),
+        ("d_t", ctypes.c_float), ("density", ctypes.c_float), ("init_volume", ctypes.c_float),
+        ("friction", ctypes.c_float), ("inertia", ctypes.c_float), ("min_volume", ctypes.c_float),
+        ("min_velocity", ctypes.c_float), ("ttl", ctypes.c_uint32), ("_pad1", ctypes.c_uint32),
+    ]
+
+
+class RainSettings(ctypes.Structure):
+    """hg_rain_settings = Rain_data (glsl/bindings.glsl:62-68), 32 bytes."""
+    _fields_ = [("amount", ctypes.c_float), ("mountain_thresh", ctypes.c_float),
+                ("mountain_multip", ctypes.c_float), ("period", ctypes.c_int32),
+                ("drops", ctypes.c_float), ("_pad", ctypes.c_uint32 * 3)]
+
+
+class FbmParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("octaves", ctypes.c_int32),
+                ("lacunarity", ctypes.c_float), ("gain", ctypes.c_float),
+                ("base_freq", ctypes.c_float), ("height_lo", ctypes.c_float),
+                ("height_hi", ctypes.c_float)]
+
+
+def erosion_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0, slippage=0.1, G=9.81,
+                     dt=0.005, Kconv=0.001, E=1.0) -> ErosionSettings:
+    """Scalar coefficients broadcast to both layers (rock, dirt).  Unnamed
+    fields take the reference grid defaults (src/state.cpp:81-92); the talus
+    angle is given in degrees and stored in radians like the reference UI."""
+    s = ErosionSettings()
+    s.Kc = Kc
+    a = math.radians(talus_deg)
+    s.Kalpha[0] = a; s.Kalpha[1] = a
+    s.Kconv = Kconv
+    s.Ks[0] = Ks; s.Ks[1] = Ks
+    s.Kd[0] = Kd; s.Kd[1] = Kd
+    s.Ke = Ke
+    s.energy_kept = E
+    s.Kspeed[0] = slippage; s.Kspeed[1] = slippage
+    s.G = G
+    s.d_t = dt
+    return s
+
+
+def rain_settings(amount=0.01) -> RainSettings:
+    r = RainSettings()
+    r.amount = amount
+    r.mountain_thresh = 0.55
+    r.mountain_multip = 0.05
+    r.period = 512
+    r.drops = 0.02
+    return r
+
+
+def fbm_params(seed=1, octaves=6, lacunarity=2.0, gain=0.5, base_freq=1.0 / 256, lo=0.0,
+               hi=64.0) -> FbmParams:
+    p = FbmParams()
+    p.seed = seed; p.octaves = octaves; p.lacunarity = lacunarity; p.gain = gain
+    p.base_freq = base_freq; p.height_lo = lo; p.height_hi = hi
+    return p
+
+
+class HGError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hg error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libhydrogen.so; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.hg_create.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.hg_destroy.argtypes = [vp]
+    L.hg_last_error.argtypes = [vp]
+    L.hg_last_error.restype = ctypes.c_char_p
+    L.hg_size.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+    L.hg_set_erosion.argtypes = [vp, ctypes.POINTER(ErosionSettings)]
+    L.hg_get_erosion.argtypes = [vp, ctypes.POINTER(ErosionSettings)]
+    L.hg_set_rain.argtypes = [vp, ctypes.POINTER(RainSettings), ctypes.c_int, ctypes.c_uint64]
+    L.hg_init_fbm.argtypes = [vp, ctypes.POINTER(FbmParams)]
+    L.hg_rain.argtypes = [vp]
+    L.hg_step_grid.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
+    L.hg_run_pass.argtypes = [vp, ctypes.c_int]
+    L.hg_get_step.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.hg_set_step.argtypes = [vp, ctypes.c_uint64]
+    L.hg_read.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
+    L.hg_write.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
+    L.hg_read_plane.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
+    L.hg_write_plane.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
+    L.hg_plane_device_ptr.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.hg_sync.argtypes = [vp]
+    L.hg_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.hg_set_timing.argtypes = [vp, ctypes.c_int]
+    L.hg_kernel_stats.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                  ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]
+    L.hg_reset_timing.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Terrain:
+    """One n x n terrain on one HIP device: the owner of an hg_ctx.
+
+    Method names follow the reference erosion layer (src/erosion.hpp:38-47):
+    dispatch_grid = one or more GRID steps, dispatch_grid_rain = one rain pass.
+    """
+
+    def __init__(self, n: int, device: int = 0, particle_count: int = 0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.hg_create(device, n, particle_count, ctypes.byref(h))
+        if rc != HG_OK:
+            raise HGError(rc, self.lib.hg_last_error(None).decode())
+        self.h = h
+        self.n = n
+
+    # -- lifetime --
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != HG_OK:
+            raise HGError(rc, self.lib.hg_last_error(self.h).decode())
+
+    # -- settings --
+    def set_erosion(self, s: ErosionSettings):
+        self._check(self.lib.hg_set_erosion(self.h, ctypes.byref(s)))
+
+    def get_erosion(self) -> ErosionSettings:
+        s = ErosionSettings()
+        self._check(self.lib.hg_get_erosion(self.h, ctypes.byref(s)))
+        return s
+
+    def set_rain(self, mode: int, amount: float = 0.01, seed: int = 0, settings: RainSettings | None = None):
+        r = settings if settings is not None else rain_settings(amount)
+        self._check(self.lib.hg_set_rain(self.h, ctypes.byref(r), mode, seed))
+
+    def init_fbm(self, p: FbmParams):
+        self._check(self.lib.hg_init_fbm(self.h, ctypes.byref(p)))
+
+    # -- stepping --
+    def dispatch_grid(self, nsteps: int = 1, flags: int = STEP_DEFAULT):
+        self._check(self.lib.hg_step_grid(self.h, nsteps, flags))
+
+    def dispatch_grid_rain(self):
+        self._check(self.lib.hg_rain(self.h))
+
+    def run_pass(self, p: int):
+        self._check(self.lib.hg_run_pass(self.h, p))
+
+    @property
+    def step(self) -> int:
+        v = ctypes.c_uint64()
+        self._check(self.lib.hg_get_step(self.h, ctypes.byref(v)))
+        return v.value
+
+    @step.setter
+    def step(self, v: int):
+        self._check(self.lib.hg_set_step(self.h, v))
+
+    def sync(self):
+        self._check(self.lib.hg_sync(self.h))
+
+    # -- state I/O --
+    def read_plane(self, p: int) -> np.ndarray:
+        a = np.empty((self.n, self.n), dtype=np.float32)
+        self._check(self.lib.hg_read_plane(self.h, p, _fptr(a), a.size))
+        return a
+
+    def write_plane(self, p: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.shape == (self.n, self.n)
+        self._check(self.lib.hg_write_plane(self.h, p, _fptr(a), a.size))
+
+    def read(self, ch: int) -> np.ndarray:
+        a = np.empty((self.n, self.n, 4), dtype=np.float32)
+        self._check(self.lib.hg_read(self.h, ch, _fptr(a), a.size))
+        return a
+
+    def write(self, ch: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.shape == (self.n, self.n, 4)
+        self._check(self.lib.hg_write(self.h, ch, _fptr(a), a.size))
+
+    def read_state(self, nplanes: int = NUM_PLANES) -> np.ndarray:
+        return np.stack([self.read_plane(p) for p in range(nplanes)])
+
+    def write_state(self, st: np.ndarray):
+        for p in range(st.shape[0]):
+            self.write_plane(p, st[p])
+
+    def device_ptr(self, p: int) -> int:
+        v = ctypes.c_void_p()
+        self._check(self.lib.hg_plane_device_ptr(self.h, p, ctypes.byref(v)))
+        return v.value
+
+    def stream(self) -> int:
+        v = ctypes.c_void_p()
+        self._check(self.lib.hg_get_stream(self.h, ctypes.byref(v)))
+        return v.value
+
+    # -- timing --
+    def set_timing(self, on: bool):
+        self._check(self.lib.hg_set_timing(self.h, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        i = 0
+        while True:
+            name = ctypes.c_char_p()
+            n = ctypes.c_uint64()
+            ms = ctypes.c_double()
+            rc = self.lib.hg_kernel_stats(self.h, i, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms))
+            if rc != HG_OK:
+                break
+            out[name.value.decode()] = (n.value, ms.value)
+            i += 1
+        return out
+
+    def reset_timing(self):
+        self._check(self.lib.hg_reset_timing(self.h))

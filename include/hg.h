@@ -1,0 +1,230 @@
+/*
+ * hg.h — C ABI of the MI355X (gfx950) erosion library.
+ *
+ * This is the drop-in boundary for hydro-gen's erosion dispatch layer
+ * (src/erosion.hpp:38-47, src/state.hpp:78-85).  Every entry point takes
+ * plain pointers and sizes; no C++ or torch types cross it.  Status codes are
+ * int: 0 = OK, < 0 = error; hg_last_error() returns the message of the last
+ * failing call on a context.
+ *
+ * Threading contract: one host thread per context.  All device work is
+ * enqueued on the context's HIP stream and is asynchronous until hg_sync()
+ * or a read.
+ */
+#ifndef HG_H
+#define HG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HG_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    HG_OK = 0,
+    HG_ERR_INVALID = -1,   /* bad argument (null pointer, size, enum, non-finite setting) */
+    HG_ERR_HIP = -2,       /* a HIP runtime call failed */
+    HG_ERR_OOM = -3,       /* device allocation failed */
+    HG_ERR_STATE = -4,     /* call not valid in the context's current state */
+    HG_ERR_UNSUPPORTED = -5
+};
+
+typedef struct hg_ctx hg_ctx;
+
+/* ---- settings blocks --------------------------------------------------- */
+
+/* Erosion_data, glsl/bindings.glsl:39-60: same field order and std140 offsets
+ * (96 bytes), so a reference Erosion_settings::data can be copied in as is.
+ * Index [0] of the per-layer vec2 fields is rock, [1] is dirt. */
+typedef struct hg_erosion_settings {
+    uint32_t particle_count; /*  0 */
+    float Kc;                /*  4  sediment capacity */
+    float Kalpha[2];         /*  8  talus angle per layer, radians */
+    float Kconv;             /* 16  rock->dirt sediment conversion */
+    float _pad0;             /* 20 */
+    float Ks[2];             /* 24  dissolving constant per layer */
+    float Kd[2];             /* 32  deposition constant per layer */
+    float Ke;                /* 40  evaporation */
+    float energy_kept;       /* 44  pipe flux damping E */
+    float Kspeed[2];         /* 48  thermal slippage speed per layer */
+    float G;                 /* 56  gravity */
+    float d_t;               /* 60  time step */
+    float density;           /* 64  (particle mode) */
+    float init_volume;       /* 68 */
+    float friction;          /* 72 */
+    float inertia;           /* 76 */
+    float min_volume;        /* 80 */
+    float min_velocity;      /* 84 */
+    uint32_t ttl;            /* 88 */
+    uint32_t _pad1;          /* 92 */
+} hg_erosion_settings;
+
+/* Rain_data, glsl/bindings.glsl:62-68 (std140, 32 bytes). */
+typedef struct hg_rain_settings {
+    float amount;            /*  0 */
+    float mountain_thresh;   /*  4 */
+    float mountain_multip;   /*  8 */
+    int32_t period;          /* 12 */
+    float drops;             /* 16 */
+    uint32_t _pad[3];        /* 20 */
+} hg_rain_settings;
+
+/* Rain modes.  The reference's own rain (glsl/rain.glsl:32-56) is seeded from
+ * the wall clock; these modes are reproducible replacements.
+ *  CONSTANT: water += amount on every cell.
+ *  UNIFORM:  water += amount * u, u = (Philox4x32-10(ctr, key) >> 8) * 2^-24 with
+ *            ctr = {x >> 2, y, step, terrain}, key = {seed lo, seed hi}, output
+ *            word (x & 3).  Resampled every step; bit-identical host/device. */
+enum { HG_RAIN_OFF = 0, HG_RAIN_CONSTANT = 1, HG_RAIN_UNIFORM = 2 };
+
+/* Simplex fBm heightmap initialisation (this project's seeded-permutation
+ * simplex; see hg_init_fbm).  Heights are min-max rescaled to
+ * [height_lo, height_hi] and stored in the rock layer; dirt, water, flux,
+ * sediment and velocity are zeroed. */
+typedef struct hg_fbm_params {
+    uint64_t seed;
+    int32_t octaves;
+    float lacunarity;
+    float gain;
+    float base_freq;         /* cycles per cell of octave 0 */
+    float height_lo;
+    float height_hi;
+} hg_fbm_params;
+
+/* ---- state channels ----------------------------------------------------- */
+
+/* RGBA32F channel views in the reference's texture layout
+ * (src/state.hpp:59-76):
+ *   HEIGHT   = (rock, dirt, water, rock+dirt+water)
+ *   FLUX     = (L, R, T, B)
+ *   VELOCITY = (u, v, d1+d2, 0)
+ *   SEDIMENT = (s_rock, s_dirt, 0, 0)
+ * Row-major, x fastest, y = 0 is the "bottom" row (direction B = -y). */
+enum { HG_CH_HEIGHT = 0, HG_CH_FLUX = 1, HG_CH_VELOCITY = 2, HG_CH_SEDIMENT = 3 };
+
+/* Single FP32 planes (structure-of-arrays storage on the device). */
+enum {
+    HG_P_ROCK = 0, HG_P_DIRT, HG_P_WATER,
+    HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
+    HG_P_SED_ROCK, HG_P_SED_DIRT,
+    HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH,
+    /* thermal outflows of the last thermal flux pass (per-pass path only):
+     * cross (L, R, T, B) then diagonal (LT, RT, LB, RB) */
+    HG_P_THERM_L, HG_P_THERM_R, HG_P_THERM_T, HG_P_THERM_B,
+    HG_P_THERM_LT, HG_P_THERM_RT, HG_P_THERM_LB, HG_P_THERM_RB,
+    HG_NUM_PLANES
+};
+#define HG_NUM_STATE_PLANES 9   /* rock..sed_dirt persist across steps */
+
+/* ---- step flags (hg_step_grid) ------------------------------------------ */
+#define HG_STEP_RAIN          0x1u  /* apply the rain mode before each step */
+#define HG_STEP_THERMAL       0x2u  /* thermal flux+transport for both layers */
+#define HG_STEP_SMOOTH        0x4u  /* extremum smoothing pass */
+#define HG_STEP_PER_PASS      0x8u  /* one kernel per reference pass (unfused) */
+#define HG_STEP_KEEP_VELOCITY 0x10u /* fused path also stores the velocity planes */
+#define HG_STEP_DEFAULT (HG_STEP_RAIN | HG_STEP_THERMAL | HG_STEP_SMOOTH)
+
+/* Single reference passes, for pass-by-pass parity (hg_run_pass). */
+enum {
+    HG_PASS_RAIN = 0,
+    HG_PASS_FLUX = 1,               /* glsl/hydro_flux.glsl */
+    HG_PASS_EROSION = 2,            /* glsl/hydro_erosion.glsl */
+    HG_PASS_TRANSPORT = 3,          /* glsl/sediment_transport.glsl */
+    HG_PASS_THERMAL_FLUX_0 = 4,     /* glsl/thermal_erosion.glsl, t_layer 0 */
+    HG_PASS_THERMAL_TRANSPORT_0 = 5,/* glsl/thermal_transport.glsl, t_layer 0 */
+    HG_PASS_THERMAL_FLUX_1 = 6,
+    HG_PASS_THERMAL_TRANSPORT_1 = 7,
+    HG_PASS_SMOOTH = 8,             /* glsl/smoothing.glsl, grid branch */
+    HG_NUM_PASSES = 9
+};
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces World::gen_textures + Erosion::setup_shaders (src/state.cpp:3-44,
+ * src/erosion.cpp:21-74): allocates the n x n state on HIP device `device`
+ * and a stream.  particle_count > 0 also allocates the droplet buffers.
+ * The settings start at the reference grid defaults (src/state.cpp:81-92). */
+int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out);
+
+/* Replaces World::delete_textures (src/state.cpp:46-55).  NULL is a no-op. */
+int hg_destroy(hg_ctx* ctx);
+
+/* Message of the last failing call on ctx (or of the last failing hg_create
+ * when ctx is NULL).  Never NULL. */
+const char* hg_last_error(const hg_ctx* ctx);
+
+/* Grid side length. */
+int hg_size(const hg_ctx* ctx, uint32_t* n);
+
+/* Replaces Erosion_settings::push_data (src/settings.hpp:61-67): the new
+ * values apply from the next enqueued pass.  Rejects non-finite values,
+ * d_t <= 0 and negative coefficients. */
+int hg_set_erosion(hg_ctx* ctx, const hg_erosion_settings* s);
+int hg_get_erosion(const hg_ctx* ctx, hg_erosion_settings* s);
+
+/* Replaces Rain_settings::push_data; `mode` is one of HG_RAIN_*. */
+int hg_set_rain(hg_ctx* ctx, const hg_rain_settings* s, int mode, uint64_t seed);
+
+/* Config heightmap init (in place of World::gen_heightmap,
+ * src/state.cpp:116-147).  Resets the step counter to 0. */
+int hg_init_fbm(hg_ctx* ctx, const hg_fbm_params* p);
+
+/* ---- stepping ----------------------------------------------------------- */
+
+/* One rain application at the current step counter, as a pass of its own
+ * (replaces Erosion::dispatch_grid_rain, src/erosion.cpp:76-89). */
+int hg_rain(hg_ctx* ctx);
+
+/* nsteps GRID steps (replaces Erosion::dispatch_grid, src/erosion.cpp:158-200)
+ * in the reference pass order: [rain] -> flux -> erosion -> sediment transport
+ * -> thermal layer 0 -> thermal layer 1 -> smoothing.  The step counter
+ * advances by nsteps. */
+int hg_step_grid(hg_ctx* ctx, uint32_t nsteps, uint32_t flags);
+
+/* One reference pass (HG_PASS_*) through the per-pass kernels. */
+int hg_run_pass(hg_ctx* ctx, int pass);
+
+int hg_get_step(const hg_ctx* ctx, uint64_t* step);
+int hg_set_step(hg_ctx* ctx, uint64_t step);
+
+/* ---- state I/O ---------------------------------------------------------- */
+
+/* Channel I/O in the reference RGBA32F layout; n_floats must be 4*n*n.
+ * Writing HEIGHT ignores the .w (total) component; writing VELOCITY ignores .w.
+ * Reads synchronise the stream. */
+int hg_read(hg_ctx* ctx, int channel, float* rgba, size_t n_floats);
+int hg_write(hg_ctx* ctx, int channel, const float* rgba, size_t n_floats);
+
+/* Plane I/O (HG_P_*), n_floats must be n*n. */
+int hg_read_plane(hg_ctx* ctx, int plane, float* dst, size_t n_floats);
+int hg_write_plane(hg_ctx* ctx, int plane, const float* src, size_t n_floats);
+
+/* Device address of the current buffer of a plane (valid until the next
+ * pass swaps buffers). */
+int hg_plane_device_ptr(hg_ctx* ctx, int plane, void** ptr);
+
+/* Block until all work enqueued on the context has finished. */
+int hg_sync(hg_ctx* ctx);
+
+/* The context's hipStream_t, as void*. */
+int hg_get_stream(hg_ctx* ctx, void** stream);
+
+/* ---- per-kernel timing -------------------------------------------------- */
+
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream and its duration is accumulated per kernel name.
+ * hg_kernel_stats() synchronises and reports kernel i (0-based, in first-
+ * launch order); returns HG_ERR_INVALID past the last one. */
+int hg_set_timing(hg_ctx* ctx, int enable);
+int hg_kernel_stats(hg_ctx* ctx, int i, const char** name, uint64_t* launches,
+                    double* total_ms);
+int hg_reset_timing(hg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HG_H */

@@ -1,0 +1,572 @@
+// hg_oracle.hpp — CPU restatement of hydro-gen's erosion hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is the checker the HIP library is compared
+// against; nothing in the product links, loads or calls it.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+//
+// Every pass is a literal restatement of the reference GLSL compute shader it
+// names (file:line of ger0/hydro-gen), templated on the arithmetic type T so
+// the same code runs in double (the FP64 oracle), float (the CPU baseline and
+// ulp-level checks) and a dual-number type (forward-mode derivatives).
+// Each pass is Jacobi: it reads the input state and writes a new one, like
+// the reference's Tex_pair ping-pong (src/shaderprogram.cpp:51-109).
+//
+// Decisions where the reference is undefined or silent (documented, not
+// pinned by running the reference, which cannot run here):
+//  * out-of-range imageLoad/texelFetch reads 0 (GL robust access);
+//  * smoothstep(1e-3, 5e-4, x) uses the textbook formula despite edge0 > edge1;
+//  * the stored "total" channel is recomputed as rock+dirt+water wherever a
+//    pass reads it (the reference stores it after every pass).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Arithmetic helpers.  `val()` strips a dual number to its value so that
+// comparisons (the branches of the reference) follow the primal computation.
+// ---------------------------------------------------------------------------
+inline double val(double x) { return x; }
+inline float val(float x) { return x; }
+
+template <class T> inline T tmax(T a, T b) { return val(a) < val(b) ? b : a; }   // GLSL max(a,b)
+template <class T> inline T tmin(T a, T b) { return val(b) < val(a) ? b : a; }   // GLSL min(a,b)
+template <class T> inline T tclamp(T x, T lo, T hi) { return tmin(tmax(x, lo), hi); }
+
+using std::sqrt;
+using std::atan;
+using std::floor;
+using std::fabs;
+
+// FP32 literal 999999999999.0 used by the reference for out-of-grid heights
+// (glsl/hydro_flux.glsl:45-52, glsl/thermal_erosion.glsl:20-26).
+constexpr float OOB_HEIGHT_F = 999999999999.0f;
+
+// ---------------------------------------------------------------------------
+// Settings: Erosion_data (glsl/bindings.glsl:39-60) in type T.
+// ---------------------------------------------------------------------------
+template <class T> struct Settings {
+    uint32_t particle_count = 0;
+    T Kc, Kalpha[2], Kconv, Ks[2], Kd[2], Ke, E, Kspeed[2], G, dt;
+};
+
+// Per-cell state, structure of arrays, index y*nx + x.  Planes 0..8 persist
+// across steps; vx, vy, vz are the velocity texture; thermal C/D are the
+// thermal outflow textures (glsl/thermal_erosion.glsl:107-108).
+template <class T> struct State {
+    int nx = 0, ny = 0;
+    std::vector<T> r, d, b;            // rock, dirt, water  (heightmap .r .g .b)
+    std::vector<T> fl, fr, ft, fb;     // flux (L, R, T, B)
+    std::vector<T> s0, s1;             // sediment (rock, dirt)
+    std::vector<T> vx, vy, vz;         // velocity (u, v, d1+d2)
+    std::vector<T> tc[4], td[4];       // thermal cross (L,R,T,B) / diagonal (LT,RT,LB,RB)
+    void resize(int x, int y) {
+        nx = x; ny = y;
+        size_t n = size_t(x) * size_t(y);
+        for (auto* p : planes()) p->assign(n, T(0));
+    }
+    std::vector<std::vector<T>*> planes() {
+        std::vector<std::vector<T>*> v = {&r, &d, &b, &fl, &fr, &ft, &fb, &s0, &s1, &vx, &vy, &vz};
+        for (int k = 0; k < 4; ++k) v.push_back(&tc[k]);
+        for (int k = 0; k < 4; ++k) v.push_back(&td[k]);
+        return v;
+    }
+    // Scratch planes for Jacobi outputs; a pass writes into scratch and then
+    // swaps the planes it produced into place.
+    std::vector<T> tmp[8];
+    std::vector<T>& scratch(int k) {
+        tmp[k].resize(size_t(nx) * size_t(ny));
+        return tmp[k];
+    }
+    inline size_t at(int x, int y) const { return size_t(y) * size_t(nx) + size_t(x); }
+    inline bool in(int x, int y) const { return x >= 0 && y >= 0 && x < nx && y < ny; }
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based RNG of
+// the config rain and the fBm permutation shuffle.
+// ---------------------------------------------------------------------------
+struct Philox {
+    static void round(uint32_t c[4], const uint32_t k[2]) {
+        uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+        uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+        uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+        uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+        uint32_t o[4] = {hi1 ^ c[1] ^ k[0], lo1, hi0 ^ c[3] ^ k[1], lo0};
+        std::memcpy(c, o, sizeof(o));
+    }
+    static void gen(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+        uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+        uint32_t k[2] = {key[0], key[1]};
+        for (int r = 0; r < 10; ++r) {
+            if (r) { k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
+            round(c, k);
+        }
+        std::memcpy(out, c, 4 * sizeof(uint32_t));
+    }
+};
+
+// Config rain, uniform mode: u in [0,1) with 24 bits, exact in FP32.
+inline double rain_uniform_u(uint64_t seed, uint64_t step, uint32_t terrain, int x, int y) {
+    uint32_t ctr[4] = {uint32_t(x) >> 2, uint32_t(y), uint32_t(step), terrain};
+    uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    uint32_t o[4];
+    Philox::gen(ctr, key, o);
+    return double(o[x & 3] >> 8) * (1.0 / 16777216.0);
+}
+
+// ---------------------------------------------------------------------------
+// Config heightmap init: seeded-permutation 2-D simplex fBm (this project's
+// own init for the configs' "simplex fBm (seeded permutation)"; the reference
+// init is glsl/heightmap.glsl:52-158 and is a different noise).
+//  * permutation: Fisher-Yates over 0..255 driven by Philox4x32-10 with
+//    key = seed, ctr = {i, 0x5eed, 0, 0}, j = word0 % (i+1), i = 255..1;
+//  * simplex: Perlin's 2-D simplex grid (skew F2 = (sqrt3-1)/2, unskew
+//    G2 = (3-sqrt3)/6), radial kernel (0.5 - r^2)^4, 8 gradients
+//    (+-1,+-1), (+-1,0), (0,+-1) chosen by perm[(i + perm[j]) & 255] & 7,
+//    sum scaled by 70;
+//  * fBm: sum_o gain^o * simplex(x * f_o, y * f_o), f_o = base_freq * lacunarity^o.
+// ---------------------------------------------------------------------------
+inline void fbm_permutation(uint64_t seed, uint8_t perm[256]) {
+    for (int i = 0; i < 256; ++i) perm[i] = uint8_t(i);
+    uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+    for (int i = 255; i >= 1; --i) {
+        uint32_t ctr[4] = {uint32_t(i), 0x5eedu, 0u, 0u}, o[4];
+        Philox::gen(ctr, key, o);
+        int j = int(o[0] % uint32_t(i + 1));
+        std::swap(perm[i], perm[j]);
+    }
+}
+
+template <class T> struct Simplex2 {
+    const uint8_t* perm;
+    static T grad_dot(int h, T x, T y) {
+        switch (h & 7) {
+            case 0: return x + y;
+            case 1: return -x + y;
+            case 2: return x - y;
+            case 3: return -x - y;
+            case 4: return x;
+            case 5: return -x;
+            case 6: return y;
+            default: return -y;
+        }
+    }
+    T corner(int gi, int gj, T x, T y) const {
+        T t = T(0.5) - x * x - y * y;
+        if (!(val(t) > 0)) return T(0);
+        int h = perm[(gi + perm[gj & 255]) & 255];
+        T t2 = t * t;
+        return t2 * t2 * grad_dot(h, x, y);
+    }
+    T operator()(T x, T y) const {
+        const T F2 = T(0.36602540378443864676);  // (sqrt(3)-1)/2
+        const T G2 = T(0.21132486540518711775);  // (3-sqrt(3))/6
+        T s = (x + y) * F2;
+        T fi = floor(x + s), fj = floor(y + s);
+        T t = (fi + fj) * G2;
+        T x0 = x - (fi - t), y0 = y - (fj - t);
+        int i1 = val(x0) > val(y0) ? 1 : 0, j1 = 1 - i1;
+        T x1 = x0 - T(i1) + G2, y1 = y0 - T(j1) + G2;
+        T x2 = x0 - T(1) + T(2) * G2, y2 = y0 - T(1) + T(2) * G2;
+        int i = int(val(fi)) & 255, j = int(val(fj)) & 255;
+        T n = corner(i, j, x0, y0) + corner(i + i1, j + j1, x1, y1) + corner(i + 1, j + 1, x2, y2);
+        return T(70) * n;
+    }
+};
+
+template <class T>
+T fbm_value(const Simplex2<T>& sx, int x, int y, int octaves, T lacunarity, T gain, T base_freq) {
+    T sum = T(0), amp = T(1), f = base_freq;
+    for (int o = 0; o < octaves; ++o) {
+        sum += amp * sx(T(x) * f, T(y) * f);
+        amp *= gain;
+        f *= lacunarity;
+    }
+    return sum;
+}
+
+// Fill `out` (nx*ny) with the min-max rescaled fBm.
+template <class T>
+void fbm_init(std::vector<T>& out, int nx, int ny, uint64_t seed, int octaves, T lacunarity, T gain,
+              T base_freq, T lo, T hi) {
+    uint8_t perm[256];
+    fbm_permutation(seed, perm);
+    Simplex2<T> sx{perm};
+    out.assign(size_t(nx) * ny, T(0));
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x)
+            out[size_t(y) * nx + x] = fbm_value(sx, x, y, octaves, lacunarity, gain, base_freq);
+    T mn = out[0], mx = out[0];
+    for (auto& v : out) { mn = tmin(mn, v); mx = tmax(mx, v); }
+    T span = mx - mn;
+    for (auto& v : out) v = val(span) > 0 ? lo + (hi - lo) * ((v - mn) / span) : lo;
+}
+
+// ---------------------------------------------------------------------------
+// Rain (config modes; the reference's glsl/rain.glsl:32-56 adds `incr` to
+// water and recomputes the total).
+// ---------------------------------------------------------------------------
+enum { RAIN_OFF = 0, RAIN_CONSTANT = 1, RAIN_UNIFORM = 2 };
+
+template <class T>
+void pass_rain(State<T>& s, int mode, T amount, uint64_t seed, uint64_t step, uint32_t terrain = 0) {
+    if (mode == RAIN_OFF) return;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < s.ny; ++y)
+        for (int x = 0; x < s.nx; ++x) {
+            T incr = mode == RAIN_CONSTANT ? amount
+                                           : amount * T(rain_uniform_u(seed, step, terrain, x, y));
+            s.b[s.at(x, y)] += incr;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// hydro_flux: glsl/hydro_flux.glsl:77-166 (helpers :33-47).
+// ---------------------------------------------------------------------------
+template <class T> void pass_flux(State<T>& s, const Settings<T>& k) {
+    const int nx = s.nx, ny = s.ny;
+    const T A = T(1), L = T(1);
+    std::vector<T>&ob = s.scratch(0), &ofl = s.scratch(1), &ofr = s.scratch(2), &oft = s.scratch(3),
+                  &ofb = s.scratch(4), &ovx = s.scratch(5), &ovy = s.scratch(6), &ovz = s.scratch(7);
+    auto wheight = [&](int x, int y) -> T {   // get_wheight, :33-39
+        if (!s.in(x, y)) return T(OOB_HEIGHT_F);
+        size_t i = s.at(x, y);
+        return s.r[i] + s.d[i] + s.b[i];
+    };
+    auto flux = [&](int x, int y, int c) -> T {  // get_flux, :41-47; c = 0..3 = L,R,T,B
+        if (!s.in(x, y)) return T(0);
+        size_t i = s.at(x, y);
+        return c == 0 ? s.fl[i] : c == 1 ? s.fr[i] : c == 2 ? s.ft[i] : s.fb[i];
+    };
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = s.at(x, y);
+            T f[4] = {s.fl[i], s.fr[i], s.ft[i], s.fb[i]};
+            T d1 = s.b[i];
+            T w = s.r[i] + s.d[i] + s.b[i];
+            T dh[4] = {w - wheight(x - 1, y), w - wheight(x + 1, y), w - wheight(x, y + 1),
+                       w - wheight(x, y - 1)};                                     // :88-92
+            T in[4] = {flux(x - 1, y, 1), flux(x + 1, y, 0), flux(x, y + 1, 3),
+                       flux(x, y - 1, 2)};                                          // :94-98
+            for (int c = 0; c < 4; ++c)                                             // :100-107
+                f[c] = tmax(T(0), k.E * f[c] + k.dt * A * (k.G * dh[c]) / L);
+            if (x <= 0) f[0] = T(0); else if (x >= nx - 1) f[1] = T(0);             // :110-119
+            if (y <= 0) f[3] = T(0); else if (y >= ny - 1) f[2] = T(0);
+            T sum_in = in[0] + in[1] + in[2] + in[3];
+            T sum_out = f[0] + f[1] + f[2] + f[3];
+            T K = tmin(T(1), (d1 * L * L) / (sum_out * k.dt));                      // :125
+            if (val(sum_out) == 0) K = T(1);  // b/0 = inf or 0/0 = NaN: GLSL min returns 1
+            for (int c = 0; c < 4; ++c) f[c] *= K;
+            sum_out *= K;
+            T d_volume = k.dt * (sum_in - sum_out);                                 // :133
+            T d2 = tmax(T(0), d1 + (d_volume / (L * L)));
+            ob[i] = d2;
+            T vz = d1 + d2;                                                         // :144
+            ovz[i] = vz;
+            if (val(vz) > 0) {                                                      // :146-161
+                ovx[i] = (flux(x - 1, y, 1) - flux(x, y, 0) + flux(x, y, 1) - flux(x + 1, y, 0)) / (L * vz);
+                ovy[i] = (flux(x, y - 1, 2) - flux(x, y, 3) + flux(x, y, 2) - flux(x, y + 1, 3)) / (L * vz);
+            } else {
+                ovx[i] = T(0);
+                ovy[i] = T(0);
+            }
+            ofl[i] = f[0]; ofr[i] = f[1]; oft[i] = f[2]; ofb[i] = f[3];
+        }
+    std::swap(s.b, ob); std::swap(s.fl, ofl); std::swap(s.fr, ofr); std::swap(s.ft, oft);
+    std::swap(s.fb, ofb); std::swap(s.vx, ovx); std::swap(s.vy, ovy); std::swap(s.vz, ovz);
+}
+
+// ---------------------------------------------------------------------------
+// hydro_erosion: glsl/hydro_erosion.glsl:23-92.
+// ---------------------------------------------------------------------------
+template <class T> T erosion_sin_slope(const State<T>& s, int x, int y) {
+    // get_terr_normal, :23-35; out-of-grid imageLoad reads 0.
+    auto rg = [&](int xx, int yy, int c) -> T {
+        if (!s.in(xx, yy)) return T(0);
+        size_t i = s.at(xx, yy);
+        return c == 0 ? s.r[i] : s.d[i];
+    };
+    const T L = T(1);
+    T dx = rg(x + 1, y, 0) + rg(x + 1, y, 1) - rg(x - 1, y, 0) - rg(x - 1, y, 1);
+    T dz = rg(x, y + 1, 0) + rg(x, y + 1, 1) - rg(x, y - 1, 0) - rg(x, y - 1, 1);
+    // normalize(cross((2L, dx, 0), (0, dz, 2L))) = (2L dx, -4L^2, 2L dz) / |.|
+    T cx = dx * (T(2) * L) - T(0) * dz;
+    T cy = T(0) * T(0) - (T(2) * L) * (T(2) * L);
+    T cz = (T(2) * L) * dz - dx * T(0);
+    T len = sqrt(cx * cx + cy * cy + cz * cz);
+    T ny = cy / len;
+    return fabs(sqrt(T(1) - ny * ny));                                             // :57
+}
+
+template <class T> void erosion_cell(T& r, T& d, T& s0, T& s1, T vx, T vy, T vz, T sin_a,
+                                     const Settings<T>& k) {
+    // :45-52
+    T dd = vz;
+    T ero_vel = sqrt(vx * vx + vy * vy);
+    if (val(dd) < T(1e-3f)) {
+        dd = tmax(T(5e-4f), dd);
+        T t = tclamp((dd - T(1e-3f)) / (T(5e-4f) - T(1e-3f)), T(0), T(1));
+        T sm = t * t * (T(3) - T(2) * t);
+        ero_vel = ero_vel * (T(1) - sm) + T(0) * sm;
+    }
+    T cap = T(0);
+    T* h[2] = {&r, &d};
+    T* sd[2] = {&s0, &s1};
+    for (int i = 1; i >= 0; --i) {                                                  // :58-83
+        T Kls = k.dt * k.Ks[i];
+        T Kld = k.dt * k.Kd[i];
+        T c = tmax(T(0), k.Kc * tmax(T(0.02f), sin_a) * ero_vel - cap);
+        if (val(c) > val(*sd[i])) {
+            T old = *h[i];
+            T delta = Kls * (c - *sd[i]);
+            *h[i] -= delta;
+            *sd[i] += delta;
+            if (val(*h[i]) < 0) {
+                *sd[i] += *h[i];
+                *h[i] = T(0);
+                cap += old;
+            } else {
+                break;
+            }
+        } else {
+            T delta = Kld * (*sd[i] - c);
+            *h[i] += delta;
+            *sd[i] -= delta;
+        }
+    }
+    T conv = s0 * k.Kconv * k.dt;                                                   // :84-88
+    s1 += conv;
+    s0 -= conv;
+}
+
+template <class T> void pass_erosion(State<T>& s, const Settings<T>& k) {
+    std::vector<T>&orr = s.scratch(0), &od = s.scratch(1), &os0 = s.scratch(2), &os1 = s.scratch(3);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < s.ny; ++y)
+        for (int x = 0; x < s.nx; ++x) {
+            size_t i = s.at(x, y);
+            T sin_a = erosion_sin_slope(s, x, y);
+            T r = s.r[i], d = s.d[i], s0 = s.s0[i], s1 = s.s1[i];
+            erosion_cell(r, d, s0, s1, s.vx[i], s.vy[i], s.vz[i], sin_a, k);
+            orr[i] = r; od[i] = d; os0[i] = s0; os1[i] = s1;
+        }
+    std::swap(s.r, orr); std::swap(s.d, od); std::swap(s.s0, os0); std::swap(s.s1, os1);
+}
+
+// ---------------------------------------------------------------------------
+// sediment_transport: glsl/sediment_transport.glsl:66-93 with img_bilinear
+// (glsl/img_interpolation.glsl:3-22) and get_lerp_sed clamping (:26-30).
+// ---------------------------------------------------------------------------
+template <class T> T mixv(T a, T b, T t) { return a * (T(1) - t) + b * t; }
+
+template <class T> void bilinear2(const State<T>& s, const std::vector<T>& p0, const std::vector<T>& p1,
+                                  T bx, T by, T& o0, T& o1) {
+    int ix = int(val(bx)), iy = int(val(by));          // ivec2(sample_pos): truncation
+    T fx = bx - floor(bx), fy = by - floor(by);         // fract
+    auto fetch = [&](const std::vector<T>& p, int x, int y) -> T {
+        return s.in(x, y) ? p[s.at(x, y)] : T(0);        // robust texelFetch reads 0
+    };
+    for (int c = 0; c < 2; ++c) {
+        const std::vector<T>& p = c == 0 ? p0 : p1;
+        T v1 = mixv(fetch(p, ix, iy), fetch(p, ix + 1, iy), fx);
+        T v2 = mixv(fetch(p, ix, iy + 1), fetch(p, ix + 1, iy + 1), fx);
+        (c == 0 ? o0 : o1) = mixv(v1, v2, fy);
+    }
+}
+
+template <class T> void pass_transport(State<T>& s, const Settings<T>& k) {
+    std::vector<T>&os0 = s.scratch(0), &os1 = s.scratch(1), &ob = s.scratch(2);
+    const T xmax = T(s.nx - 1), ymax = T(s.ny - 1);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < s.ny; ++y)
+        for (int x = 0; x < s.nx; ++x) {
+            size_t i = s.at(x, y);
+            T bx = T(x) - s.vx[i] * k.dt;                                           // :70
+            T by = T(y) - s.vy[i] * k.dt;
+            bx = tclamp(bx, T(0), xmax);                                            // :26-30
+            by = tclamp(by, T(0), ymax);
+            bilinear2(s, s.s0, s.s1, bx, by, os0[i], os1[i]);
+            ob[i] = s.b[i] * (T(1) - k.Ke * k.dt);                                  // :75
+        }
+    std::swap(s.s0, os0); std::swap(s.s1, os1); std::swap(s.b, ob);
+}
+
+// ---------------------------------------------------------------------------
+// thermal_erosion: glsl/thermal_erosion.glsl:20-115 (layer `layer`).
+// Directions in the reference's order: cross L,R,T,B; diagonal LT,RT,LB,RB.
+// ---------------------------------------------------------------------------
+static const int TDX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+static const int TDY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+
+template <class T> void pass_thermal_flux(State<T>& s, const Settings<T>& k, int layer) {
+    std::vector<T>* o[8];
+    for (int kk = 0; kk < 8; ++kk) o[kk] = &s.scratch(kk);
+    const T Lc = T(1);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < s.ny; ++y)
+        for (int x = 0; x < s.nx; ++x) {
+            size_t i = s.at(x, y);
+            T t[2] = {s.r[i], s.d[i]};
+            T dh[8];
+            for (int kk = 0; kk < 8; ++kk) dh[kk] = T(0);
+            for (int l = 0; l <= layer; ++l)                                        // :32-44
+                for (int kk = 0; kk < 8; ++kk) {
+                    int xx = x + TDX[kk], yy = y + TDY[kk];
+                    T nb = s.in(xx, yy) ? (l == 0 ? s.r[s.at(xx, yy)] : s.d[s.at(xx, yy)])
+                                        : T(OOB_HEIGHT_F);
+                    dh[kk] += t[l] - nb;
+                }
+            T H = T(0);                                                             // :46-58
+            for (int kk = 0; kk < 8; ++kk)
+                if (val(dh[kk]) > val(H)) H = dh[kk];
+            H = tmin(t[layer], H);
+            bool mark[8];
+            T bk = T(0), sharp = T(1);
+            for (int kk = 0; kk < 8; ++kk) {                                        // :63-91
+                mark[kk] = false;
+                T bb = dh[kk];
+                if (!(val(bb) > 0)) continue;
+                T dist = Lc;
+                if (kk >= 4) dist *= T(sqrtf(2.0f));
+                T alph = atan(bb / dist);
+                T Ka = k.Kalpha[layer];
+                if (val(alph) > val(Ka)) {
+                    T newsh = T(1) + alph - Ka;
+                    if (val(newsh) > val(sharp)) sharp = newsh;
+                    bk += bb;
+                    mark[kk] = true;
+                }
+            }
+            sharp *= sharp * sharp;                                                 // :92
+            T S = k.dt * k.Kspeed[layer] * sharp * Lc * H / T(2);                   // :95
+            for (int kk = 0; kk < 8; ++kk) {                                        // :98-106
+                (*o[kk])[i] = mark[kk] ? S * dh[kk] / bk : T(0);
+            }
+        }
+    for (int kk = 0; kk < 4; ++kk) std::swap(s.tc[kk], *o[kk]);
+    for (int kk = 0; kk < 4; ++kk) std::swap(s.td[kk], *o[kk + 4]);
+}
+
+// thermal_transport: glsl/thermal_transport.glsl:146-180 (gather_inflow :31-52).
+template <class T> void pass_thermal_transport(State<T>& s, int layer) {
+    std::vector<T>& oh = s.scratch(0);
+    const std::vector<T>& h = layer == 0 ? s.r : s.d;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < s.ny; ++y)
+        for (int x = 0; x < s.nx; ++x) {
+            size_t i = s.at(x, y);
+            auto c = [&](int xx, int yy, int comp) -> T {
+                return s.in(xx, yy) ? s.tc[comp][s.at(xx, yy)] : T(0);
+            };
+            auto dg = [&](int xx, int yy, int comp) -> T {
+                return s.in(xx, yy) ? s.td[comp][s.at(xx, yy)] : T(0);
+            };
+            T in = T(0);
+            in += c(x - 1, y, 1);      // from the left cell, its R outflow
+            in += c(x + 1, y, 0);      // from the right cell, its L
+            in += c(x, y + 1, 3);      // from the top cell, its B
+            in += c(x, y - 1, 2);      // from the bottom cell, its T
+            in += dg(x - 1, y + 1, 3); // LT cell, its RB
+            in += dg(x + 1, y + 1, 2); // RT cell, its LB
+            in += dg(x - 1, y - 1, 1); // LB cell, its RT
+            in += dg(x + 1, y - 1, 0); // RB cell, its LT
+            T sum = T(0);
+            for (int kk = 0; kk < 4; ++kk) sum -= s.tc[kk][i];
+            for (int kk = 0; kk < 4; ++kk) sum -= s.td[kk][i];
+            sum += in;
+            oh[i] = h[i] + sum;
+        }
+    std::swap(layer == 0 ? s.r : s.d, oh);
+}
+
+// ---------------------------------------------------------------------------
+// smoothing, grid branch (particle_count == 0): glsl/smoothing.glsl:22-74,97-103.
+// ---------------------------------------------------------------------------
+template <class T> void pass_smooth(State<T>& s, const Settings<T>& k) {
+    std::vector<T>&orr = s.scratch(0), &od = s.scratch(1);
+    const int nx = s.nx, ny = s.ny;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = s.at(x, y);
+            if (x == 0 || y == 0 || x == nx - 1 || y == ny - 1) {                   // :29-34
+                orr[i] = s.r[i]; od[i] = s.d[i];
+                continue;
+            }
+            T tr = s.r[i], tg = s.d[i];
+            size_t il = s.at(x - 1, y), ir = s.at(x + 1, y), it = s.at(x, y + 1), ib = s.at(x, y - 1);
+            T lr = s.r[il], lg = s.d[il], rr = s.r[ir], rg = s.d[ir];
+            T t_r = s.r[it], t_g = s.d[it], br = s.r[ib], bg = s.d[ib];
+            T dlr = tr - lr, dlg = tg - lg; dlg += dlr;                             // :40-48
+            T drr = tr - rr, drg = tg - rg; drg += drr;
+            T dtr = tr - t_r, dtg = tg - t_g; dtg += dtr;
+            T dbr = tr - br, dbg = tg - bg; dbg += dbr;
+            T g_hdiff = fabs((dlg + drg + dtg + dbg) / T(4));                       // :50-55
+            T r_hdiff = fabs((dlr + drr + dtr + dbr) / T(4));
+            T xcr = dlr * drr, xcg = dlg * drg;                                     // :60-61
+            T ycr = dtr * dbr, ycg = dtg * dbg;
+            T nr = tr, ng = tg;
+            if (((val(-dlr) > val(r_hdiff) || val(-drr) > val(r_hdiff)) && val(xcr) > 0) ||
+                ((val(-dtr) > val(r_hdiff) || val(-dbr) > val(r_hdiff)) && val(ycr) > 0))
+                nr = (tr + lr + rr + t_r + br) / T(5);                              // :63-67
+            if (((val(-dlg) > val(g_hdiff) || val(-drg) > val(g_hdiff)) && val(xcg) > 0) ||
+                ((val(-dtg) > val(g_hdiff) || val(-dbg) > val(g_hdiff)) && val(ycg) > 0))
+                ng = (tg + lg + rg + t_g + bg) / T(5);                              // :69-73
+            T m = tclamp(k.Kspeed[1] * k.dt, T(0), T(1));                           // :75
+            orr[i] = m * nr + (T(1) - m) * s.r[i];                                  // :100
+            od[i] = m * ng + (T(1) - m) * s.d[i];
+        }
+    std::swap(s.r, orr); std::swap(s.d, od);
+}
+
+// ---------------------------------------------------------------------------
+// One GRID step: Erosion::dispatch_grid (src/erosion.cpp:158-200) with
+// run_thermal_erosion (:103-121), preceded by the config rain.
+// ---------------------------------------------------------------------------
+enum { PASS_RAIN = 0, PASS_FLUX, PASS_EROSION, PASS_TRANSPORT, PASS_TFLUX0, PASS_TTRANS0, PASS_TFLUX1,
+       PASS_TTRANS1, PASS_SMOOTH };
+enum { STEP_RAIN = 1, STEP_THERMAL = 2, STEP_SMOOTH = 4 };
+
+struct RainCfg {
+    int mode = RAIN_OFF;
+    double amount = 0;
+    uint64_t seed = 0;
+    uint32_t terrain = 0;
+};
+
+template <class T>
+void run_pass(State<T>& s, const Settings<T>& k, const RainCfg& rc, uint64_t step, int pass) {
+    switch (pass) {
+        case PASS_RAIN: pass_rain(s, rc.mode, T(rc.amount), rc.seed, step, rc.terrain); break;
+        case PASS_FLUX: pass_flux(s, k); break;
+        case PASS_EROSION: pass_erosion(s, k); break;
+        case PASS_TRANSPORT: pass_transport(s, k); break;
+        case PASS_TFLUX0: pass_thermal_flux(s, k, 0); break;
+        case PASS_TTRANS0: pass_thermal_transport(s, 0); break;
+        case PASS_TFLUX1: pass_thermal_flux(s, k, 1); break;
+        case PASS_TTRANS1: pass_thermal_transport(s, 1); break;
+        case PASS_SMOOTH: pass_smooth(s, k); break;
+        default: break;
+    }
+}
+
+template <class T>
+void grid_step(State<T>& s, const Settings<T>& k, const RainCfg& rc, uint64_t step, unsigned flags) {
+    if (flags & STEP_RAIN) run_pass(s, k, rc, step, PASS_RAIN);
+    run_pass(s, k, rc, step, PASS_FLUX);
+    run_pass(s, k, rc, step, PASS_EROSION);
+    run_pass(s, k, rc, step, PASS_TRANSPORT);
+    if (flags & STEP_THERMAL)
+        for (int p = PASS_TFLUX0; p <= PASS_TTRANS1; ++p) run_pass(s, k, rc, step, p);
+    if (flags & STEP_SMOOTH) run_pass(s, k, rc, step, PASS_SMOOTH);
+}
+
+}  // namespace orc

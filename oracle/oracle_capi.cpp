@@ -1,0 +1,142 @@
+// oracle_capi.cpp — extern "C" wrappers of the CPU oracle for ctypes.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg, never by the product library.
+//
+// State is passed as one contiguous array of ORC_NUM_PLANES planes of n = nx*ny
+// values each, in the order of hg.h's HG_P_* enum:
+//   rock, dirt, water, flux L/R/T/B, sed rock, sed dirt, vel u/v/depth,
+//   thermal L/R/T/B, thermal LT/RT/LB/RB.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include "hg_oracle.hpp"
+#include "../include/hg.h"
+
+static_assert(sizeof(hg_erosion_settings) == 96, "Erosion_data is 96 bytes in std140");
+
+namespace {
+constexpr int ORC_NUM_PLANES = HG_NUM_PLANES;
+
+template <class T> orc::Settings<T> to_settings(const hg_erosion_settings* s) {
+    orc::Settings<T> k;
+    k.particle_count = s->particle_count;
+    k.Kc = T(s->Kc);
+    k.Kalpha[0] = T(s->Kalpha[0]); k.Kalpha[1] = T(s->Kalpha[1]);
+    k.Kconv = T(s->Kconv);
+    k.Ks[0] = T(s->Ks[0]); k.Ks[1] = T(s->Ks[1]);
+    k.Kd[0] = T(s->Kd[0]); k.Kd[1] = T(s->Kd[1]);
+    k.Ke = T(s->Ke);
+    k.E = T(s->energy_kept);
+    k.Kspeed[0] = T(s->Kspeed[0]); k.Kspeed[1] = T(s->Kspeed[1]);
+    k.G = T(s->G);
+    k.dt = T(s->d_t);
+    return k;
+}
+
+template <class T> void load(orc::State<T>& st, const T* planes, int nx, int ny) {
+    st.resize(nx, ny);
+    size_t n = size_t(nx) * ny;
+    auto ps = st.planes();
+    for (int p = 0; p < ORC_NUM_PLANES; ++p) std::memcpy(ps[p]->data(), planes + p * n, n * sizeof(T));
+}
+
+template <class T> void store(orc::State<T>& st, T* planes) {
+    size_t n = size_t(st.nx) * st.ny;
+    auto ps = st.planes();
+    for (int p = 0; p < ORC_NUM_PLANES; ++p) std::memcpy(planes + p * n, ps[p]->data(), n * sizeof(T));
+}
+
+template <class T>
+int run(T* planes, int nx, int ny, const hg_erosion_settings* s, int rain_mode, double rain_amount,
+        uint64_t seed, uint32_t terrain, uint64_t step0, int nsteps, unsigned flags, int single_pass) {
+    if (!planes || !s || nx < 2 || ny < 2) return -1;
+    orc::State<T> st;
+    load(st, planes, nx, ny);
+    orc::Settings<T> k = to_settings<T>(s);
+    orc::RainCfg rc;
+    rc.mode = rain_mode;
+    rc.amount = rain_amount;
+    rc.seed = seed;
+    rc.terrain = terrain;
+    if (single_pass >= 0) {
+        orc::run_pass(st, k, rc, step0, single_pass);
+    } else {
+        for (int i = 0; i < nsteps; ++i) orc::grid_step(st, k, rc, step0 + uint64_t(i), flags);
+    }
+    store(st, planes);
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int orc_num_planes(void) { return ORC_NUM_PLANES; }
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    orc::Philox::gen(ctr, key, out);
+}
+
+double orc_rain_u(uint64_t seed, uint64_t step, uint32_t terrain, int x, int y) {
+    return orc::rain_uniform_u(seed, step, terrain, x, y);
+}
+
+void orc_fbm_permutation(uint64_t seed, uint8_t perm[256]) { orc::fbm_permutation(seed, perm); }
+
+double orc_simplex2_f64(uint64_t seed, double x, double y) {
+    uint8_t perm[256];
+    orc::fbm_permutation(seed, perm);
+    orc::Simplex2<double> sx{perm};
+    return sx(x, y);
+}
+
+int orc_fbm_init_f64(double* out, int nx, int ny, const hg_fbm_params* p) {
+    if (!out || !p || nx < 1 || ny < 1) return -1;
+    std::vector<double> v;
+    orc::fbm_init<double>(v, nx, ny, p->seed, p->octaves, double(p->lacunarity), double(p->gain),
+                          double(p->base_freq), double(p->height_lo), double(p->height_hi));
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return 0;
+}
+
+int orc_fbm_init_f32(float* out, int nx, int ny, const hg_fbm_params* p) {
+    if (!out || !p || nx < 1 || ny < 1) return -1;
+    std::vector<float> v;
+    orc::fbm_init<float>(v, nx, ny, p->seed, p->octaves, p->lacunarity, p->gain, p->base_freq,
+                         p->height_lo, p->height_hi);
+    std::memcpy(out, v.data(), v.size() * sizeof(float));
+    return 0;
+}
+
+// nsteps GRID steps (flags: 1 rain, 2 thermal, 4 smoothing), step counter
+// starting at step0.  rain_amount is the FP32 setting value.
+int orc_run_f64(double* planes, int nx, int ny, const hg_erosion_settings* s, int rain_mode,
+                float rain_amount, uint64_t seed, uint32_t terrain, uint64_t step0, int nsteps,
+                unsigned flags) {
+    return run<double>(planes, nx, ny, s, rain_mode, double(rain_amount), seed, terrain, step0, nsteps,
+                       flags, -1);
+}
+
+int orc_run_f32(float* planes, int nx, int ny, const hg_erosion_settings* s, int rain_mode,
+                float rain_amount, uint64_t seed, uint32_t terrain, uint64_t step0, int nsteps,
+                unsigned flags) {
+    return run<float>(planes, nx, ny, s, rain_mode, double(rain_amount), seed, terrain, step0, nsteps,
+                      flags, -1);
+}
+
+// One reference pass (HG_PASS_* numbering).
+int orc_pass_f64(double* planes, int nx, int ny, const hg_erosion_settings* s, int rain_mode,
+                 float rain_amount, uint64_t seed, uint32_t terrain, uint64_t step, int pass) {
+    if (pass < 0 || pass >= HG_NUM_PASSES) return -1;
+    return run<double>(planes, nx, ny, s, rain_mode, double(rain_amount), seed, terrain, step, 0, 0,
+                       pass);
+}
+
+int orc_pass_f32(float* planes, int nx, int ny, const hg_erosion_settings* s, int rain_mode,
+                 float rain_amount, uint64_t seed, uint32_t terrain, uint64_t step, int pass) {
+    if (pass < 0 || pass >= HG_NUM_PASSES) return -1;
+    return run<float>(planes, nx, ny, s, rain_mode, double(rain_amount), seed, terrain, step, 0, 0,
+                      pass);
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg.  The product library never imports it.
+
+State is a float64 (or float32) numpy array of shape (NUM_PLANES, ny, nx) in
+the HG_P_* plane order of include/hg.h.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+NUM_PLANES = 20
+ROCK, DIRT, WATER, FL, FR, FT, FB, S0, S1, VX, VY, VZ = range(12)
+THERM = list(range(12, 20))
+PASS_RAIN, PASS_FLUX, PASS_EROSION, PASS_TRANSPORT, PASS_TFLUX0, PASS_TTRANS0, PASS_TFLUX1, \
+    PASS_TTRANS1, PASS_SMOOTH = range(9)
+STEP_RAIN, STEP_THERMAL, STEP_SMOOTH = 1, 2, 4
+STEP_DEFAULT = 7
+RAIN_OFF, RAIN_CONSTANT, RAIN_UNIFORM = 0, 1, 2
+
+
+class ErosionSettings(ctypes.Structure):
+    """Erosion_data (glsl/bindings.glsl:39-60), std140 layout, 96 bytes."""
+    _fields_ = [
+        ("particle_count", ctypes.c_uint32), ("Kc", ctypes.c_float),
+        ("Kalpha", ctypes.c_float * 2), ("Kconv", ctypes.c_float), ("_pad0", ctypes.c_float),
+        ("Ks", ctypes.c_float * 2), ("Kd", ctypes.c_float * 2), ("Ke", ctypes.c_float),
+        ("energy_kept", ctypes.c_float), ("Kspeed", ctypes.c_float * 2), ("G", ctypes.c_float),
+        ("d_t", ctypes.c_float), ("density", ctypes.c_float), ("init_volume", ctypes.c_float),
+        ("friction", ctypes.c_float), ("inertia", ctypes.c_float), ("min_volume", ctypes.c_float),
+        ("min_velocity", ctypes.c_float), ("ttl", ctypes.c_uint32), ("_pad1", ctypes.c_uint32),
+    ]
+
+
+class FbmParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("octaves", ctypes.c_int32),
+                ("lacunarity", ctypes.c_float), ("gain", ctypes.c_float),
+                ("base_freq", ctypes.c_float), ("height_lo", ctypes.c_float),
+                ("height_hi", ctypes.c_float)]
+
+
+def config_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0, slippage=0.1,
+                    G=9.81, dt=0.005, Kconv=0.001, E=1.0) -> ErosionSettings:
+    """The configs' scalar coefficients broadcast to both layers; the unnamed
+    fields take the reference grid defaults (src/state.cpp:81-92)."""
+    s = ErosionSettings()
+    s.particle_count = 0
+    s.Kc = Kc
+    a = math.radians(talus_deg)
+    s.Kalpha[0] = a; s.Kalpha[1] = a
+    s.Kconv = Kconv
+    s.Ks[0] = Ks; s.Ks[1] = Ks
+    s.Kd[0] = Kd; s.Kd[1] = Kd
+    s.Ke = Ke
+    s.energy_kept = E
+    s.Kspeed[0] = slippage; s.Kspeed[1] = slippage
+    s.G = G
+    s.d_t = dt
+    return s
+
+
+def reference_grid_defaults() -> ErosionSettings:
+    """src/state.cpp:81-92."""
+    s = ErosionSettings()
+    s.Kc = 0.2
+    s.Kalpha[0] = 1.3; s.Kalpha[1] = 0.6
+    s.Kconv = 0.001
+    s.Ks[0] = 0.03; s.Ks[1] = 0.09
+    s.Kd[0] = 0.01; s.Kd[1] = 0.03
+    s.Ke = 0.03
+    s.energy_kept = 1.0
+    s.Kspeed[0] = 0.5; s.Kspeed[1] = 2.0
+    s.G = 1.0
+    s.d_t = 0.001
+    return s
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_rain_u.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                                 ctypes.c_int]
+        L.orc_rain_u.restype = ctypes.c_double
+        L.orc_fbm_permutation.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8)]
+        L.orc_simplex2_f64.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
+        L.orc_simplex2_f64.restype = ctypes.c_double
+        for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            p = ctypes.POINTER(t)
+            getattr(L, f"orc_fbm_init_{name}").argtypes = [p, ctypes.c_int, ctypes.c_int,
+                                                           ctypes.POINTER(FbmParams)]
+            getattr(L, f"orc_run_{name}").argtypes = [
+                p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), ctypes.c_int,
+                ctypes.c_float, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                ctypes.c_uint]
+            getattr(L, f"orc_pass_{name}").argtypes = [
+                p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), ctypes.c_int,
+                ctypes.c_float, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    t = ctypes.c_double if a.dtype == np.float64 else ctypes.c_float
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def rain_u(seed, step, terrain, x, y) -> float:
+    return lib().orc_rain_u(seed, step, terrain, x, y)
+
+
+def fbm_permutation(seed: int) -> np.ndarray:
+    p = (ctypes.c_uint8 * 256)()
+    lib().orc_fbm_permutation(seed, p)
+    return np.frombuffer(bytes(p), dtype=np.uint8).copy()
+
+
+def simplex2(seed: int, x: float, y: float) -> float:
+    return lib().orc_simplex2_f64(seed, x, y)
+
+
+def fbm_params(seed=1, octaves=6, lacunarity=2.0, gain=0.5, base_freq=1.0 / 256, lo=0.0, hi=64.0):
+    p = FbmParams()
+    p.seed = seed; p.octaves = octaves; p.lacunarity = lacunarity; p.gain = gain
+    p.base_freq = base_freq; p.height_lo = lo; p.height_hi = hi
+    return p
+
+
+def fbm_init(n: int, params: FbmParams, dtype=np.float64, ny: int | None = None) -> np.ndarray:
+    ny = n if ny is None else ny
+    out = np.zeros((ny, n), dtype=dtype)
+    fn = lib().orc_fbm_init_f64 if dtype == np.float64 else lib().orc_fbm_init_f32
+    if fn(_ptr(out), n, ny, ctypes.byref(params)) != 0:
+        raise ValueError("orc_fbm_init failed")
+    return out
+
+
+def new_state(n: int, dtype=np.float64, ny: int | None = None) -> np.ndarray:
+    ny = n if ny is None else ny
+    return np.zeros((NUM_PLANES, ny, n), dtype=dtype)
+
+
+def run(state: np.ndarray, settings: ErosionSettings, nsteps: int, flags: int = STEP_DEFAULT,
+        rain_mode: int = RAIN_OFF, rain_amount: float = 0.0, seed: int = 0, terrain: int = 0,
+        step0: int = 0) -> np.ndarray:
+    """nsteps GRID steps in place; returns state."""
+    _, ny, nx = state.shape
+    fn = lib().orc_run_f64 if state.dtype == np.float64 else lib().orc_run_f32
+    rc = fn(_ptr(state), nx, ny, ctypes.byref(settings), rain_mode, rain_amount, seed, terrain,
+            step0, nsteps, flags)
+    if rc != 0:
+        raise ValueError("orc_run failed")
+    return state
+
+
+def run_pass(state: np.ndarray, settings: ErosionSettings, pass_id: int, rain_mode: int = RAIN_OFF,
+             rain_amount: float = 0.0, seed: int = 0, terrain: int = 0, step: int = 0) -> np.ndarray:
+    _, ny, nx = state.shape
+    fn = lib().orc_pass_f64 if state.dtype == np.float64 else lib().orc_pass_f32
+    rc = fn(_ptr(state), nx, ny, ctypes.byref(settings), rain_mode, rain_amount, seed, terrain,
+            step, pass_id)
+    if rc != 0:
+        raise ValueError("orc_pass failed")
+    return state

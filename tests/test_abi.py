@@ -1,0 +1,62 @@
+"""The C-ABI library: it loads without a GPU, exports every symbol that
+include/hg.h declares, and its settings structs have the reference's std140
+layout (glsl/bindings.glsl:39-68).  No compute calls (no GPU here)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import hydrogen as hg
+import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "hg.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(hg.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(hg.EXPORTS) == names, "binding list out of date with include/hg.h"
+
+
+def test_erosion_settings_std140_layout():
+    S = hg.ErosionSettings
+    assert ctypes.sizeof(S) == 96
+    expect = {"particle_count": 0, "Kc": 4, "Kalpha": 8, "Kconv": 16, "Ks": 24, "Kd": 32, "Ke": 40,
+              "energy_kept": 44, "Kspeed": 48, "G": 56, "d_t": 60, "density": 64, "init_volume": 68,
+              "friction": 72, "inertia": 76, "min_volume": 80, "min_velocity": 84, "ttl": 88}
+    for name, off in expect.items():
+        assert getattr(S, name).offset == off, name
+    # the oracle's independent copy of the struct agrees
+    for name, off in expect.items():
+        assert getattr(O.ErosionSettings, name).offset == off, name
+    assert ctypes.sizeof(O.ErosionSettings) == 96
+
+
+def test_rain_settings_std140_layout():
+    R = hg.RainSettings
+    assert ctypes.sizeof(R) == 32
+    assert (R.amount.offset, R.mountain_thresh.offset, R.mountain_multip.offset, R.period.offset,
+            R.drops.offset) == (0, 4, 8, 12, 16)
+
+
+def test_create_without_device_fails_cleanly():
+    # In a container without a GPU hg_create must fail with a message, not crash.
+    import torch
+    if torch.cuda.is_available():
+        return
+    try:
+        hg.Terrain(64)
+    except hg.HGError as e:
+        assert e.code in (hg.HG_ERR_HIP, hg.HG_ERR_INVALID)
+    else:
+        raise AssertionError("hg_create succeeded without a device")
