@@ -1,0 +1,149 @@
+"""Benchmark: the pipe+thermal GRID step on one MI355X.
+
+Workload (the metric's headline configuration): one 4096 x 4096 FP32 terrain,
+seeded simplex fBm (6 octaves, lacunarity 2, gain 0.5, base frequency 1/1024)
+rescaled to heights [0, 512]; rain U(0, 0.02) per cell per step from
+Philox4x32-10; coefficients g 9.81, dt 0.005, A = L = 1, Kc 0.05,
+Ks = Kd = 0.02, Ke 0.01, talus 35 deg, slippage 0.1; thermal erosion and
+smoothing every step.  W warm-up steps, then K timed steps (default: steps
+100-2000 of the 2000-step run).
+
+Prints one JSON line.  `value` is cell-steps/s over the timed steps with the
+state resident in HBM.  `roofline` is the dominant kernel's algorithmic HBM
+bytes per launch over its average launch time, measured with HIP events on
+the library's stream inside the timed region.  `cpu_baseline` is the FP32
+oracle (a CPU port of the same passes, OpenMP over rows) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hydro-gen_amd"))
+
+import numpy as np  # noqa: E402
+
+import hydrogen as hg  # noqa: E402
+
+METRIC = "cell-steps/s at 4096² + % HBM roofline; batched and fwd+adjoint vs plain"
+HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
+# Algorithmic HBM bytes per cell for each fused kernel (each persistent plane
+# read once and written once):
+#   fused_hydro  : rock, dirt, water, flux L/R/T/B, sediment rock/dirt -> 9 x (4+4) = 72 B
+#   fused_thermal: rock, dirt -> 2 x (4+4) = 16 B
+ALGO_BYTES = {"fused_hydro": 72, "fused_thermal": 16}
+STEP_ALGO_BYTES = 72         # minimum for a step that materialises the 9 planes once
+
+
+def build_terrain(n: int, seed: int = 1) -> hg.Terrain:
+    t = hg.Terrain(n, device=0)
+    t.set_erosion(hg.erosion_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0,
+                                      slippage=0.1, G=9.81, dt=0.005))
+    t.set_rain(hg.RAIN_UNIFORM, amount=0.02, seed=seed)
+    t.init_fbm(hg.fbm_params(seed=seed, octaves=6, lacunarity=2.0, gain=0.5,
+                             base_freq=1.0 / 1024, lo=0.0, hi=512.0))
+    return t
+
+
+def cpu_baseline(t: hg.Terrain, steps: int) -> dict:
+    """FP32 oracle (CPU port of the same passes) on `steps` steps of the same
+    4096^2 state; OpenMP threads from OMP_NUM_THREADS."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    state = np.zeros((O.NUM_PLANES, t.n, t.n), dtype=np.float32)
+    state[: hg.NUM_STATE_PLANES] = t.read_state(hg.NUM_STATE_PLANES)
+    s = O.config_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0, slippage=0.1,
+                          G=9.81, dt=0.005)
+    O.run(state, s, 1, O.STEP_DEFAULT, O.RAIN_UNIFORM, 0.02, seed=1, step0=t.step)  # warm caches
+    t0 = time.perf_counter()
+    O.run(state, s, steps, O.STEP_DEFAULT, O.RAIN_UNIFORM, 0.02, seed=1, step0=t.step + 1)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": t.n * t.n * steps / dt, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} FP32 steps of the same {t.n}x{t.n} workload (state after the "
+                      f"warm-up), oracle/liboracle.so, {cores} OpenMP threads, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1900)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--per-pass", action="store_true", help="time the unfused per-pass kernels")
+    ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: the erosion path is measured per GPU and "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        return 2
+
+    n = args.n
+    flags = hg.STEP_DEFAULT | (hg.STEP_PER_PASS if args.per_pass else 0)
+    t = build_terrain(n)
+    t.dispatch_grid(args.warmup, flags)
+    t.sync()
+
+    t.set_timing(True)
+    t.reset_timing()
+    t.sync()
+    t0 = time.perf_counter()
+    t.dispatch_grid(args.steps, flags)
+    t.sync()
+    wall = time.perf_counter() - t0
+    stats = t.kernel_stats()
+    t.set_timing(False)
+
+    cell_steps = float(n) * n * args.steps
+    value = cell_steps / wall
+    kern = {k: {"launches": c, "avg_us": ms / c * 1e3} for k, (c, ms) in stats.items()}
+    dom = max(stats, key=lambda k: stats[k][1])
+    dom_avg_s = stats[dom][1] / stats[dom][0] / 1e3
+    per_cell = ALGO_BYTES.get(dom)
+    roofline = None
+    if per_cell is not None:
+        ach = per_cell * n * n / dom_avg_s
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
+                    "algo_bytes_per_launch": per_cell * n * n}
+
+    h = t.read_plane(hg.P_ROCK)
+    finite = bool(np.isfinite(h).all())
+    cpu = cpu_baseline(t, args.cpu_steps) if args.cpu_steps > 0 else None
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pipe+thermal GRID step, {n}x{n} FP32, simplex fBm [0,512], "
+                        "rain U(0,0.02) Philox per step, thermal+smoothing every step",
+            "grid": n,
+            "path": "per-pass" if args.per_pass else "fused",
+            "step_hbm_roofline_frac": value * STEP_ALGO_BYTES / HBM_PEAK,
+            "kernels": kern,
+            "state_finite": finite,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    t.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

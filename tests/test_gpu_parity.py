@@ -157,22 +157,46 @@ def test_multistep_vs_fp64_oracle_128():
 @pytest.mark.slow
 def test_config0_512_500_steps_vs_fp64_oracle():
     """Config 0: 512^2, fBm (6 octaves, f=1/256) to [0,64], rain 0.01/step,
-    500 steps, heights/water/sediment compared every 100 steps."""
+    500 steps, heights/water/sediment compared every 100 steps.
+
+    Free-running gates (GPU FP32 vs FP64 oracle from the same initial state):
+    height rel-L2 < 1e-4, water < 1e-3, sediment < 1e-2.  Sediment gets the
+    loose bound because cells with thin water have velocities ~ flux/depth, so
+    their semi-Lagrangian back-trace point is ill-conditioned; rounding alone
+    moves it by cells (the FP32 build of the oracle shows the same excursions).
+    Shadowed gates isolate per-step arithmetic from that amplification: at every
+    checkpoint the oracle restarts from the GPU's own state and both run 25
+    steps, which must agree to 1e-5 (heights), 1e-4 (water, sediment)."""
     n = 512
+    s = O.config_settings(**C0)
     with _terrain(n, rain_mode=hg.RAIN_CONSTANT, amount=0.01) as t:
         t.init_fbm(hg.fbm_params(seed=1, base_freq=1.0 / 256, hi=64.0))
         ref = t.read_state().astype(np.float64)
         for k in range(5):
             t.dispatch_grid(100, hg.STEP_DEFAULT)
             got = t.read_state(hg.NUM_STATE_PLANES)
-            O.run(ref, O.config_settings(**C0), 100, O.STEP_DEFAULT, O.RAIN_CONSTANT, 0.01, step0=100 * k)
+            O.run(ref, s, 100, O.STEP_DEFAULT, O.RAIN_CONSTANT, 0.01, step0=100 * k)
             h_ref = ref[O.ROCK] + ref[O.DIRT]
             h_got = got[hg.P_ROCK].astype(np.float64) + got[hg.P_DIRT]
             r = rel_l2(h_got, h_ref)
             rw = rel_l2(got[hg.P_WATER], ref[O.WATER])
             rs = rel_l2(got[hg.P_SED_ROCK].astype(np.float64) + got[hg.P_SED_DIRT], ref[O.S0] + ref[O.S1])
-            print(f"step {100 * (k + 1)}: height relL2 {r:.2e} water {rw:.2e} sediment {rs:.2e}", flush=True)
-            assert r < 1e-4 and rw < 1e-3 and rs < 1e-3
+            # shadow: oracle from the GPU state, 25 more steps on both
+            sh = O.new_state(n)
+            sh[: hg.NUM_STATE_PLANES] = got
+            O.run(sh, s, 25, O.STEP_DEFAULT, O.RAIN_CONSTANT, 0.01, step0=100 * (k + 1))
+            with _terrain(n, rain_mode=hg.RAIN_CONSTANT, amount=0.01) as t2:
+                t2.write_state(got)
+                t2.step = 100 * (k + 1)
+                t2.dispatch_grid(25, hg.STEP_DEFAULT)
+                g2 = t2.read_state(hg.NUM_STATE_PLANES)
+            sr = rel_l2(g2[hg.P_ROCK].astype(np.float64) + g2[hg.P_DIRT], sh[O.ROCK] + sh[O.DIRT])
+            sw = rel_l2(g2[hg.P_WATER], sh[O.WATER])
+            ss = rel_l2(g2[hg.P_SED_ROCK].astype(np.float64) + g2[hg.P_SED_DIRT], sh[O.S0] + sh[O.S1])
+            print(f"step {100 * (k + 1)}: free height {r:.2e} water {rw:.2e} sediment {rs:.2e} | "
+                  f"shadow height {sr:.2e} water {sw:.2e} sediment {ss:.2e}", flush=True)
+            assert r < 1e-4 and rw < 1e-3 and rs < 1e-2
+            assert sr < 1e-5 and sw < 1e-4 and ss < 1e-4
 
 
 def test_channel_io_roundtrip():
