@@ -14,7 +14,10 @@
 //    monotonic;
 //  * thermal outflow S*D_k/bk is formed as (S/bk)*D_k, so a cell's outflows
 //    are described by one scalar g = S/bk and its height differences;
-//  * mix(a,b,t) = a*(1-t)+b*t is evaluated as a + t*(b-a).
+//  * mix(a,b,t) = a*(1-t)+b*t is evaluated as a + t*(b-a);
+//  * divisions by a per-cell value use the hardware reciprocal (1 ulp), and
+//    the thermal height change is accumulated per direction,
+//    h += sum_k (in_k - out_k), instead of -sum(out) + sum(in).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -62,6 +65,9 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 
 __host__ __device__ inline float u24(uint32_t w) { return float(w >> 8) * (1.0f / 16777216.0f); }
 
+// Hardware reciprocal (v_rcp_f32, 1 ulp); rcp(0) = +inf.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // ---------------------------------------------------------------------------
 // hydro_flux (glsl/hydro_flux.glsl:77-166) for one cell.
 //  f[4]      : own old flux (L,R,T,B), replaced by the new flux
@@ -84,7 +90,7 @@ __device__ __forceinline__ float flux_cell(float f[4], float d1, float w, const 
     const float sum_in = (fin[0] + fin[1]) + (fin[2] + fin[3]);
     float sum_out = (f[0] + f[1]) + (f[2] + f[3]);
     // K = min(1, b/(sum_out*dt)); sum_out == 0 gives K = 1 (:125)
-    const float q = d1 / (sum_out * k.dt);
+    const float q = d1 * rcp(sum_out * k.dt);
     const float K = (q < 1.0f) ? q : 1.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) f[i] *= K;
@@ -92,7 +98,7 @@ __device__ __forceinline__ float flux_cell(float f[4], float d1, float w, const 
     const float d2 = fmaxf(0.0f, d1 + k.dt * (sum_in - sum_out));                      // :133-134
     vz = d1 + d2;                                                                      // :144
     if (vz > 0.0f) {                                                                   // :146-161
-        const float inv = 1.0f / vz;
+        const float inv = rcp(vz);
         vx = ((fin[0] - fo0) + (fo1 - fin[1])) * inv;
         vy = ((fin[3] - fo3) + (fo2 - fin[2])) * inv;
     } else {
@@ -118,7 +124,7 @@ __device__ __forceinline__ void erosion_cell(float& r, float& d, float& s0, floa
     }
     const float dx = hr - hl, dz = ht - hb;                                            // :23-35
     const float g2 = dx * dx + dz * dz;
-    const float sin_a = sqrtf(g2 / (g2 + 4.0f));                                       // :57
+    const float sin_a = sqrtf(g2 * rcp(g2 + 4.0f));                                    // :57
     const float cap0 = k.Kc * fmaxf(0.02f, sin_a) * ero;
     // layer loop, dirt (1) then rock (0), :58-83
     float cap = 0.0f;
@@ -178,24 +184,37 @@ __device__ __forceinline__ bool talus_marked(float D, int k, int layer, const KP
 
 __device__ __forceinline__ float thermal_g(const float D[8], float h_layer, int layer,
                                            const KParams& p) {
-    float H = 0.0f;                                                                    // :46-58
-#pragma unroll
-    for (int k = 0; k < 8; ++k) H = fmaxf(H, D[k]);
-    H = fminf(h_layer, H);
-    float bk = 0.0f, rmax = 0.0f;
     const float ta = p.tan_alpha[layer], td = p.thr_diag[layer];
+    float bk = 0.0f, mc = D[0], md = D[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (D[k] > ta) { bk += D[k]; rmax = fmaxf(rmax, D[k]); }
+    for (int k = 0; k < 4; ++k) {
+        bk += D[k] > ta ? D[k] : 0.0f;
+        mc = fmaxf(mc, D[k]);
+    }
 #pragma unroll
-    for (int k = 4; k < 8; ++k)
-        if (D[k] > td) { bk += D[k]; rmax = fmaxf(rmax, D[k] * 0.70710678118654752f); }
+    for (int k = 4; k < 8; ++k) {
+        bk += D[k] > td ? D[k] : 0.0f;
+        md = fmaxf(md, D[k]);
+    }
     if (!(bk > 0.0f)) return 0.0f;
-    float sharp = 1.0f + atanf(rmax) - p.Kalpha[layer];                                // :75-83
-    sharp = fmaxf(sharp, 1.0f);
+    // H = min(h_l, max(0, max_k D_k))                                                  :46-58
+    const float H = fminf(h_layer, fmaxf(fmaxf(mc, md), 0.0f));
+    // The largest marked D_k/d_k is the largest D_k/d_k overall when any direction
+    // is marked (unmarked cross D <= tan a < marked diagonal D/sqrt2 and vice versa).
+    const float rmax = fmaxf(mc, md * 0.70710678118654752f);
+    float sharp = fmaxf(1.0f + atanf(rmax) - p.Kalpha[layer], 1.0f);                   // :75-83
     sharp = sharp * (sharp * sharp);                                                   // :92
-    const float S = p.dt * p.Kspeed[layer] * sharp * H * 0.5f;                         // :95
-    return S / bk;
+    const float S = (p.dt * p.Kspeed[layer] * 0.5f) * sharp * H;                       // :95
+    return S * rcp(bk);
+}
+
+// Height change of cell p in direction k (thermal_transport.glsl:146-172):
+// in_k - out_k, where out_k = g_p*D_k when p marks k and in_k = g_n*(-D_k) when
+// the neighbour n = p+k marks the opposite direction (its difference to p is
+// exactly -D_k).  At most one of the two is non-zero since the threshold is >= 0.
+__device__ __forceinline__ float thermal_term(float Dk, float g_self, float g_nb, float thr) {
+    const float G = Dk > thr ? g_self : (-Dk > thr ? g_nb : 0.0f);
+    return (-Dk) * G;
 }
 
 // ---------------------------------------------------------------------------

@@ -32,12 +32,12 @@ __device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * 
 // Eroded sediment of one cell, recomputed from the step's input planes
 // (rain -> flux -> erosion at q).  Used for back-trace taps outside the halo.
 // ---------------------------------------------------------------------------
-__device__ void eroded_sediment_global(const Planes& in, int qx, int qy, int nx, int ny,
+__device__ void eroded_sediment_global(const HydroIO& io, int qx, int qy, int nx, int ny,
                                        uint32_t step, bool rain, const KParams& k, float& s0o,
                                        float& s1o) {
-    const float* R = in.p[HG_P_ROCK];
-    const float* D = in.p[HG_P_DIRT];
-    const float* B = in.p[HG_P_WATER];
+    const float* R = io.r;
+    const float* D = io.d;
+    const float* B = io.b;
     auto water = [&](int x, int y) -> float {
         float b = B[at(x, y, nx)];
         if (rain) b = b + rain_incr(x, y, step, 0, k);
@@ -54,21 +54,19 @@ __device__ void eroded_sediment_global(const Planes& in, int qx, int qy, int nx,
         return R[j] + D[j];
     };
     const size_t i = at(qx, qy, nx);
-    float f[4] = {in.p[HG_P_FLUX_L][i], in.p[HG_P_FLUX_R][i], in.p[HG_P_FLUX_T][i], in.p[HG_P_FLUX_B][i]};
+    float f[4] = {io.fl[i], io.fr[i], io.ft[i], io.fb[i]};
     const float b = water(qx, qy);
     const float w = (R[i] + D[i]) + b;
     const float wn[4] = {total(qx - 1, qy), total(qx + 1, qy), total(qx, qy + 1), total(qx, qy - 1)};
-    const float fin[4] = {qx > 0 ? in.p[HG_P_FLUX_R][i - 1] : 0.0f,
-                          qx < nx - 1 ? in.p[HG_P_FLUX_L][i + 1] : 0.0f,
-                          qy < ny - 1 ? in.p[HG_P_FLUX_B][i + nx] : 0.0f,
-                          qy > 0 ? in.p[HG_P_FLUX_T][i - nx] : 0.0f};
+    const float fin[4] = {qx > 0 ? io.fr[i - 1] : 0.0f, qx < nx - 1 ? io.fl[i + 1] : 0.0f,
+                          qy < ny - 1 ? io.fb[i + nx] : 0.0f, qy > 0 ? io.ft[i - nx] : 0.0f};
     const unsigned wall = (qx == 0 ? 1u : 0u) | (qx == nx - 1 ? 2u : 0u) | (qy == 0 ? 4u : 0u) |
                           (qy == ny - 1 ? 8u : 0u);
     float vx, vy, vz;
     (void)flux_cell(f, b, w, wn, fin, wall, k, vx, vy, vz);
     float r = R[i], d = D[i];
-    s0o = in.p[HG_P_SED_ROCK][i];
-    s1o = in.p[HG_P_SED_DIRT][i];
+    s0o = io.s0[i];
+    s1o = io.s1[i];
     erosion_cell(r, d, s0o, s1o, vx, vy, vz, rd(qx - 1, qy), rd(qx + 1, qy), rd(qx, qy + 1),
                  rd(qx, qy - 1), k);
 }
@@ -96,8 +94,8 @@ struct HydroTile {
 };
 
 template <int TX, int TY, int HA, int NT>
-__global__ __launch_bounds__(NT) void k_fused_hydro(Planes in, Planes out, int nx, int ny,
-                                                    uint32_t step, int rain, int keep_vel, KParams k) {
+__global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, uint32_t step,
+                                                    int rain, KParams k) {
     using T = HydroTile<TX, TY, HA, NT>;
     // LDS planes (pitch WI): rd = rock+dirt, b = water after rain, fl..fb = old flux.
     __shared__ float lds[6 * T::NLDS];
@@ -123,13 +121,13 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(Planes in, Planes out, int n
             float rd[4], b[4], fl[4], fr[4], ft[4], fb[4];
             if (gy >= 0 && gy < ny && gx >= 0 && gx + 3 < nx) {
                 const size_t g = at(gx, gy, nx);
-                const float4 r4 = *reinterpret_cast<const float4*>(in.p[HG_P_ROCK] + g);
-                const float4 d4 = *reinterpret_cast<const float4*>(in.p[HG_P_DIRT] + g);
-                const float4 b4 = *reinterpret_cast<const float4*>(in.p[HG_P_WATER] + g);
-                const float4 l4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_L] + g);
-                const float4 rr4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_R] + g);
-                const float4 t4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_T] + g);
-                const float4 bb4 = *reinterpret_cast<const float4*>(in.p[HG_P_FLUX_B] + g);
+                const float4 r4 = *reinterpret_cast<const float4*>(io.r + g);
+                const float4 d4 = *reinterpret_cast<const float4*>(io.d + g);
+                const float4 b4 = *reinterpret_cast<const float4*>(io.b + g);
+                const float4 l4 = *reinterpret_cast<const float4*>(io.fl + g);
+                const float4 rr4 = *reinterpret_cast<const float4*>(io.fr + g);
+                const float4 t4 = *reinterpret_cast<const float4*>(io.ft + g);
+                const float4 bb4 = *reinterpret_cast<const float4*>(io.fb + g);
                 rd[0] = r4.x + d4.x; rd[1] = r4.y + d4.y; rd[2] = r4.z + d4.z; rd[3] = r4.w + d4.w;
                 b[0] = b4.x; b[1] = b4.y; b[2] = b4.z; b[3] = b4.w;
                 fl[0] = l4.x; fl[1] = l4.y; fl[2] = l4.z; fl[3] = l4.w;
@@ -192,24 +190,24 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(Planes in, Planes out, int n
         float vx, vy, vz;
         const float d2 = flux_cell(f, b, w, wn, fin, wall, k, vx, vy, vz);
         const size_t g = at(gx, gy, nx);
-        float r = in.p[HG_P_ROCK][g], d = in.p[HG_P_DIRT][g];
-        float s0 = in.p[HG_P_SED_ROCK][g], s1 = in.p[HG_P_SED_DIRT][g];
+        float r = io.r[g], d = io.d[g];
+        float s0 = io.s0[g], s1 = io.s1[g];
         erosion_cell(r, d, s0, s1, vx, vy, vz, s_rd[li - 1], s_rd[li + 1], s_rd[li + T::WI],
                      s_rd[li - T::WI], k);
         es0[it] = s0; es1[it] = s1; evx[it] = vx; evy[it] = vy;
         const bool tile = ex >= HA && ex < HA + TX && ey >= HA && ey < HA + TY;
         if (tile) {
-            out.p[HG_P_ROCK][g] = r;
-            out.p[HG_P_DIRT][g] = d;
-            out.p[HG_P_WATER][g] = d2 * k.evap;     // sediment_transport.glsl:75
-            out.p[HG_P_FLUX_L][g] = f[0];
-            out.p[HG_P_FLUX_R][g] = f[1];
-            out.p[HG_P_FLUX_T][g] = f[2];
-            out.p[HG_P_FLUX_B][g] = f[3];
-            if (keep_vel) {
-                out.p[HG_P_VEL_U][g] = vx;
-                out.p[HG_P_VEL_V][g] = vy;
-                out.p[HG_P_VEL_DEPTH][g] = vz;
+            io.ro[g] = r;
+            io.dout[g] = d;
+            io.bo[g] = d2 * k.evap;     // sediment_transport.glsl:75
+            io.flo[g] = f[0];
+            io.fro[g] = f[1];
+            io.fto[g] = f[2];
+            io.fbo[g] = f[3];
+            if (io.vx) {
+                io.vx[g] = vx;
+                io.vy[g] = vy;
+                io.vz[g] = vz;
             }
         }
     }
@@ -255,168 +253,251 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(Planes in, Planes out, int n
             a0[0] = s_s0[e00]; a0[1] = s_s0[e10]; a0[2] = s_s0[e01]; a0[3] = s_s0[e11];
             a1[0] = s_s1[e00]; a1[1] = s_s1[e10]; a1[2] = s_s1[e01]; a1[3] = s_s1[e11];
         } else {
-            eroded_sediment_global(in, ix, iy, nx, ny, step, rain != 0, k, a0[0], a1[0]);
-            eroded_sediment_global(in, ix1, iy, nx, ny, step, rain != 0, k, a0[1], a1[1]);
-            eroded_sediment_global(in, ix, iy1, nx, ny, step, rain != 0, k, a0[2], a1[2]);
-            eroded_sediment_global(in, ix1, iy1, nx, ny, step, rain != 0, k, a0[3], a1[3]);
+            eroded_sediment_global(io, ix, iy, nx, ny, step, rain != 0, k, a0[0], a1[0]);
+            eroded_sediment_global(io, ix1, iy, nx, ny, step, rain != 0, k, a0[1], a1[1]);
+            eroded_sediment_global(io, ix, iy1, nx, ny, step, rain != 0, k, a0[2], a1[2]);
+            eroded_sediment_global(io, ix1, iy1, nx, ny, step, rain != 0, k, a0[3], a1[3]);
         }
         const size_t g = at(gx, gy, nx);
-        out.p[HG_P_SED_ROCK][g] = lerp1(lerp1(a0[0], a0[1], fx), lerp1(a0[2], a0[3], fx), fy);
-        out.p[HG_P_SED_DIRT][g] = lerp1(lerp1(a1[0], a1[1], fx), lerp1(a1[2], a1[3], fx), fy);
+        io.s0o[g] = lerp1(lerp1(a0[0], a0[1], fx), lerp1(a0[2], a0[3], fx), fy);
+        io.s1o[g] = lerp1(lerp1(a1[0], a1[1], fx), lerp1(a1[2], a1[3], fx), fy);
     }
 }
 
 // ---------------------------------------------------------------------------
 // thermal kernel: thermal layer 0, thermal layer 1, smoothing
+//
+// LDS holds rock, dirt and the outflow scale g on the tile +- 5 cells.  Stages
+// (radius = cells around the tile they are evaluated on):
+//   A r4: g of layer 0      B r3: rock += thermal change (written in place)
+//   C r2: g of layer 1      D r1: dirt += thermal change (written in place)
+//   E r0: smoothing, write back.
+// In-place updates go through registers between two barriers.
 // ---------------------------------------------------------------------------
-template <int TX, int TY, int NT>
-struct ThermalTile {
-    static constexpr int H = 5;                     // dependency radius
-    static constexpr int W0 = TX + 2 * H, H0 = TY + 2 * H;
-    static constexpr int N0 = W0 * H0;
+constexpr int TH = 5;   // dependency radius of thermal L0 + L1 + smoothing
+
+template <int TX, int TY, int NT, int RAD>
+struct Region {
+    static constexpr int W = TX + 2 * RAD, N = W * (TY + 2 * RAD);
+    static constexpr int ITER = (N + NT - 1) / NT;
 };
 
-__device__ __forceinline__ int region_count(int r, int TXv, int TYv) { return (TXv + 2 * r) * (TYv + 2 * r); }
+// Calls fn(it, lx, ly) for the cells of the region of radius RAD (LDS coords).
+template <int TX, int TY, int NT, int RAD, class F>
+__device__ __forceinline__ void region_for(int tid, F&& fn) {
+    using R = Region<TX, TY, NT, RAD>;
+#pragma unroll
+    for (int it = 0; it < R::ITER; ++it) {
+        const int i = tid + it * NT;
+        if (R::N % NT == 0 || i < R::N) {
+            const int ry = i / R::W, rx = i - ry * R::W;
+            fn(it, rx + (TH - RAD), ry + (TH - RAD));
+        }
+    }
+}
 
 template <int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_fused_thermal(Planes in, Planes out, int nx, int ny,
-                                                      int do_thermal, int do_smooth, KParams k) {
-    using T = ThermalTile<TX, TY, NT>;
-    constexpr int W0 = T::W0, N0 = T::N0;
-    // s_a: rock (stages A,B), then new dirt (stages D,E)
-    // s_d: dirt; s_g: layer-0 then layer-1 outflow scale g; s_r2: rock after layer 0
-    __shared__ float s_a[N0], s_d[N0], s_g[N0], s_r2[N0];
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_fused_thermal(ThermIO io, int nx, int ny, int do_thermal,
+                                                      int do_smooth, KParams k) {
+    constexpr int W0 = TX + 2 * TH, N0 = W0 * (TY + 2 * TH);
+    __shared__ float s_r[N0], s_d[N0], s_g[N0];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TX - T::H, y0 = blockIdx.y * TY - T::H;   // grid coords of LDS (0,0)
+    const int x0 = blockIdx.x * TX - TH, y0 = blockIdx.y * TY - TH;   // grid coords of LDS (0,0)
     constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+    constexpr int OFF[8] = {-1, 1, W0, -W0, W0 - 1, W0 + 1, -W0 - 1, -W0 + 1};
+    (void)DX; (void)DY;
+    auto in_grid = [&](int lx, int ly) {
+        const int gx = x0 + lx, gy = y0 + ly;
+        return gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+    };
 
-    // ---- load rock, dirt (out of grid: HG_OOB_HEIGHT, thermal_erosion.glsl:20-26)
+    // load rock, dirt; out of grid reads HG_OOB_HEIGHT (thermal_erosion.glsl:20-26)
     for (int i = tid; i < N0; i += NT) {
         const int ly = i / W0, lx = i - ly * W0;
         const int gx = x0 + lx, gy = y0 + ly;
         float r = HG_OOB_HEIGHT, d = HG_OOB_HEIGHT;
         if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
             const size_t g = at(gx, gy, nx);
-            r = in.p[HG_P_ROCK][g];
-            d = in.p[HG_P_DIRT][g];
+            r = io.r[g];
+            d = io.d[g];
         }
-        s_a[i] = r;
+        s_r[i] = r;
         s_d[i] = d;
-        s_r2[i] = r;
     }
     __syncthreads();
 
-    // Iterate the cells of the region of radius `rad` around the tile.
-    auto for_region = [&](int rad, auto&& fn) {
-        const int w = TX + 2 * rad, h = TY + 2 * rad;
-        const int n = w * h;
-        for (int i = tid; i < n; i += NT) {
-            const int ry = i / w, rx = i - ry * w;
-            const int lx = rx + (T::H - rad), ly = ry + (T::H - rad);
-            const int gx = x0 + lx, gy = y0 + ly;
-            fn(ly * W0 + lx, gx, gy);
-        }
-    };
-    auto in_grid = [&](int gx, int gy) { return gx >= 0 && gy >= 0 && gx < nx && gy < ny; };
-
     if (do_thermal) {
-        // ---- A: layer-0 outflow scale on radius 4
-        for_region(4, [&](int li, int gx, int gy) {
+        // ---- A: layer-0 outflow scale
+        region_for<TX, TY, NT, 4>(tid, [&](int, int lx, int ly) {
+            const int li = ly * W0 + lx;
             float g = 0.0f;
-            if (in_grid(gx, gy)) {
-                const float r = s_a[li];
+            if (in_grid(lx, ly)) {
+                const float r = s_r[li];
                 float D[8];
 #pragma unroll
-                for (int kk = 0; kk < 8; ++kk) D[kk] = r - s_a[li + DY[kk] * W0 + DX[kk]];
+                for (int kk = 0; kk < 8; ++kk) D[kk] = r - s_r[li + OFF[kk]];
                 g = thermal_g(D, r, 0, k);
             }
             s_g[li] = g;
         });
         __syncthreads();
-        // ---- B: layer-0 transport on radius 3 -> s_r2
-        for_region(3, [&](int li, int gx, int gy) {
-            if (!in_grid(gx, gy)) return;
-            const float r = s_a[li], g = s_g[li];
-            float fin[8], fout[8];
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-                const int lj = li + DY[kk] * W0 + DX[kk];
-                const float Dk = r - s_a[lj];
-                fout[kk] = talus_marked(Dk, kk, 0, k) ? g * Dk : 0.0f;
-                fin[kk] = talus_marked(-Dk, kk, 0, k) ? s_g[lj] * (-Dk) : 0.0f;
-            }
-            s_r2[li] = r + thermal_delta(fin, fout);
-        });
-        __syncthreads();
-        // ---- C: layer-1 outflow scale on radius 2 (rock after layer 0)
-        for_region(2, [&](int li, int gx, int gy) {
-            float g = 0.0f;
-            if (in_grid(gx, gy)) {
-                const float r = s_r2[li], d = s_d[li];
-                float D[8];
+        // ---- B: layer-0 transport, rock in place
+        {
+            float nr[Region<TX, TY, NT, 3>::ITER];
+            region_for<TX, TY, NT, 3>(tid, [&](int it, int lx, int ly) {
+                const int li = ly * W0 + lx;
+                const float r = s_r[li], g = s_g[li];
+                float acc = 0.0f;
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
-                    const int lj = li + DY[kk] * W0 + DX[kk];
-                    D[kk] = (r - s_r2[lj]) + (d - s_d[lj]);
+                    const float Dk = r - s_r[li + OFF[kk]];
+                    acc = acc + thermal_term(Dk, g, s_g[li + OFF[kk]],
+                                             kk < 4 ? k.tan_alpha[0] : k.thr_diag[0]);
                 }
+                nr[it] = r + acc;
+            });
+            __syncthreads();
+            region_for<TX, TY, NT, 3>(tid, [&](int it, int lx, int ly) {
+                if (in_grid(lx, ly)) s_r[ly * W0 + lx] = nr[it];
+            });
+            __syncthreads();
+        }
+        // ---- C: layer-1 outflow scale (rock after layer 0)
+        region_for<TX, TY, NT, 2>(tid, [&](int, int lx, int ly) {
+            const int li = ly * W0 + lx;
+            float g = 0.0f;
+            if (in_grid(lx, ly)) {
+                const float r = s_r[li], d = s_d[li];
+                float D[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) D[kk] = (r - s_r[li + OFF[kk]]) + (d - s_d[li + OFF[kk]]);
                 g = thermal_g(D, d, 1, k);
             }
             s_g[li] = g;
         });
         __syncthreads();
-        // ---- D: layer-1 transport on radius 1 -> s_a (new dirt)
-        for_region(1, [&](int li, int gx, int gy) {
-            if (!in_grid(gx, gy)) return;
-            const float r = s_r2[li], d = s_d[li], g = s_g[li];
-            float fin[8], fout[8];
+        // ---- D: layer-1 transport, dirt in place
+        {
+            float nd[Region<TX, TY, NT, 1>::ITER];
+            region_for<TX, TY, NT, 1>(tid, [&](int it, int lx, int ly) {
+                const int li = ly * W0 + lx;
+                const float r = s_r[li], d = s_d[li], g = s_g[li];
+                float acc = 0.0f;
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-                const int lj = li + DY[kk] * W0 + DX[kk];
-                const float Dk = (r - s_r2[lj]) + (d - s_d[lj]);
-                fout[kk] = talus_marked(Dk, kk, 1, k) ? g * Dk : 0.0f;
-                fin[kk] = talus_marked(-Dk, kk, 1, k) ? s_g[lj] * (-Dk) : 0.0f;
-            }
-            s_a[li] = d + thermal_delta(fin, fout);
-        });
-        __syncthreads();
-    } else {
-        // no thermal: new dirt = dirt
-        for (int i = tid; i < N0; i += NT) s_a[i] = s_d[i];
-        __syncthreads();
+                for (int kk = 0; kk < 8; ++kk) {
+                    const float Dk = (r - s_r[li + OFF[kk]]) + (d - s_d[li + OFF[kk]]);
+                    acc = acc + thermal_term(Dk, g, s_g[li + OFF[kk]],
+                                             kk < 4 ? k.tan_alpha[1] : k.thr_diag[1]);
+                }
+                nd[it] = d + acc;
+            });
+            __syncthreads();
+            region_for<TX, TY, NT, 1>(tid, [&](int it, int lx, int ly) {
+                if (in_grid(lx, ly)) s_d[ly * W0 + lx] = nd[it];
+            });
+            __syncthreads();
+        }
     }
 
     // ---- E: smoothing on the tile, write back
-    for_region(0, [&](int li, int gx, int gy) {
-        if (!in_grid(gx, gy)) return;
-        float r = s_r2[li], d = s_a[li];
+    region_for<TX, TY, NT, 0>(tid, [&](int, int lx, int ly) {
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx >= nx || gy >= ny) return;
+        const int li = ly * W0 + lx;
+        float r = s_r[li], d = s_d[li];
         if (do_smooth && gx > 0 && gy > 0 && gx < nx - 1 && gy < ny - 1)
-            smooth_cell(r, d, s_r2[li - 1], s_a[li - 1], s_r2[li + 1], s_a[li + 1], s_r2[li + W0],
-                        s_a[li + W0], s_r2[li - W0], s_a[li - W0], k.smooth_m);
+            smooth_cell(r, d, s_r[li - 1], s_d[li - 1], s_r[li + 1], s_d[li + 1], s_r[li + W0],
+                        s_d[li + W0], s_r[li - W0], s_d[li - W0], k.smooth_m);
         const size_t g = at(gx, gy, nx);
-        out.p[HG_P_ROCK][g] = r;
-        out.p[HG_P_DIRT][g] = d;
+        io.ro[g] = r;
+        io.dout[g] = d;
     });
 }
 
-constexpr int HYDRO_TX = 64, HYDRO_TY = 16, HYDRO_HA = 2, HYDRO_NT = 256;
-constexpr int THERM_TX = 64, THERM_TY = 16, THERM_NT = 256;
+// Compiled tile variants (the autotuner picks among them per grid size).
+//                      TX   TY  HA   NT
+#define HG_HYDRO_VARIANTS(X) \
+    X(0,  64, 16, 2, 256)    \
+    X(1,  64, 32, 2, 512)    \
+    X(2, 128, 16, 2, 512)    \
+    X(3,  64,  8, 2, 256)    \
+    X(4,  64, 16, 3, 256)    \
+    X(5, 128,  8, 2, 256)
+//                        TX   TY   NT
+#define HG_THERMAL_VARIANTS(X) \
+    X(0,  64, 16, 256)         \
+    X(1,  64, 32, 256)         \
+    X(2,  64, 32, 512)         \
+    X(3,  64, 64, 512)         \
+    X(4, 128, 32, 512)         \
+    X(5,  32, 32, 256)         \
+    X(6, 128, 16, 512)
 
 }  // namespace
 
-hipError_t launch_fused_hydro(const Planes& in, const Planes& out, int nx, int ny, uint32_t step,
-                              bool rain, bool keep_velocity, const KParams& k, hipStream_t s) {
-    const dim3 grid((nx + HYDRO_TX - 1) / HYDRO_TX, (ny + HYDRO_TY - 1) / HYDRO_TY);
-    k_fused_hydro<HYDRO_TX, HYDRO_TY, HYDRO_HA, HYDRO_NT><<<grid, HYDRO_NT, 0, s>>>(
-        in, out, nx, ny, step, rain ? 1 : 0, keep_velocity ? 1 : 0, k);
+int hydro_variant_count() {
+#define X(i, tx, ty, ha, nt) +1
+    return 0 HG_HYDRO_VARIANTS(X);
+#undef X
+}
+
+int thermal_variant_count() {
+#define X(i, tx, ty, nt) +1
+    return 0 HG_THERMAL_VARIANTS(X);
+#undef X
+}
+
+bool hydro_variant_info(int v, int* tx, int* ty, int* halo, int* nt) {
+    switch (v) {
+#define X(i, TX_, TY_, HA_, NT_) \
+    case i: *tx = TX_; *ty = TY_; *halo = HA_; *nt = NT_; return true;
+        HG_HYDRO_VARIANTS(X)
+#undef X
+        default: return false;
+    }
+}
+
+bool thermal_variant_info(int v, int* tx, int* ty, int* halo, int* nt) {
+    switch (v) {
+#define X(i, TX_, TY_, NT_) \
+    case i: *tx = TX_; *ty = TY_; *halo = TH; *nt = NT_; return true;
+        HG_THERMAL_VARIANTS(X)
+#undef X
+        default: return false;
+    }
+}
+
+hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step, bool rain,
+                              const KParams& k, hipStream_t s) {
+    switch (variant) {
+#define X(i, TX_, TY_, HA_, NT_)                                                              \
+    case i: {                                                                                 \
+        const dim3 grid((nx + TX_ - 1) / TX_, (ny + TY_ - 1) / TY_);                          \
+        k_fused_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, step, rain ? 1 : 0, k); \
+        break;                                                                                \
+    }
+        HG_HYDRO_VARIANTS(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_fused_thermal(const Planes& in, const Planes& out, int nx, int ny, bool thermal,
+hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,
                                 bool smooth, const KParams& k, hipStream_t s) {
-    const dim3 grid((nx + THERM_TX - 1) / THERM_TX, (ny + THERM_TY - 1) / THERM_TY);
-    k_fused_thermal<THERM_TX, THERM_TY, THERM_NT><<<grid, THERM_NT, 0, s>>>(
-        in, out, nx, ny, thermal ? 1 : 0, smooth ? 1 : 0, k);
+    switch (variant) {
+#define X(i, TX_, TY_, NT_)                                                                  \
+    case i: {                                                                                \
+        const dim3 grid((nx + TX_ - 1) / TX_, (ny + TY_ - 1) / TY_);                         \
+        k_fused_thermal<TX_, TY_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, thermal ? 1 : 0,     \
+                                                             smooth ? 1 : 0, k);             \
+        break;                                                                               \
+    }
+        HG_THERMAL_VARIANTS(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

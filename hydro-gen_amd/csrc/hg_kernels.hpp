@@ -14,17 +14,6 @@ struct Planes {
     float* p[HG_NUM_PLANES];
 };
 
-// Sum of 8 values in a fixed pairwise order (thermal in/out totals).
-__device__ __forceinline__ float sum8(const float v[8]) {
-    return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-}
-
-// Thermal height change of one cell: inflow minus own outflow
-// (glsl/thermal_transport.glsl:146-172).
-__device__ __forceinline__ float thermal_delta(const float in[8], const float out[8]) {
-    return sum8(in) - sum8(out);
-}
-
 // Rain increment of cell (x, y) at `step` (config rain modes).
 __device__ __forceinline__ float rain_incr(int x, int y, uint32_t step, uint32_t terrain,
                                            const KParams& k) {
@@ -45,9 +34,22 @@ hipError_t launch_pass(int pass, const Planes& in, const Planes& out, int nx, in
                        const KParams& k, hipStream_t s);
 
 // fused kernels (hg_fused.hip)
-hipError_t launch_fused_hydro(const Planes& in, const Planes& out, int nx, int ny, uint32_t step,
-                              bool rain, bool keep_velocity, const KParams& k, hipStream_t s);
-hipError_t launch_fused_thermal(const Planes& in, const Planes& out, int nx, int ny, bool thermal,
+struct HydroIO {
+    const float *r, *d, *b, *fl, *fr, *ft, *fb, *s0, *s1;   // step input
+    float *ro, *dout, *bo, *flo, *fro, *fto, *fbo, *s0o, *s1o; // step output
+    float *vx, *vy, *vz;                                     // velocity out (null = not kept)
+};
+struct ThermIO {
+    const float *r, *d;
+    float *ro, *dout;
+};
+int hydro_variant_count();
+int thermal_variant_count();
+bool hydro_variant_info(int v, int* tx, int* ty, int* halo, int* nt);
+bool thermal_variant_info(int v, int* tx, int* ty, int* halo, int* nt);
+hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step, bool rain,
+                              const KParams& k, hipStream_t s);
+hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,
                                 bool smooth, const KParams& k, hipStream_t s);
 
 // heightmap init (hg_init.hip)

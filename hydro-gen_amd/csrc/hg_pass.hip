@@ -135,16 +135,16 @@ __global__ __launch_bounds__(BX * BY) void k_thermal_transport(Planes in, Planes
     const size_t i = at(x, y, nx);
     // neighbour k sends its outflow in the opposite direction
     const int opp[8] = {1, 0, 3, 2, 7, 6, 5, 4};
-    float fin[8], fout[8];
+    float acc = 0.0f;
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
         const int xx = x + TDX[kk], yy = y + TDY[kk];
         const bool ok = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
-        fin[kk] = ok ? in.p[HG_P_THERM_L + opp[kk]][at(xx, yy, nx)] : 0.0f;
-        fout[kk] = in.p[HG_P_THERM_L + kk][i];
+        const float fin = ok ? in.p[HG_P_THERM_L + opp[kk]][at(xx, yy, nx)] : 0.0f;
+        acc = acc + (fin - in.p[HG_P_THERM_L + kk][i]);
     }
     const int hp = layer == 0 ? HG_P_ROCK : HG_P_DIRT;
-    out.p[hp][i] = in.p[hp][i] + thermal_delta(fin, fout);
+    out.p[hp][i] = in.p[hp][i] + acc;
 }
 
 // glsl/smoothing.glsl:22-74,97-103 (grid branch)

@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -24,6 +26,12 @@ using hgk::Planes;
 namespace {
 
 thread_local std::string g_create_error;
+
+// Autotuner results per (grid size, device architecture): {hydro, thermal}.
+std::mutex g_tune_mu;
+std::map<std::pair<int, std::string>, std::pair<int, int>> g_tune_cache;
+constexpr int DEFAULT_HYDRO_VARIANT = 0;
+constexpr int DEFAULT_THERMAL_VARIANT = 1;
 
 struct KernelStat {
     std::string name;
@@ -52,6 +60,9 @@ struct hg_ctx {
     uint64_t seed = 0;
     uint64_t step = 0;
     KParams kp{};
+    int hydro_variant = 0;           // fused kernel tile variants (see hg_autotune)
+    int thermal_variant = 0;
+    std::string arch;
     std::string err;
     // timing
     bool timing = false;
@@ -240,6 +251,38 @@ int run_pass(hg_ctx* c, int pass) {
     return HG_OK;
 }
 
+int fused_step(hg_ctx* c, bool rain, bool thermal, bool smooth, bool keep_velocity) {
+    hgk::HydroIO io;
+    io.r = c->cur[HG_P_ROCK]; io.d = c->cur[HG_P_DIRT]; io.b = c->cur[HG_P_WATER];
+    io.fl = c->cur[HG_P_FLUX_L]; io.fr = c->cur[HG_P_FLUX_R]; io.ft = c->cur[HG_P_FLUX_T];
+    io.fb = c->cur[HG_P_FLUX_B]; io.s0 = c->cur[HG_P_SED_ROCK]; io.s1 = c->cur[HG_P_SED_DIRT];
+    io.ro = c->alt[HG_P_ROCK]; io.dout = c->alt[HG_P_DIRT]; io.bo = c->alt[HG_P_WATER];
+    io.flo = c->alt[HG_P_FLUX_L]; io.fro = c->alt[HG_P_FLUX_R]; io.fto = c->alt[HG_P_FLUX_T];
+    io.fbo = c->alt[HG_P_FLUX_B]; io.s0o = c->alt[HG_P_SED_ROCK]; io.s1o = c->alt[HG_P_SED_DIRT];
+    io.vx = keep_velocity ? c->alt[HG_P_VEL_U] : nullptr;
+    io.vy = keep_velocity ? c->alt[HG_P_VEL_V] : nullptr;
+    io.vz = keep_velocity ? c->alt[HG_P_VEL_DEPTH] : nullptr;
+    const uint32_t step = uint32_t(c->step);
+    int rc = timed_launch(c, "fused_hydro", [&] {
+        return hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, rain, c->kp, c->stream);
+    });
+    if (rc != HG_OK) return rc;
+    swap_planes(c, {HG_P_ROCK, HG_P_DIRT, HG_P_WATER, HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
+                    HG_P_SED_ROCK, HG_P_SED_DIRT});
+    if (keep_velocity) swap_planes(c, {HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH});
+    if (!thermal && !smooth) return HG_OK;
+    hgk::ThermIO tio;
+    tio.r = c->cur[HG_P_ROCK]; tio.d = c->cur[HG_P_DIRT];
+    tio.ro = c->alt[HG_P_ROCK]; tio.dout = c->alt[HG_P_DIRT];
+    rc = timed_launch(c, "fused_thermal", [&] {
+        return hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->kp,
+                                         c->stream);
+    });
+    if (rc != HG_OK) return rc;
+    swap_planes(c, {HG_P_ROCK, HG_P_DIRT});
+    return HG_OK;
+}
+
 bool finite_settings(const hg_erosion_settings* s) {
     const float v[] = {s->Kc, s->Kalpha[0], s->Kalpha[1], s->Kconv, s->Ks[0], s->Ks[1], s->Kd[0],
                        s->Kd[1], s->Ke, s->energy_kept, s->Kspeed[0], s->Kspeed[1], s->G, s->d_t,
@@ -301,6 +344,18 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
     c->es = reference_grid_defaults();
     c->rs = reference_rain_defaults();
     update_kparams(c);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->arch = prop.gcnArchName;
+    c->hydro_variant = DEFAULT_HYDRO_VARIANT;
+    c->thermal_variant = DEFAULT_THERMAL_VARIANT;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find({c->n, c->arch});
+        if (it != g_tune_cache.end()) {
+            c->hydro_variant = it->second.first;
+            c->thermal_variant = it->second.second;
+        }
+    }
     *out = c;
     return HG_OK;
 }
@@ -417,25 +472,7 @@ int hg_step_grid(hg_ctx* c, uint32_t nsteps, uint32_t flags) {
                     rc = run_pass(c, p);
             if (rc == HG_OK && smooth) rc = run_pass(c, HG_PASS_SMOOTH);
         } else {
-            const bool kv = (flags & HG_STEP_KEEP_VELOCITY) != 0;
-            const Planes in = planes_of(c->cur), out = planes_of(c->alt);
-            const uint32_t step = uint32_t(c->step);
-            rc = timed_launch(c, "fused_hydro", [&] {
-                return hgk::launch_fused_hydro(in, out, c->n, c->n, step, rain, kv, c->kp, c->stream);
-            });
-            if (rc == HG_OK) {
-                swap_planes(c, {HG_P_ROCK, HG_P_DIRT, HG_P_WATER, HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T,
-                                HG_P_FLUX_B, HG_P_SED_ROCK, HG_P_SED_DIRT});
-                if (kv) swap_planes(c, {HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH});
-                if (thermal || smooth) {
-                    const Planes in2 = planes_of(c->cur), out2 = planes_of(c->alt);
-                    rc = timed_launch(c, "fused_thermal", [&] {
-                        return hgk::launch_fused_thermal(in2, out2, c->n, c->n, thermal, smooth, c->kp,
-                                                         c->stream);
-                    });
-                    if (rc == HG_OK) swap_planes(c, {HG_P_ROCK, HG_P_DIRT});
-                }
-            }
+            rc = fused_step(c, rain, thermal, smooth, (flags & HG_STEP_KEEP_VELOCITY) != 0);
         }
         if (rc != HG_OK) return rc;
         c->step += 1;
@@ -557,6 +594,101 @@ int hg_kernel_stats(hg_ctx* c, int i, const char** name, uint64_t* launches, dou
     if (name) *name = c->stats[i].name.c_str();
     if (launches) *launches = c->stats[i].launches;
     if (total_ms) *total_ms = c->stats[i].total_ms;
+    return HG_OK;
+}
+
+int hg_tile_variant_count(int kernel) {
+    if (kernel == HG_KERNEL_HYDRO) return hgk::hydro_variant_count();
+    if (kernel == HG_KERNEL_THERMAL) return hgk::thermal_variant_count();
+    return HG_ERR_INVALID;
+}
+
+int hg_tile_variant_info(int kernel, int variant, int* tile_x, int* tile_y, int* halo, int* threads) {
+    int tx = 0, ty = 0, h = 0, nt = 0;
+    bool ok = false;
+    if (kernel == HG_KERNEL_HYDRO) ok = hgk::hydro_variant_info(variant, &tx, &ty, &h, &nt);
+    else if (kernel == HG_KERNEL_THERMAL) ok = hgk::thermal_variant_info(variant, &tx, &ty, &h, &nt);
+    if (!ok) return HG_ERR_INVALID;
+    if (tile_x) *tile_x = tx;
+    if (tile_y) *tile_y = ty;
+    if (halo) *halo = h;
+    if (threads) *threads = nt;
+    return HG_OK;
+}
+
+int hg_set_tiles(hg_ctx* c, int hydro_variant, int thermal_variant) {
+    if (!c) return HG_ERR_INVALID;
+    if (hydro_variant >= hgk::hydro_variant_count() || thermal_variant >= hgk::thermal_variant_count())
+        return fail(c, HG_ERR_INVALID, "hg_set_tiles: no such variant");
+    if (hydro_variant >= 0) c->hydro_variant = hydro_variant;
+    if (thermal_variant >= 0) c->thermal_variant = thermal_variant;
+    return HG_OK;
+}
+
+int hg_get_tiles(const hg_ctx* c, int* hydro_variant, int* thermal_variant) {
+    if (!c) return HG_ERR_INVALID;
+    if (hydro_variant) *hydro_variant = c->hydro_variant;
+    if (thermal_variant) *thermal_variant = c->thermal_variant;
+    return HG_OK;
+}
+
+int hg_autotune(hg_ctx* c, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best) {
+    if (!c) return HG_ERR_INVALID;
+    if (steps == 0) steps = 10;
+    DeviceGuard guard(c->device);
+    const size_t pn = plane_n(c);
+    // back up the persistent planes and the step counter; the trials advance them
+    void* backup = nullptr;
+    HG_HIP(c, hipMalloc(&backup, pn * sizeof(float) * HG_NUM_STATE_PLANES));
+    float* bk = static_cast<float*>(backup);
+    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p)
+        HG_HIP(c, hipMemcpyAsync(bk + p * pn, c->cur[p], pn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    const uint64_t step0 = c->step;
+    const bool saved_timing = c->timing;
+    const bool rain = (flags & HG_STEP_RAIN) && c->rain_mode != HG_RAIN_OFF;
+    const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
+    int rc = resolve_timing(c);
+    std::vector<KernelStat> saved_stats = c->stats;
+    c->timing = true;
+    const int h0 = c->hydro_variant, t0 = c->thermal_variant;
+    double best[2] = {1e300, 1e300};
+    int pick[2] = {h0, t0};
+    for (int kind = 0; kind < 2 && rc == HG_OK; ++kind) {
+        const int nv = kind == 0 ? hgk::hydro_variant_count() : hgk::thermal_variant_count();
+        for (int v = 0; v < nv && rc == HG_OK; ++v) {
+            if (kind == 0) c->hydro_variant = v; else c->thermal_variant = v;
+            rc = fused_step(c, rain, thermal, smooth, false);   // warm-up
+            if (rc == HG_OK) rc = resolve_timing(c);
+            c->stats.clear();
+            for (uint32_t s2 = 0; s2 < steps && rc == HG_OK; ++s2) rc = fused_step(c, rain, thermal, smooth, false);
+            if (rc == HG_OK) rc = resolve_timing(c);
+            const char* name = kind == 0 ? "fused_hydro" : "fused_thermal";
+            for (auto& st : c->stats)
+                if (st.name == name && st.launches && st.total_ms / st.launches < best[kind]) {
+                    best[kind] = st.total_ms / st.launches;
+                    pick[kind] = v;
+                }
+        }
+        if (kind == 0) c->hydro_variant = pick[0]; else c->thermal_variant = pick[1];
+    }
+    c->stats = saved_stats;
+    c->timing = saved_timing;
+    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p)
+        (void)hipMemcpyAsync(c->cur[p], bk + p * pn, pn * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(backup);
+    c->step = step0;
+    if (rc != HG_OK) {
+        c->hydro_variant = h0;
+        c->thermal_variant = t0;
+        return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[{c->n, c->arch}] = {pick[0], pick[1]};
+    }
+    if (hydro_best) *hydro_best = pick[0];
+    if (thermal_best) *thermal_best = pick[1];
     return HG_OK;
 }
 

@@ -35,7 +35,9 @@ EXPORTS = [
     "hg_set_rain", "hg_init_fbm", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
     "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
+    "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_autotune",
 ]
+KERNEL_HYDRO, KERNEL_THERMAL = 0, 1
 
 
 class ErosionSettings(ctypes.Structure):
@@ -147,6 +149,12 @@ def load() -> ctypes.CDLL:
     L.hg_kernel_stats.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                   ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]
     L.hg_reset_timing.argtypes = [vp]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.hg_tile_variant_count.argtypes = [ctypes.c_int]
+    L.hg_tile_variant_info.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
+    L.hg_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hg_get_tiles.argtypes = [vp, ip, ip]
+    L.hg_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ip, ip]
     _lib = L
     return L
 
@@ -291,3 +299,28 @@ class Terrain:
 
     def reset_timing(self):
         self._check(self.lib.hg_reset_timing(self.h))
+
+    # -- tiles / autotuning --
+    def set_tiles(self, hydro: int = -1, thermal: int = -1):
+        self._check(self.lib.hg_set_tiles(self.h, hydro, thermal))
+
+    def get_tiles(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.hg_get_tiles(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, steps: int = 10, flags: int = STEP_DEFAULT):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.hg_autotune(self.h, steps, flags, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+def tile_variants(kernel: int) -> list:
+    """[(tile_x, tile_y, halo, threads), ...] of the compiled variants of a fused kernel."""
+    L = load()
+    out = []
+    for v in range(L.hg_tile_variant_count(kernel)):
+        a = [ctypes.c_int() for _ in range(4)]
+        if L.hg_tile_variant_info(kernel, v, *[ctypes.byref(x) for x in a]) == HG_OK:
+            out.append(tuple(x.value for x in a))
+    return out

@@ -224,6 +224,24 @@ int hg_kernel_stats(hg_ctx* ctx, int i, const char** name, uint64_t* launches,
                     double* total_ms);
 int hg_reset_timing(hg_ctx* ctx);
 
+/* ---- tile variants and autotuning -------------------------------------- */
+
+/* The two fused kernels of a GRID step are compiled for several tile shapes
+ * (tile width x height, halo, threads per workgroup).  The autotuner times
+ * each on the context's own grid and keeps the fastest; the choice is cached
+ * per (grid size, device architecture) for contexts created later in the
+ * process.  These have no counterpart in the reference (its workgroup is a
+ * fixed 8x8, glsl/bindings.glsl:9-10). */
+enum { HG_KERNEL_HYDRO = 0, HG_KERNEL_THERMAL = 1 };
+int hg_tile_variant_count(int kernel);
+int hg_tile_variant_info(int kernel, int variant, int* tile_x, int* tile_y, int* halo, int* threads);
+/* -1 keeps the current choice. */
+int hg_set_tiles(hg_ctx* ctx, int hydro_variant, int thermal_variant);
+int hg_get_tiles(const hg_ctx* ctx, int* hydro_variant, int* thermal_variant);
+/* Times `steps` fused steps per variant (state and step counter are restored
+ * afterwards) and selects the fastest of each kernel. */
+int hg_autotune(hg_ctx* ctx, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best);
+
 #ifdef __cplusplus
 }
 #endif
