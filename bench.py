@@ -49,6 +49,67 @@ def build_terrain(n: int, seed: int = 1) -> hg.Terrain:
     return t
 
 
+def batched_workload(seed: int = 2026, count: int = 64):
+    """Batched config: `count` terrains, sides drawn uniformly from
+    {256, 384, ..., 2048}, own fBm seed, coefficients Kc~U(0.02,0.1),
+    Ks,Kd~U(0.01,0.05), Ke~U(0.005,0.02), talus~U(30,45) deg."""
+    rng = np.random.default_rng(seed)
+    sides = [int(x) for x in rng.choice(np.arange(256, 2049, 128), size=count)]
+    coeffs = [dict(Kc=float(rng.uniform(0.02, 0.1)), Ks=float(rng.uniform(0.01, 0.05)),
+                   Kd=float(rng.uniform(0.01, 0.05)), Ke=float(rng.uniform(0.005, 0.02)),
+                   talus_deg=float(rng.uniform(30, 45)), slippage=0.1, G=9.81, dt=0.005)
+              for _ in range(count)]
+    return sides, coeffs
+
+
+def bench_batched(steps: int, base_steps: int, tiles=None) -> dict:
+    sides, coeffs = batched_workload()
+    cells = float(sum(n * n for n in sides))
+    b = hg.Batch(sides)
+    if tiles:
+        b.set_tiles(*tiles)
+    b.set_rain(hg.RAIN_UNIFORM, amount=0.02, seed=7)
+    for i, n in enumerate(sides):
+        b.set_erosion(hg.erosion_settings(**coeffs[i]), terrain=i)
+        b.init_fbm(i, hg.fbm_params(seed=1000 + i, base_freq=1.0 / 256, hi=float(n) / 8))
+    out = {"terrains": len(sides), "cells": int(cells), "sides_min": min(sides), "sides_max": max(sides)}
+    # baselines: per-terrain launches, single-terrain fused kernels / one kernel per pass
+    for key, flags, k in (("per_terrain_fused", hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN, base_steps),
+                          ("per_terrain_per_pass",
+                           hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN | hg.STEP_PER_PASS, base_steps)):
+        b.step(2, flags)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(k, flags)
+        b.sync()
+        dt = time.perf_counter() - t0
+        out[key] = {"steps": k, "cell_steps_per_s": cells * k / dt, "ms_per_step": dt / k * 1e3}
+    # batched: one launch per pass over all terrains' tiles, `steps` steps
+    # captured in one hipGraph; flat grid (a workgroup per tile) and persistent grid
+    best = None
+    for persistent in (False, True):
+        b.set_persistent(persistent)
+        b.capture(steps)
+        b.run()          # warm-up replay
+        b.sync()
+        t0 = time.perf_counter()
+        b.run()
+        b.sync()
+        dt = time.perf_counter() - t0
+        v = cells * steps / dt
+        key = "batched_graph_persistent" if persistent else "batched_graph"
+        out[key] = {"steps": steps, "cell_steps_per_s": v, "ms_per_step": dt / steps * 1e3,
+                    "hbm_roofline_frac": v * STEP_ALGO_BYTES / HBM_PEAK}
+        best = v if best is None else max(best, v)
+    v = best
+    out["speedup_vs_per_terrain_per_pass"] = v / out["per_terrain_per_pass"]["cell_steps_per_s"]
+    out["speedup_vs_per_terrain_fused"] = v / out["per_terrain_fused"]["cell_steps_per_s"]
+    h = b.read_plane(len(sides) - 1, hg.P_ROCK)
+    out["state_finite"] = bool(np.isfinite(h).all())
+    b.close()
+    return out
+
+
 def cpu_baseline(t: hg.Terrain, steps: int) -> dict:
     """FP32 oracle (CPU port of the same passes) on `steps` steps of the same
     4096^2 state; OpenMP threads from OMP_NUM_THREADS."""
@@ -76,6 +137,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--per-pass", action="store_true", help="time the unfused per-pass kernels")
     ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample (0 = skip)")
+    ap.add_argument("--batched-steps", type=int, default=1000,
+                    help="steps of the 64-terrain batched graph (0 = skip)")
+    ap.add_argument("--tune", action="store_true", default=True)
+    ap.add_argument("--no-tune", dest="tune", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: the erosion path is measured per GPU and "
@@ -85,6 +150,9 @@ def main() -> int:
     n = args.n
     flags = hg.STEP_DEFAULT | (hg.STEP_PER_PASS if args.per_pass else 0)
     t = build_terrain(n)
+    tiles = None
+    if args.tune and not args.per_pass:
+        tiles = t.autotune(10, flags)
     t.dispatch_grid(args.warmup, flags)
     t.sync()
 
@@ -114,6 +182,8 @@ def main() -> int:
     h = t.read_plane(hg.P_ROCK)
     finite = bool(np.isfinite(h).all())
     cpu = cpu_baseline(t, args.cpu_steps) if args.cpu_steps > 0 else None
+    t.close()
+    batched = bench_batched(args.batched_steps, 10, tiles) if args.batched_steps > 0 else None
 
     out = {
         "metric": METRIC,
@@ -135,13 +205,14 @@ def main() -> int:
             "path": "per-pass" if args.per_pass else "fused",
             "step_hbm_roofline_frac": value * STEP_ALGO_BYTES / HBM_PEAK,
             "kernels": kern,
+            "tile_variants": {"hydro": tiles[0], "thermal": tiles[1]} if tiles else None,
             "state_finite": finite,
+            "batched": batched,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    t.close()
     return 0
 
 

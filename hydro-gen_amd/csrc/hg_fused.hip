@@ -33,14 +33,14 @@ __device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * 
 // (rain -> flux -> erosion at q).  Used for back-trace taps outside the halo.
 // ---------------------------------------------------------------------------
 __device__ void eroded_sediment_global(const HydroIO& io, int qx, int qy, int nx, int ny,
-                                       uint32_t step, bool rain, const KParams& k, float& s0o,
-                                       float& s1o) {
+                                       uint32_t step, uint32_t terrain, bool rain, const KParams& k,
+                                       float& s0o, float& s1o) {
     const float* R = io.r;
     const float* D = io.d;
     const float* B = io.b;
     auto water = [&](int x, int y) -> float {
         float b = B[at(x, y, nx)];
-        if (rain) b = b + rain_incr(x, y, step, 0, k);
+        if (rain) b = b + rain_incr(x, y, step, terrain, k);
         return b;
     };
     auto total = [&](int x, int y) -> float {
@@ -93,12 +93,14 @@ struct HydroTile {
     static_assert(TX % 4 == 0, "tile width must be a multiple of 4");
 };
 
+// One output tile (bx, by) of the hydro kernel; `lds` holds 6*NLDS floats.
+// Ends with LDS still in use: callers looping over tiles must barrier first.
 template <int TX, int TY, int HA, int NT>
-__global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, uint32_t step,
-                                                    int rain, KParams k) {
+__device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx, int ny, int bx,
+                                           int by, uint32_t step, uint32_t terrain, int rain,
+                                           const KParams& k) {
     using T = HydroTile<TX, TY, HA, NT>;
     // LDS planes (pitch WI): rd = rock+dirt, b = water after rain, fl..fb = old flux.
-    __shared__ float lds[6 * T::NLDS];
     float* s_rd = lds;
     float* s_b = lds + T::NLDS;
     float* s_fl = lds + 2 * T::NLDS;
@@ -106,8 +108,12 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, 
     float* s_ft = lds + 4 * T::NLDS;
     float* s_fb = lds + 5 * T::NLDS;
 
-    const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    // Opaque to the optimiser: values derived from the thread index are then
+    // recomputed per tile instead of being hoisted out of the batched tile
+    // loop and held in registers across tiles.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int x0 = bx * TX, y0 = by * TY;
     const int lx0 = x0 - T::OX, ly0 = y0 - T::OY;   // grid coords of LDS (0,0)
 
     // ---- stage 0: stage inputs in aligned quads; rain is added here -------
@@ -138,7 +144,7 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, 
                     float inc[4];
                     if (k.rain_mode == HG_RAIN_UNIFORM) {
                         uint32_t o[4];
-                        philox4x32_10(uint32_t(gx) >> 2, uint32_t(gy), step, 0u, k.seed_lo, k.seed_hi, o);
+                        philox4x32_10(uint32_t(gx) >> 2, uint32_t(gy), step, terrain, k.seed_lo, k.seed_hi, o);
 #pragma unroll
                         for (int j = 0; j < 4; ++j) inc[j] = k.rain_amount * u24(o[j]);
                     } else {
@@ -253,15 +259,22 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, 
             a0[0] = s_s0[e00]; a0[1] = s_s0[e10]; a0[2] = s_s0[e01]; a0[3] = s_s0[e11];
             a1[0] = s_s1[e00]; a1[1] = s_s1[e10]; a1[2] = s_s1[e01]; a1[3] = s_s1[e11];
         } else {
-            eroded_sediment_global(io, ix, iy, nx, ny, step, rain != 0, k, a0[0], a1[0]);
-            eroded_sediment_global(io, ix1, iy, nx, ny, step, rain != 0, k, a0[1], a1[1]);
-            eroded_sediment_global(io, ix, iy1, nx, ny, step, rain != 0, k, a0[2], a1[2]);
-            eroded_sediment_global(io, ix1, iy1, nx, ny, step, rain != 0, k, a0[3], a1[3]);
+            eroded_sediment_global(io, ix, iy, nx, ny, step, terrain, rain != 0, k, a0[0], a1[0]);
+            eroded_sediment_global(io, ix1, iy, nx, ny, step, terrain, rain != 0, k, a0[1], a1[1]);
+            eroded_sediment_global(io, ix, iy1, nx, ny, step, terrain, rain != 0, k, a0[2], a1[2]);
+            eroded_sediment_global(io, ix1, iy1, nx, ny, step, terrain, rain != 0, k, a0[3], a1[3]);
         }
         const size_t g = at(gx, gy, nx);
         io.s0o[g] = lerp1(lerp1(a0[0], a0[1], fx), lerp1(a0[2], a0[3], fx), fy);
         io.s1o[g] = lerp1(lerp1(a1[0], a1[1], fx), lerp1(a1[2], a1[3], fx), fy);
     }
+}
+
+template <int TX, int TY, int HA, int NT>
+__global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, uint32_t step,
+                                                    uint32_t terrain, int rain, KParams k) {
+    __shared__ float lds[6 * HydroTile<TX, TY, HA, NT>::NLDS];
+    hydro_tile<TX, TY, HA, NT>(lds, io, nx, ny, blockIdx.x, blockIdx.y, step, terrain, rain, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -296,13 +309,25 @@ __device__ __forceinline__ void region_for(int tid, F&& fn) {
     }
 }
 
+template <int TX, int TY>
+struct ThermalTile {
+    static constexpr int W0 = TX + 2 * TH, N0 = W0 * (TY + 2 * TH);
+};
+
+// One output tile (bx, by) of the thermal kernel; `lds` holds 3*N0 floats.
 template <int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_fused_thermal(ThermIO io, int nx, int ny, int do_thermal,
-                                                      int do_smooth, KParams k) {
-    constexpr int W0 = TX + 2 * TH, N0 = W0 * (TY + 2 * TH);
-    __shared__ float s_r[N0], s_d[N0], s_g[N0];
-    const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TX - TH, y0 = blockIdx.y * TY - TH;   // grid coords of LDS (0,0)
+__device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int nx, int ny, int bx,
+                                             int by, int do_thermal, int do_smooth, const KParams& k) {
+    constexpr int W0 = ThermalTile<TX, TY>::W0, N0 = ThermalTile<TX, TY>::N0;
+    float* s_r = lds;
+    float* s_d = lds + N0;
+    float* s_g = lds + 2 * N0;
+    // Opaque to the optimiser: values derived from the thread index are then
+    // recomputed per tile instead of being hoisted out of the batched tile
+    // loop and held in registers across tiles.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int x0 = bx * TX - TH, y0 = by * TY - TH;   // grid coords of LDS (0,0)
     constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
     constexpr int OFF[8] = {-1, 1, W0, -W0, W0 - 1, W0 + 1, -W0 - 1, -W0 + 1};
@@ -415,6 +440,80 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     });
 }
 
+template <int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_fused_thermal(
+    ThermIO io, int nx, int ny, int do_thermal, int do_smooth, KParams k) {
+    __shared__ float lds[3 * ThermalTile<TX, TY>::N0];
+    thermal_tile<TX, TY, NT>(lds, io, nx, ny, blockIdx.x, blockIdx.y, do_thermal, do_smooth, k);
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent kernels: one launch per pass covers the tiles of every
+// terrain of a batch.  Workgroup b processes work-list entries b, b+G, ...
+// (G = grid size); an entry packs (terrain << 24) | (tile_y << 12) | tile_x.
+// Each terrain's descriptor holds both buffers of its 9 state planes; bit p of
+// `cur` selects which buffer of plane p is the current (input) one.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ HydroIO batch_hydro_io(const BatchTerrain& T, uint32_t cur) {
+    HydroIO io;
+    const float** in[9] = {&io.r, &io.d, &io.b, &io.fl, &io.fr, &io.ft, &io.fb, &io.s0, &io.s1};
+    float** out[9] = {&io.ro, &io.dout, &io.bo, &io.flo, &io.fro, &io.fto, &io.fbo, &io.s0o, &io.s1o};
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const int c = (cur >> p) & 1;
+        *in[p] = T.plane[p][c];
+        *out[p] = T.plane[p][c ^ 1];
+    }
+    io.vx = io.vy = io.vz = nullptr;
+    return io;
+}
+
+template <int TX, int TY, int HA, int NT>
+__global__ __launch_bounds__(NT) void k_batch_hydro(const BatchTerrain* __restrict__ terr,
+                                                    const uint32_t* __restrict__ tiles,
+                                                    int ntiles, uint32_t cur, const uint32_t* step_ptr,
+                                                    uint32_t step_off, int rain) {
+    __shared__ float lds[6 * HydroTile<TX, TY, HA, NT>::NLDS];
+    const uint32_t step = *step_ptr + step_off;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t e = tiles[t];
+        if (e == 0xffffffffu) continue;
+        const int ti = int(e >> 24), by = int((e >> 12) & 0xfffu), bx = int(e & 0xfffu);
+        // copy the descriptor into registers: a reference into global memory
+        // would be re-read after every global store of the tile
+        const BatchTerrain& T = terr[ti];
+        const HydroIO io = batch_hydro_io(T, cur);
+        const KParams k = T.k;
+        const int n = T.n;
+        hydro_tile<TX, TY, HA, NT>(lds, io, n, n, bx, by, step, uint32_t(ti), rain, k);
+        __syncthreads();
+    }
+}
+
+template <int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_thermal(
+    const BatchTerrain* __restrict__ terr, const uint32_t* __restrict__ tiles, int ntiles,
+    uint32_t cur, int do_thermal,
+    int do_smooth) {
+    __shared__ float lds[3 * ThermalTile<TX, TY>::N0];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t e = tiles[t];
+        if (e == 0xffffffffu) continue;
+        const int ti = int(e >> 24), by = int((e >> 12) & 0xfffu), bx = int(e & 0xfffu);
+        const BatchTerrain& T = terr[ti];
+        ThermIO io;
+        const int cr = cur & 1, cd = (cur >> 1) & 1;
+        io.r = T.plane[HG_P_ROCK][cr];
+        io.ro = T.plane[HG_P_ROCK][cr ^ 1];
+        io.d = T.plane[HG_P_DIRT][cd];
+        io.dout = T.plane[HG_P_DIRT][cd ^ 1];
+        const KParams k = T.k;
+        const int n = T.n;
+        thermal_tile<TX, TY, NT>(lds, io, n, n, bx, by, do_thermal, do_smooth, k);
+        __syncthreads();
+    }
+}
+
 // Compiled tile variants (the autotuner picks among them per grid size).
 //                      TX   TY  HA   NT
 #define HG_HYDRO_VARIANTS(X) \
@@ -468,13 +567,13 @@ bool thermal_variant_info(int v, int* tx, int* ty, int* halo, int* nt) {
     }
 }
 
-hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step, bool rain,
-                              const KParams& k, hipStream_t s) {
+hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step,
+                              uint32_t terrain, bool rain, const KParams& k, hipStream_t s) {
     switch (variant) {
 #define X(i, TX_, TY_, HA_, NT_)                                                              \
     case i: {                                                                                 \
         const dim3 grid((nx + TX_ - 1) / TX_, (ny + TY_ - 1) / TY_);                          \
-        k_fused_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, step, rain ? 1 : 0, k); \
+        k_fused_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, rain ? 1 : 0, k); \
         break;                                                                                \
     }
         HG_HYDRO_VARIANTS(X)
@@ -482,6 +581,78 @@ hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, ui
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_batch_hydro(int variant, const BatchTerrain* terr, const uint32_t* tiles, int ntiles,
+                              int grid, uint32_t cur, const uint32_t* step_ptr, uint32_t step_off,
+                              bool rain, hipStream_t s) {
+    switch (variant) {
+#define X(i, TX_, TY_, HA_, NT_)                                                                   \
+    case i:                                                                                        \
+        k_batch_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur, step_ptr, \
+                                                                step_off, rain ? 1 : 0);        \
+        break;
+        HG_HYDRO_VARIANTS(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_thermal(int variant, const BatchTerrain* terr, const uint32_t* tiles, int ntiles,
+                                int grid, uint32_t cur, bool thermal, bool smooth, hipStream_t s) {
+    switch (variant) {
+#define X(i, TX_, TY_, NT_)                                                                   \
+    case i:                                                                                   \
+        k_batch_thermal<TX_, TY_, NT_><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur,          \
+                                                             thermal ? 1 : 0, smooth ? 1 : 0); \
+        break;
+        HG_THERMAL_VARIANTS(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_step_advance(uint32_t* step_ptr, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_ptr += n;
+}
+
+hipError_t launch_step_advance(uint32_t* step_ptr, uint32_t n, hipStream_t s) {
+    k_step_advance<<<1, 64, 0, s>>>(step_ptr, n);
+    return hipGetLastError();
+}
+
+int batch_hydro_blocks_per_cu(int variant) {
+    int nb = 0;
+    switch (variant) {
+#define X(i, TX_, TY_, HA_, NT_)                                                                  \
+    case i:                                                                                       \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_hydro<TX_, TY_, HA_, NT_>, \
+                                                         NT_, 0) != hipSuccess)                  \
+            nb = 1;                                                                               \
+        break;
+        HG_HYDRO_VARIANTS(X)
+#undef X
+        default: nb = 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
+int batch_thermal_blocks_per_cu(int variant) {
+    int nb = 0;
+    switch (variant) {
+#define X(i, TX_, TY_, NT_)                                                                   \
+    case i:                                                                                   \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_thermal<TX_, TY_, NT_>, \
+                                                         NT_, 0) != hipSuccess)              \
+            nb = 1;                                                                           \
+        break;
+        HG_THERMAL_VARIANTS(X)
+#undef X
+        default: nb = 1;
+    }
+    return nb > 0 ? nb : 1;
 }
 
 hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,

@@ -31,7 +31,7 @@ __device__ __forceinline__ float rain_incr(int x, int y, uint32_t step, uint32_t
 
 // per-pass kernels (hg_pass.hip): one per reference dispatch
 hipError_t launch_pass(int pass, const Planes& in, const Planes& out, int nx, int ny, uint32_t step,
-                       const KParams& k, hipStream_t s);
+                       uint32_t terrain, const KParams& k, hipStream_t s);
 
 // fused kernels (hg_fused.hip)
 struct HydroIO {
@@ -47,10 +47,26 @@ int hydro_variant_count();
 int thermal_variant_count();
 bool hydro_variant_info(int v, int* tx, int* ty, int* halo, int* nt);
 bool thermal_variant_info(int v, int* tx, int* ty, int* halo, int* nt);
-hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step, bool rain,
-                              const KParams& k, hipStream_t s);
+hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step,
+                              uint32_t terrain, bool rain, const KParams& k, hipStream_t s);
 hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,
                                 bool smooth, const KParams& k, hipStream_t s);
+
+// batched persistent kernels (hg_fused.hip)
+struct BatchTerrain {
+    float* plane[HG_NUM_STATE_PLANES][2];   // both buffers of each state plane
+    int n;
+    KParams k;
+};
+hipError_t launch_batch_hydro(int variant, const BatchTerrain* terr, const uint32_t* tiles, int ntiles,
+                              int grid, uint32_t cur, const uint32_t* step_ptr, uint32_t step_off,
+                              bool rain, hipStream_t s);
+// *step_ptr += n (a kernel, so it can sit in a captured graph)
+hipError_t launch_step_advance(uint32_t* step_ptr, uint32_t n, hipStream_t s);
+hipError_t launch_batch_thermal(int variant, const BatchTerrain* terr, const uint32_t* tiles, int ntiles,
+                                int grid, uint32_t cur, bool thermal, bool smooth, hipStream_t s);
+int batch_hydro_blocks_per_cu(int variant);
+int batch_thermal_blocks_per_cu(int variant);
 
 // heightmap init (hg_init.hip)
 void fbm_permutation(uint64_t seed, uint8_t perm[256]);

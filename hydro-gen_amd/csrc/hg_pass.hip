@@ -16,11 +16,11 @@ constexpr int BX = 64, BY = 4;
 __device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * size_t(nx) + size_t(x); }
 
 __global__ __launch_bounds__(BX * BY) void k_rain(Planes in, Planes out, int nx, int ny,
-                                                  uint32_t step, KParams k) {
+                                                  uint32_t step, uint32_t terrain, KParams k) {
     const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
     if (x >= nx || y >= ny) return;
     const size_t i = at(x, y, nx);
-    out.p[HG_P_WATER][i] = in.p[HG_P_WATER][i] + rain_incr(x, y, step, 0, k);
+    out.p[HG_P_WATER][i] = in.p[HG_P_WATER][i] + rain_incr(x, y, step, terrain, k);
 }
 
 // glsl/hydro_flux.glsl:77-166
@@ -165,10 +165,10 @@ __global__ __launch_bounds__(BX * BY) void k_smooth(Planes in, Planes out, int n
 }  // namespace
 
 hipError_t launch_pass(int pass, const Planes& in, const Planes& out, int nx, int ny, uint32_t step,
-                       const KParams& k, hipStream_t s) {
+                       uint32_t terrain, const KParams& k, hipStream_t s) {
     const dim3 block(BX, BY), grid((nx + BX - 1) / BX, (ny + BY - 1) / BY);
     switch (pass) {
-        case HG_PASS_RAIN: k_rain<<<grid, block, 0, s>>>(in, out, nx, ny, step, k); break;
+        case HG_PASS_RAIN: k_rain<<<grid, block, 0, s>>>(in, out, nx, ny, step, terrain, k); break;
         case HG_PASS_FLUX: k_flux<<<grid, block, 0, s>>>(in, out, nx, ny, k); break;
         case HG_PASS_EROSION: k_erosion<<<grid, block, 0, s>>>(in, out, nx, ny, k); break;
         case HG_PASS_TRANSPORT: k_transport<<<grid, block, 0, s>>>(in, out, nx, ny, k); break;

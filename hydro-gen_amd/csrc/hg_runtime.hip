@@ -227,7 +227,7 @@ int run_pass(hg_ctx* c, int pass) {
     const Planes in = planes_of(c->cur), out = planes_of(c->alt);
     const uint32_t step = uint32_t(c->step);
     int rc = timed_launch(c, pass_name(pass), [&] {
-        return hgk::launch_pass(pass, in, out, c->n, c->n, step, c->kp, c->stream);
+        return hgk::launch_pass(pass, in, out, c->n, c->n, step, 0u, c->kp, c->stream);
     });
     if (rc != HG_OK) return rc;
     switch (pass) {
@@ -264,7 +264,7 @@ int fused_step(hg_ctx* c, bool rain, bool thermal, bool smooth, bool keep_veloci
     io.vz = keep_velocity ? c->alt[HG_P_VEL_DEPTH] : nullptr;
     const uint32_t step = uint32_t(c->step);
     int rc = timed_launch(c, "fused_hydro", [&] {
-        return hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, rain, c->kp, c->stream);
+        return hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->kp, c->stream);
     });
     if (rc != HG_OK) return rc;
     swap_planes(c, {HG_P_ROCK, HG_P_DIRT, HG_P_WATER, HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,

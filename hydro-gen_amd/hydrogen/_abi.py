@@ -24,6 +24,7 @@ CH_HEIGHT, CH_FLUX, CH_VELOCITY, CH_SEDIMENT = 0, 1, 2, 3
 NUM_PLANES = 20
 NUM_STATE_PLANES = 9
 STEP_RAIN, STEP_THERMAL, STEP_SMOOTH, STEP_PER_PASS, STEP_KEEP_VELOCITY = 0x1, 0x2, 0x4, 0x8, 0x10
+STEP_PER_TERRAIN = 0x20
 STEP_DEFAULT = STEP_RAIN | STEP_THERMAL | STEP_SMOOTH
 (PASS_RAIN, PASS_FLUX, PASS_EROSION, PASS_TRANSPORT, PASS_THERMAL_FLUX_0, PASS_THERMAL_TRANSPORT_0,
  PASS_THERMAL_FLUX_1, PASS_THERMAL_TRANSPORT_1, PASS_SMOOTH) = range(9)
@@ -36,6 +37,11 @@ EXPORTS = [
     "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_autotune",
+    "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
+    "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles",
+    "hg_batch_set_persistent", "hg_batch_set_xcd_order",
+    "hg_batch_step", "hg_batch_capture", "hg_batch_run", "hg_batch_sync", "hg_batch_get_step",
+    "hg_batch_read_plane", "hg_batch_write_plane",
 ]
 KERNEL_HYDRO, KERNEL_THERMAL = 0, 1
 
@@ -155,6 +161,25 @@ def load() -> ctypes.CDLL:
     L.hg_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hg_get_tiles.argtypes = [vp, ip, ip]
     L.hg_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ip, ip]
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.hg_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, u32p, ctypes.POINTER(vp)]
+    L.hg_batch_destroy.argtypes = [vp]
+    L.hg_batch_last_error.argtypes = [vp]
+    L.hg_batch_last_error.restype = ctypes.c_char_p
+    L.hg_batch_count.argtypes = [vp, ip]
+    L.hg_batch_set_erosion.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ErosionSettings)]
+    L.hg_batch_set_rain.argtypes = [vp, ctypes.POINTER(RainSettings), ctypes.c_int, ctypes.c_uint64]
+    L.hg_batch_init_fbm.argtypes = [vp, ctypes.c_int, ctypes.POINTER(FbmParams)]
+    L.hg_batch_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hg_batch_step.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
+    L.hg_batch_set_persistent.argtypes = [vp, ctypes.c_int]
+    L.hg_batch_set_xcd_order.argtypes = [vp, ctypes.c_int]
+    L.hg_batch_capture.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
+    L.hg_batch_run.argtypes = [vp]
+    L.hg_batch_sync.argtypes = [vp]
+    L.hg_batch_get_step.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.hg_batch_read_plane.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_size_t]
+    L.hg_batch_write_plane.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_size_t]
     _lib = L
     return L
 
@@ -324,3 +349,92 @@ def tile_variants(kernel: int) -> list:
         if L.hg_tile_variant_info(kernel, v, *[ctypes.byref(x) for x in a]) == HG_OK:
             out.append(tuple(x.value for x in a))
     return out
+
+
+class Batch:
+    """Many square terrains of uneven sides advanced in lockstep (hg_batch_*)."""
+
+    def __init__(self, sizes, device: int = 0):
+        self.lib = load()
+        self.sizes = [int(x) for x in sizes]
+        arr = (ctypes.c_uint32 * len(self.sizes))(*self.sizes)
+        h = ctypes.c_void_p()
+        rc = self.lib.hg_batch_create(device, len(self.sizes), arr, ctypes.byref(h))
+        if rc != HG_OK:
+            raise HGError(rc, self.lib.hg_batch_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != HG_OK:
+            raise HGError(rc, self.lib.hg_batch_last_error(self.h).decode())
+
+    def set_erosion(self, s: ErosionSettings, terrain: int = -1):
+        self._check(self.lib.hg_batch_set_erosion(self.h, terrain, ctypes.byref(s)))
+
+    def set_rain(self, mode: int, amount: float = 0.01, seed: int = 0):
+        r = rain_settings(amount)
+        self._check(self.lib.hg_batch_set_rain(self.h, ctypes.byref(r), mode, seed))
+
+    def init_fbm(self, terrain: int, p: FbmParams):
+        self._check(self.lib.hg_batch_init_fbm(self.h, terrain, ctypes.byref(p)))
+
+    def set_tiles(self, hydro: int = -1, thermal: int = -1):
+        self._check(self.lib.hg_batch_set_tiles(self.h, hydro, thermal))
+
+    def step(self, nsteps: int = 1, flags: int = STEP_DEFAULT):
+        self._check(self.lib.hg_batch_step(self.h, nsteps, flags))
+
+    def set_xcd_order(self, on: bool):
+        self._check(self.lib.hg_batch_set_xcd_order(self.h, 1 if on else 0))
+
+    def set_persistent(self, on: bool):
+        self._check(self.lib.hg_batch_set_persistent(self.h, 1 if on else 0))
+
+    def capture(self, nsteps: int, flags: int = STEP_DEFAULT):
+        self._check(self.lib.hg_batch_capture(self.h, nsteps, flags))
+
+    def run(self):
+        self._check(self.lib.hg_batch_run(self.h))
+
+    def sync(self):
+        self._check(self.lib.hg_batch_sync(self.h))
+
+    @property
+    def step_count(self) -> int:
+        v = ctypes.c_uint64()
+        self._check(self.lib.hg_batch_get_step(self.h, ctypes.byref(v)))
+        return v.value
+
+    def read_plane(self, terrain: int, p: int) -> np.ndarray:
+        n = self.sizes[terrain]
+        a = np.empty((n, n), dtype=np.float32)
+        self._check(self.lib.hg_batch_read_plane(self.h, terrain, p, _fptr(a), a.size))
+        return a
+
+    def write_plane(self, terrain: int, p: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self._check(self.lib.hg_batch_write_plane(self.h, terrain, p, _fptr(a), a.size))
+
+    def read_state(self, terrain: int) -> np.ndarray:
+        return np.stack([self.read_plane(terrain, p) for p in range(NUM_STATE_PLANES)])
+
+    def write_state(self, terrain: int, st: np.ndarray):
+        for p in range(NUM_STATE_PLANES):
+            self.write_plane(terrain, p, st[p])

@@ -126,6 +126,7 @@ enum {
 #define HG_STEP_SMOOTH        0x4u  /* extremum smoothing pass */
 #define HG_STEP_PER_PASS      0x8u  /* one kernel per reference pass (unfused) */
 #define HG_STEP_KEEP_VELOCITY 0x10u /* fused path also stores the velocity planes */
+#define HG_STEP_PER_TERRAIN   0x20u /* batches: launch each terrain separately (baseline) */
 #define HG_STEP_DEFAULT (HG_STEP_RAIN | HG_STEP_THERMAL | HG_STEP_SMOOTH)
 
 /* Single reference passes, for pass-by-pass parity (hg_run_pass). */
@@ -241,6 +242,45 @@ int hg_get_tiles(const hg_ctx* ctx, int* hydro_variant, int* thermal_variant);
 /* Times `steps` fused steps per variant (state and step counter are restored
  * afterwards) and selects the fastest of each kernel. */
 int hg_autotune(hg_ctx* ctx, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best);
+
+/* ---- batches of terrains -------------------------------------------------- */
+
+/* A batch holds `count` square terrains of uneven sides (multiples of 8), each
+ * with its own erosion coefficients, all advanced in lockstep.  A batched step
+ * is one persistent launch of each fused kernel over the tiles of every
+ * terrain.  The rain counter carries the terrain index (ctr word 3), so each
+ * terrain of a batch rains independently.  No counterpart in the reference,
+ * which erodes one terrain per context. */
+typedef struct hg_batch hg_batch;
+int hg_batch_create(int device, int count, const uint32_t* sizes, hg_batch** out);
+int hg_batch_destroy(hg_batch* b);
+const char* hg_batch_last_error(const hg_batch* b);
+int hg_batch_count(const hg_batch* b, int* count);
+/* terrain = -1 sets every terrain. */
+int hg_batch_set_erosion(hg_batch* b, int terrain, const hg_erosion_settings* s);
+int hg_batch_set_rain(hg_batch* b, const hg_rain_settings* s, int mode, uint64_t seed);
+int hg_batch_init_fbm(hg_batch* b, int terrain, const hg_fbm_params* p);
+int hg_batch_set_tiles(hg_batch* b, int hydro_variant, int thermal_variant);
+/* 0 (default): each batched launch has one workgroup per tile of the work-list;
+ * 1: a persistent grid of resident workgroups strides over the work-list. */
+int hg_batch_set_persistent(hg_batch* b, int persistent);
+/* 1: the work-list is cut into 8 contiguous chunks interleaved so that each
+ * XCD works through one chunk; 0 (default): natural (terrain, row, column). */
+int hg_batch_set_xcd_order(hg_batch* b, int on);
+/* nsteps steps with persistent launches.  HG_STEP_PER_TERRAIN instead launches
+ * the single-terrain fused kernels terrain by terrain; with HG_STEP_PER_PASS as
+ * well, one kernel per reference pass per terrain (the reference's dispatch
+ * pattern).  All three give identical results. */
+int hg_batch_step(hg_batch* b, uint32_t nsteps, uint32_t flags);
+/* Records nsteps (even) persistent steps into a hipGraph; hg_batch_run
+ * replays it, continuing the step counter (kept in device memory). */
+int hg_batch_capture(hg_batch* b, uint32_t nsteps, uint32_t flags);
+int hg_batch_run(hg_batch* b);
+int hg_batch_sync(hg_batch* b);
+int hg_batch_get_step(const hg_batch* b, uint64_t* step);
+/* State planes (HG_P_ROCK .. HG_P_SED_DIRT) of one terrain, n*n floats. */
+int hg_batch_read_plane(hg_batch* b, int terrain, int plane, float* dst, size_t n_floats);
+int hg_batch_write_plane(hg_batch* b, int terrain, int plane, const float* src, size_t n_floats);
 
 #ifdef __cplusplus
 }
