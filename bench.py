@@ -110,6 +110,62 @@ def bench_batched(steps: int, base_steps: int, tiles=None) -> dict:
     return out
 
 
+def bench_particles(iterations: int, oracle_iters: int) -> dict:
+    """Particle config: 2048^2 fBm (frequency 1/512) in [0, 256], 2^20 droplets
+    per iteration, 64 lifetime steps, inertia 0.05, capacity 4, deposition and
+    erosion 0.3, evaporation 0.01, gravity 4, min slope 0.01, brush radius 3.
+    Reports droplet-steps/s, the mass-balance residual, and, for the first
+    `oracle_iters` iterations, the height-histogram L1 distance to the serial
+    FP64 oracle next to that of the serial FP32 oracle (the rounding floor)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    n = 2048
+    d = hg.droplet_settings(count=1 << 20, seed=42)
+    t = hg.Terrain(n)
+    t.init_fbm(hg.fbm_params(seed=3, base_freq=1.0 / 512, hi=256.0))
+    h0 = t.read_plane(hg.P_ROCK)
+    t.set_droplets(d)
+    out = {"grid": n, "droplets_per_iteration": 1 << 20, "iterations": iterations}
+    t.dispatch_particle(oracle_iters)
+    h_early = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
+    t.sync()
+    t0 = time.perf_counter()
+    t.dispatch_particle(iterations - oracle_iters)
+    t.sync()
+    dt = time.perf_counter() - t0
+    ex, steps = t.particle_stats()
+    h = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
+    t.close()
+    m0 = float(h0.astype(np.float64).sum())
+    frac_timed = (iterations - oracle_iters) / iterations
+    out["droplet_steps"] = int(steps)
+    out["droplet_steps_per_s"] = steps * frac_timed / dt
+    out["ms_per_iteration"] = dt / (iterations - oracle_iters) * 1e3
+    out["mass_residual_rel"] = (float(h.sum()) + ex - m0) / m0
+    out["exported_mass"] = ex
+    if oracle_iters > 0:
+        od = O.droplet_settings(count=1 << 20, seed=42)
+        r64, d64 = h0.astype(np.float64), np.zeros((n, n))
+        r32, d32 = h0.copy(), np.zeros((n, n), np.float32)
+        t0 = time.perf_counter()
+        for it in range(oracle_iters):
+            O.droplets(r64, d64, od, it)
+        cpu_dt = time.perf_counter() - t0
+        for it in range(oracle_iters):
+            O.droplets(r32, d32, od, it)
+        h64, h32 = r64 + d64, r32.astype(np.float64) + d32
+
+        def l1(a, b, bins=256):
+            lo, hi = min(a.min(), b.min()), max(a.max(), b.max())
+            ha, _ = np.histogram(a, bins=bins, range=(lo, hi))
+            hb, _ = np.histogram(b, bins=bins, range=(lo, hi))
+            return float(np.abs(ha / ha.sum() - hb / hb.sum()).sum())
+        out["oracle_check"] = {"iterations": oracle_iters, "hist_l1_gpu_vs_fp64": l1(h_early, h64),
+                               "hist_l1_fp32_vs_fp64": l1(h32, h64), "hist_l1_init_vs_fp64": l1(h0, h64),
+                               "fp64_oracle_seconds": cpu_dt, "cores": 1}
+    return out
+
+
 def cpu_baseline(t: hg.Terrain, steps: int) -> dict:
     """FP32 oracle (CPU port of the same passes) on `steps` steps of the same
     4096^2 state; OpenMP threads from OMP_NUM_THREADS."""
@@ -139,6 +195,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=1000,
                     help="steps of the 64-terrain batched graph (0 = skip)")
+    ap.add_argument("--particle-iters", type=int, default=100,
+                    help="droplet iterations of the particle workload (0 = skip)")
+    ap.add_argument("--particle-oracle-iters", type=int, default=1)
     ap.add_argument("--tune", action="store_true", default=True)
     ap.add_argument("--no-tune", dest="tune", action="store_false")
     args = ap.parse_args()
@@ -184,6 +243,8 @@ def main() -> int:
     cpu = cpu_baseline(t, args.cpu_steps) if args.cpu_steps > 0 else None
     t.close()
     batched = bench_batched(args.batched_steps, 10, tiles) if args.batched_steps > 0 else None
+    particles = (bench_particles(args.particle_iters, args.particle_oracle_iters)
+                 if args.particle_iters > 0 else None)
 
     out = {
         "metric": METRIC,
@@ -208,6 +269,7 @@ def main() -> int:
             "tile_variants": {"hydro": tiles[0], "thermal": tiles[1]} if tiles else None,
             "state_finite": finite,
             "batched": batched,
+            "particles": particles,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -68,6 +68,23 @@ hipError_t launch_batch_thermal(int variant, const BatchTerrain* terr, const uin
 int batch_hydro_blocks_per_cu(int variant);
 int batch_thermal_blocks_per_cu(int variant);
 
+// droplet erosion (hg_particle.hip)
+#define HG_BRUSH_MAX 128
+struct DropletParams {
+    uint32_t count, max_lifetime;
+    float inertia, capacity, deposition, erosion, evaporation, gravity, min_slope;
+    float init_water, init_speed;
+    uint32_t seed_lo, seed_hi;
+};
+struct Brush {
+    int r, count;
+    int8_t dx[HG_BRUSH_MAX], dy[HG_BRUSH_MAX];
+    float w[HG_BRUSH_MAX];   // normalised to sum 1 over the full brush
+};
+Brush make_brush(int r);
+hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
+                           uint32_t iteration, double* exported, unsigned long long* steps, hipStream_t s);
+
 // heightmap init (hg_init.hip)
 void fbm_permutation(uint64_t seed, uint8_t perm[256]);
 hipError_t launch_fbm(float* out, int nx, int ny, const uint8_t* perm_host, int octaves,

@@ -1,0 +1,185 @@
+// hg_particle.hip — droplet (particle) erosion.
+//
+// One thread per droplet; every droplet of an iteration runs concurrently and
+// changes the height planes with float atomic adds (global_atomic_add_f32),
+// so a droplet sees the others' changes in whatever order the memory system
+// applies them.  Height reads are plain loads.  Mass bookkeeping: every amount
+// taken from the map is added to the droplet's load and every amount deposited
+// is subtracted from it; sediment carried over the map edge is summed in
+// `exported`, so sum(rock + dirt) + exported is conserved up to FP32 rounding.
+//
+// Model (Beyer-style droplets; the reference's particle mode,
+// glsl/particle.glsl:64-136 and glsl/particle_erosion.glsl:22-128, is a
+// different model): see hg_droplet_settings in include/hg.h.
+#include <cmath>
+
+#include "hg_kernels.hpp"
+
+namespace hgk {
+
+namespace {
+
+__device__ __forceinline__ void atomic_addf(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// Height reads are relaxed device-scope atomic loads: they are served by L2,
+// where the atomic adds land, and never by a stale line of the non-coherent
+// per-CU vector cache (a droplet must see its own earlier erosion).
+__device__ __forceinline__ float ld(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float height(const float* R, const float* D, size_t i) { return ld(R + i) + ld(D + i); }
+
+// Cell of a position inside [0, n-1]^2; n-1 itself maps to cell n-2 (with
+// fraction 1) so the 2x2 stencil stays in the grid.
+__device__ __forceinline__ int cell_of(float p, int n) { return min(int(p), n - 2); }
+
+// Bilinear deposit of `amount` into dirt around (px, py).
+__device__ __forceinline__ void deposit(float* D, int n, float px, float py, float amount) {
+    const int x = cell_of(px, n), y = cell_of(py, n);
+    const float u = px - float(x), v = py - float(y);
+    const size_t i = size_t(y) * n + x;
+    atomic_addf(D + i, amount * (1.0f - u) * (1.0f - v));
+    atomic_addf(D + i + 1, amount * u * (1.0f - v));
+    atomic_addf(D + i + n, amount * (1.0f - u) * v);
+    atomic_addf(D + i + n + 1, amount * u * v);
+}
+
+__global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* __restrict__ D, int n,
+                                                  DropletParams p, Brush brush, uint32_t iteration,
+                                                  double* exported_out,
+                                                  unsigned long long* steps_out) {
+    __shared__ double s_exp[256];
+    __shared__ unsigned s_steps[256];
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    double exported = 0.0;
+    unsigned steps = 0;
+    if (id < p.count) {
+        uint32_t o[4];
+        philox4x32_10(id, iteration, 0x64726f70u, 0u, p.seed_lo, p.seed_hi, o);
+        const float span = float(n - 1);
+        float px = u24(o[0]) * span, py = u24(o[1]) * span;
+        float dx = 0.0f, dy = 0.0f, speed = p.init_speed, water = p.init_water, sed = 0.0f;
+        bool alive = true;
+        for (uint32_t life = 0; life < p.max_lifetime; ++life) {
+            ++steps;
+            const int cx = cell_of(px, n), cy = cell_of(py, n);
+            const float u = px - float(cx), v = py - float(cy);
+            const size_t c = size_t(cy) * n + cx;
+            const float h00 = height(R, D, c), h10 = height(R, D, c + 1);
+            const float h01 = height(R, D, c + n), h11 = height(R, D, c + n + 1);
+            const float gx = (h10 - h00) * (1.0f - v) + (h11 - h01) * v;
+            const float gy = (h01 - h00) * (1.0f - u) + (h11 - h10) * u;
+            const float h = h00 * (1.0f - u) * (1.0f - v) + h10 * u * (1.0f - v) + h01 * (1.0f - u) * v +
+                            h11 * u * v;
+            dx = dx * p.inertia - gx * (1.0f - p.inertia);
+            dy = dy * p.inertia - gy * (1.0f - p.inertia);
+            // IEEE round-to-nearest sqrt and divide (as on the host): a droplet's
+            // path is sensitive to the last bit of its direction
+            const float len = __fsqrt_rn(dx * dx + dy * dy);
+            if (len > 0.0f) {
+                dx = __fdiv_rn(dx, len);
+                dy = __fdiv_rn(dy, len);
+            }
+            const float ox = px, oy = py;
+            if (dx == 0.0f && dy == 0.0f) {       // not moving: drop the load here
+                deposit(D, n, px, py, sed);
+                sed = 0.0f;
+                alive = false;
+                break;
+            }
+            px += dx;
+            py += dy;
+            if (!(px >= 0.0f && py >= 0.0f && px < span && py < span)) {   // over the edge (or NaN)
+                exported += double(sed);
+                sed = 0.0f;
+                alive = false;
+                break;
+            }
+            const int nx2 = cell_of(px, n), ny2 = cell_of(py, n);
+            const float u2 = px - float(nx2), v2 = py - float(ny2);
+            const size_t c2 = size_t(ny2) * n + nx2;
+            const float hn = height(R, D, c2) * (1.0f - u2) * (1.0f - v2) + height(R, D, c2 + 1) * u2 * (1.0f - v2) +
+                             height(R, D, c2 + n) * (1.0f - u2) * v2 + height(R, D, c2 + n + 1) * u2 * v2;
+            const float dh = hn - h;
+            const float cap = fmaxf(-dh, p.min_slope) * speed * water * p.capacity;
+            if (sed > cap || dh > 0.0f) {
+                const float amount = dh > 0.0f ? fminf(dh, sed) : (sed - cap) * p.deposition;
+                sed -= amount;
+                deposit(D, n, ox, oy, amount);
+            } else {
+                const float amount = fminf((cap - sed) * p.erosion, -dh);
+                const bool interior = cx >= brush.r && cy >= brush.r && cx < n - brush.r && cy < n - brush.r;
+                float wsum = 1.0f;
+                if (!interior) {
+                    wsum = 0.0f;
+                    for (int t = 0; t < brush.count; ++t) {
+                        const int x = cx + brush.dx[t], y = cy + brush.dy[t];
+                        if (x >= 0 && y >= 0 && x < n && y < n) wsum += brush.w[t];
+                    }
+                }
+                const float scale = interior ? amount : __fdiv_rn(amount, wsum);
+                for (int t = 0; t < brush.count; ++t) {
+                    const int x = cx + brush.dx[t], y = cy + brush.dy[t];
+                    if (!interior && !(x >= 0 && y >= 0 && x < n && y < n)) continue;
+                    const size_t i = size_t(y) * n + x;
+                    const float want = scale * brush.w[t];
+                    const float td = fminf(fmaxf(ld(D + i), 0.0f), want);
+                    const float tr = fminf(fmaxf(ld(R + i), 0.0f), want - td);
+                    if (td > 0.0f) atomic_addf(D + i, -td);
+                    if (tr > 0.0f) atomic_addf(R + i, -tr);
+                    sed += td + tr;
+                }
+            }
+            speed = __fsqrt_rn(fmaxf(0.0f, speed * speed - dh * p.gravity));
+            water = water * (1.0f - p.evaporation);
+        }
+        if (alive) deposit(D, n, px, py, sed);
+    }
+    s_exp[threadIdx.x] = exported;
+    s_steps[threadIdx.x] = steps;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_exp[threadIdx.x] += s_exp[threadIdx.x + s];
+            s_steps[threadIdx.x] += s_steps[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (s_exp[0] != 0.0) unsafeAtomicAdd(exported_out, s_exp[0]);
+        atomicAdd(steps_out, (unsigned long long)s_steps[0]);
+    }
+}
+
+}  // namespace
+
+Brush make_brush(int r) {
+    // weights 1 - dist/r over cells with dist < r, normalised in double and
+    // rounded to float once (the oracle builds the same table)
+    Brush b{};
+    b.r = r;
+    double raw[HG_BRUSH_MAX];
+    double sum = 0;
+    for (int dy = -r; dy <= r; ++dy)
+        for (int dx = -r; dx <= r; ++dx) {
+            const double d = std::sqrt(double(dx * dx + dy * dy));
+            if (d < r && b.count < HG_BRUSH_MAX) {
+                b.dx[b.count] = int8_t(dx);
+                b.dy[b.count] = int8_t(dy);
+                raw[b.count] = 1.0 - d / r;
+                sum += raw[b.count];
+                ++b.count;
+            }
+        }
+    for (int t = 0; t < b.count; ++t) b.w[t] = float(raw[t] / sum);
+    return b;
+}
+
+hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
+                           uint32_t iteration, double* exported, unsigned long long* steps, hipStream_t s) {
+    if (p.count == 0) return hipSuccess;
+    k_droplets<<<(p.count + 255) / 256, 256, 0, s>>>(rock, dirt, n, p, brush, iteration, exported, steps);
+    return hipGetLastError();
+}
+
+}  // namespace hgk

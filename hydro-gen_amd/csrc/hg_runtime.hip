@@ -60,6 +60,10 @@ struct hg_ctx {
     uint64_t seed = 0;
     uint64_t step = 0;
     KParams kp{};
+    hg_droplet_settings ds{};
+    uint32_t particle_iter = 0;
+    double* dev_exported = nullptr;  // cumulative droplet statistics (device)
+    unsigned long long* dev_dsteps = nullptr;
     int hydro_variant = 0;           // fused kernel tile variants (see hg_autotune)
     int thermal_variant = 0;
     std::string arch;
@@ -315,7 +319,7 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
     c->n = int(n);
     const size_t pn = size_t(n) * n;
     const size_t pitch = (pn * sizeof(float) + 255) / 256 * 256;
-    const size_t bytes = pitch * 2 * HG_NUM_PLANES + 256;
+    const size_t bytes = pitch * 2 * HG_NUM_PLANES + 256;   // + init reduction, droplet counters
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -333,6 +337,8 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
         c->alt[p] = reinterpret_cast<float*>(base + pitch * (2 * p + 1));
     }
     c->minmax = reinterpret_cast<unsigned*>(base + pitch * 2 * HG_NUM_PLANES);
+    c->dev_exported = reinterpret_cast<double*>(base + pitch * 2 * HG_NUM_PLANES + 64);
+    c->dev_dsteps = reinterpret_cast<unsigned long long*>(base + pitch * 2 * HG_NUM_PLANES + 128);
     e = hipMemsetAsync(c->block, 0, bytes, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -344,6 +350,12 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
     c->es = reference_grid_defaults();
     c->rs = reference_rain_defaults();
     update_kparams(c);
+    // droplet defaults: the particle configuration's model parameters
+    c->ds.count = particle_count;
+    c->ds.max_lifetime = 64;
+    c->ds.inertia = 0.05f; c->ds.capacity = 4.0f; c->ds.deposition = 0.3f; c->ds.erosion = 0.3f;
+    c->ds.evaporation = 0.01f; c->ds.gravity = 4.0f; c->ds.min_slope = 0.01f; c->ds.brush_radius = 3;
+    c->ds.init_water = 1.0f; c->ds.init_speed = 1.0f; c->ds.seed = 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->arch = prop.gcnArchName;
     c->hydro_variant = DEFAULT_HYDRO_VARIANT;
@@ -436,7 +448,9 @@ int hg_init_fbm(hg_ctx* c, const hg_fbm_params* p) {
     if (rc != HG_OK) return rc;
     HG_HIP(c, hgk::launch_minmax(c->cur[HG_P_ROCK], pn, c->minmax, c->stream));
     HG_HIP(c, hgk::launch_rescale(c->cur[HG_P_ROCK], pn, c->minmax, p->height_lo, p->height_hi, c->stream));
+    HG_HIP(c, hipMemsetAsync(c->dev_exported, 0, 128, c->stream));
     c->step = 0;
+    c->particle_iter = 0;
     return HG_OK;
 }
 
@@ -477,6 +491,70 @@ int hg_step_grid(hg_ctx* c, uint32_t nsteps, uint32_t flags) {
         if (rc != HG_OK) return rc;
         c->step += 1;
     }
+    return HG_OK;
+}
+
+int hg_set_droplets(hg_ctx* c, const hg_droplet_settings* s) {
+    if (!c) return HG_ERR_INVALID;
+    if (!s) return fail(c, HG_ERR_INVALID, "hg_set_droplets: settings is NULL");
+    const float v[] = {s->inertia, s->capacity, s->deposition, s->erosion, s->evaporation, s->gravity,
+                       s->min_slope, s->init_water, s->init_speed};
+    for (float x : v)
+        if (!std::isfinite(x) || x < 0.0f) return fail(c, HG_ERR_INVALID, "hg_set_droplets: parameters must be finite and >= 0");
+    if (s->inertia > 1.0f || s->evaporation > 1.0f)
+        return fail(c, HG_ERR_INVALID, "hg_set_droplets: inertia and evaporation must be <= 1");
+    if (s->brush_radius < 1 || s->brush_radius > 6)
+        return fail(c, HG_ERR_INVALID, "hg_set_droplets: brush_radius must be in [1, 6]");
+    if (s->max_lifetime > 100000) return fail(c, HG_ERR_INVALID, "hg_set_droplets: max_lifetime too large");
+    c->ds = *s;
+    return HG_OK;
+}
+
+int hg_step_particle(hg_ctx* c, uint32_t iterations, uint32_t flags) {
+    if (!c) return HG_ERR_INVALID;
+    if (flags & ~(HG_STEP_THERMAL | HG_STEP_SMOOTH)) return fail(c, HG_ERR_INVALID, "hg_step_particle: only thermal/smooth flags");
+    DeviceGuard guard(c->device);
+    hgk::DropletParams p;
+    p.count = c->ds.count; p.max_lifetime = c->ds.max_lifetime;
+    p.inertia = c->ds.inertia; p.capacity = c->ds.capacity; p.deposition = c->ds.deposition;
+    p.erosion = c->ds.erosion; p.evaporation = c->ds.evaporation; p.gravity = c->ds.gravity;
+    p.min_slope = c->ds.min_slope; p.init_water = c->ds.init_water; p.init_speed = c->ds.init_speed;
+    p.seed_lo = uint32_t(c->ds.seed); p.seed_hi = uint32_t(c->ds.seed >> 32);
+    const hgk::Brush brush = hgk::make_brush(c->ds.brush_radius);
+    const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        const uint32_t iter = c->particle_iter;
+        int rc = timed_launch(c, "droplets", [&] {
+            return hgk::launch_droplets(c->cur[HG_P_ROCK], c->cur[HG_P_DIRT], c->n, p, brush, iter,
+                                        c->dev_exported, c->dev_dsteps, c->stream);
+        });
+        if (rc != HG_OK) return rc;
+        if (thermal || smooth) {
+            hgk::ThermIO tio;
+            tio.r = c->cur[HG_P_ROCK]; tio.d = c->cur[HG_P_DIRT];
+            tio.ro = c->alt[HG_P_ROCK]; tio.dout = c->alt[HG_P_DIRT];
+            rc = timed_launch(c, "fused_thermal", [&] {
+                return hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->kp,
+                                                 c->stream);
+            });
+            if (rc != HG_OK) return rc;
+            swap_planes(c, {HG_P_ROCK, HG_P_DIRT});
+        }
+        c->particle_iter += 1;
+    }
+    return HG_OK;
+}
+
+int hg_particle_stats(hg_ctx* c, double* exported_mass, uint64_t* droplet_steps) {
+    if (!c) return HG_ERR_INVALID;
+    DeviceGuard guard(c->device);
+    double ex = 0;
+    unsigned long long st = 0;
+    HG_HIP(c, hipMemcpyAsync(&ex, c->dev_exported, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HG_HIP(c, hipMemcpyAsync(&st, c->dev_dsteps, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    if (exported_mass) *exported_mass = ex;
+    if (droplet_steps) *droplet_steps = uint64_t(st);
     return HG_OK;
 }
 

@@ -36,6 +36,7 @@ EXPORTS = [
     "hg_set_rain", "hg_init_fbm", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
     "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
+    "hg_set_droplets", "hg_step_particle", "hg_particle_stats",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_autotune",
     "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
     "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles",
@@ -111,6 +112,29 @@ def fbm_params(seed=1, octaves=6, lacunarity=2.0, gain=0.5, base_freq=1.0 / 256,
     return p
 
 
+class DropletSettings(ctypes.Structure):
+    """hg_droplet_settings (include/hg.h)."""
+    _fields_ = [("count", ctypes.c_uint32), ("max_lifetime", ctypes.c_uint32),
+                ("inertia", ctypes.c_float), ("capacity", ctypes.c_float),
+                ("deposition", ctypes.c_float), ("erosion", ctypes.c_float),
+                ("evaporation", ctypes.c_float), ("gravity", ctypes.c_float),
+                ("min_slope", ctypes.c_float), ("brush_radius", ctypes.c_int32),
+                ("init_water", ctypes.c_float), ("init_speed", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+def droplet_settings(count=1 << 20, max_lifetime=64, inertia=0.05, capacity=4.0, deposition=0.3,
+                     erosion=0.3, evaporation=0.01, gravity=4.0, min_slope=0.01, brush_radius=3,
+                     init_water=1.0, init_speed=1.0, seed=0) -> DropletSettings:
+    """Defaults: the particle configuration (2^20 droplets, 64 steps, ...)."""
+    d = DropletSettings()
+    d.count = count; d.max_lifetime = max_lifetime; d.inertia = inertia; d.capacity = capacity
+    d.deposition = deposition; d.erosion = erosion; d.evaporation = evaporation
+    d.gravity = gravity; d.min_slope = min_slope; d.brush_radius = brush_radius
+    d.init_water = init_water; d.init_speed = init_speed; d.seed = seed
+    return d
+
+
 class HGError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"hg error {code}: {msg}")
@@ -155,6 +179,9 @@ def load() -> ctypes.CDLL:
     L.hg_kernel_stats.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                   ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]
     L.hg_reset_timing.argtypes = [vp]
+    L.hg_set_droplets.argtypes = [vp, ctypes.POINTER(DropletSettings)]
+    L.hg_step_particle.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
+    L.hg_particle_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
     ip = ctypes.POINTER(ctypes.c_int)
     L.hg_tile_variant_count.argtypes = [ctypes.c_int]
     L.hg_tile_variant_info.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
@@ -246,6 +273,18 @@ class Terrain:
     # -- stepping --
     def dispatch_grid(self, nsteps: int = 1, flags: int = STEP_DEFAULT):
         self._check(self.lib.hg_step_grid(self.h, nsteps, flags))
+
+    def set_droplets(self, d: DropletSettings):
+        self._check(self.lib.hg_set_droplets(self.h, ctypes.byref(d)))
+
+    def dispatch_particle(self, iterations: int = 1, flags: int = 0):
+        """Droplet iterations (Erosion::dispatch_particle counterpart)."""
+        self._check(self.lib.hg_step_particle(self.h, iterations, flags))
+
+    def particle_stats(self):
+        ex, st = ctypes.c_double(), ctypes.c_uint64()
+        self._check(self.lib.hg_particle_stats(self.h, ctypes.byref(ex), ctypes.byref(st)))
+        return ex.value, st.value
 
     def dispatch_grid_rain(self):
         self._check(self.lib.hg_rain(self.h))

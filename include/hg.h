@@ -225,6 +225,40 @@ int hg_kernel_stats(hg_ctx* ctx, int i, const char** name, uint64_t* launches,
                     double* total_ms);
 int hg_reset_timing(hg_ctx* ctx);
 
+/* ---- particle (droplet) erosion ------------------------------------------ */
+
+/* Droplet model parameters (the configs' Beyer-style droplets; the
+ * reference's own particle mode, glsl/particle.glsl, is a different model
+ * that does not conserve mass).  Terrain height is rock + dirt; erosion takes
+ * dirt first, then rock; deposits go to dirt. */
+typedef struct hg_droplet_settings {
+    uint32_t count;          /* droplets per iteration */
+    uint32_t max_lifetime;   /* steps per droplet */
+    float inertia;
+    float capacity;
+    float deposition;
+    float erosion;
+    float evaporation;
+    float gravity;
+    float min_slope;
+    int32_t brush_radius;    /* 1..6 cells */
+    float init_water;
+    float init_speed;
+    uint64_t seed;
+} hg_droplet_settings;
+
+int hg_set_droplets(hg_ctx* ctx, const hg_droplet_settings* s);
+/* `iterations` droplet iterations (in place of Erosion::dispatch_particle,
+ * src/erosion.cpp:132-156).  Every droplet of an iteration runs in parallel
+ * and writes heights with float atomic adds, so results depend on the order
+ * the hardware applies them in.  flags: HG_STEP_THERMAL / HG_STEP_SMOOTH run
+ * the thermal and smoothing passes after each iteration, like the reference's
+ * particle dispatch. */
+int hg_step_particle(hg_ctx* ctx, uint32_t iterations, uint32_t flags);
+/* Cumulative since hg_init_fbm: mass exported over the map edge and the
+ * number of droplet steps simulated. */
+int hg_particle_stats(hg_ctx* ctx, double* exported_mass, uint64_t* droplet_steps);
+
 /* ---- tile variants and autotuning -------------------------------------- */
 
 /* The two fused kernels of a GRID step are compiled for several tile shapes

@@ -569,4 +569,138 @@ void grid_step(State<T>& s, const Settings<T>& k, const RainCfg& rc, uint64_t st
     if (flags & STEP_SMOOTH) run_pass(s, k, rc, step, PASS_SMOOTH);
 }
 
+// ---------------------------------------------------------------------------
+// Droplet (particle) erosion, the configs' Beyer-style model (H. Beyer,
+// "Implementation of a method for hydraulic erosion", 2015), restated here as
+// a serial FP64 model: droplets are simulated one after another, in id order.
+// The reference's own particle mode (glsl/particle.glsl:64-136,
+// glsl/particle_erosion.glsl:22-128) is a different, non-mass-conserving model.
+//  * start: Philox4x32-10(ctr = {id, iteration, 0x64726f70, 0}, key = seed),
+//    pos = u24(word k) * (N - 1) for k = 0 (x), 1 (y);
+//  * each lifetime step: bilinear height h and gradient g at pos; dir =
+//    dir*inertia - g*(1-inertia), normalised; pos += dir; stop when dir = 0
+//    (deposit everything at pos) or pos leaves [0, N-1)^2 (sediment leaves the
+//    map and is counted as exported); dh = h(pos_new) - h;
+//    cap = max(-dh, min_slope)*speed*water*capacity;
+//    if sed > cap or dh > 0: deposit min(dh, sed) (uphill) or
+//    (sed-cap)*deposition at the old position, bilinear weights, into dirt;
+//    else: erode min((cap-sed)*erosion, -dh) through the brush (weights
+//    1 - dist/r over cells with dist < r around the old cell, renormalised
+//    over in-map cells), taking from dirt first, then rock, never below 0;
+//    speed = sqrt(max(0, speed^2 - dh*gravity)); water *= 1 - evaporation;
+//  * after the last lifetime step the remaining sediment is deposited at pos.
+// Terrain height is rock + dirt.  Returns the exported mass.
+// ---------------------------------------------------------------------------
+struct DropletParams {
+    uint32_t count, max_lifetime;
+    double inertia, capacity, deposition, erosion, evaporation, gravity, min_slope;
+    int brush_radius;
+    double init_water, init_speed;
+    uint64_t seed;
+};
+
+struct BrushTap { int dx, dy; double w; };
+
+// Brush taps: weights 1 - dist/r over cells with dist < r, normalised (double).
+inline std::vector<BrushTap> droplet_brush(int r) {
+    std::vector<BrushTap> taps;
+    double sum = 0;
+    for (int dy = -r; dy <= r; ++dy)
+        for (int dx = -r; dx <= r; ++dx) {
+            const double d = std::sqrt(double(dx * dx + dy * dy));
+            if (d < r) { taps.push_back({dx, dy, 1.0 - d / r}); sum += 1.0 - d / r; }
+        }
+    for (auto& t : taps) t.w /= sum;
+    return taps;
+}
+
+template <class T>
+double droplets(std::vector<T>& rock, std::vector<T>& dirt, int n, const DropletParams& p, uint32_t iteration) {
+    const auto brush = droplet_brush(p.brush_radius);
+    auto H = [&](int x, int y) -> T { const size_t i = size_t(y) * n + x; return rock[i] + dirt[i]; };
+    // cell of a position in [0, n-1]; n-1 maps to cell n-2 with fraction 1
+    auto cell = [&](T q) { return std::min(int(val(q)), n - 2); };
+    auto deposit = [&](T px, T py, T amount) {
+        const int x = cell(px), y = cell(py);
+        const T u = px - T(x), v = py - T(y);
+        dirt[size_t(y) * n + x] += amount * (T(1) - u) * (T(1) - v);
+        dirt[size_t(y) * n + x + 1] += amount * u * (T(1) - v);
+        dirt[size_t(y + 1) * n + x] += amount * (T(1) - u) * v;
+        dirt[size_t(y + 1) * n + x + 1] += amount * u * v;
+    };
+    const uint32_t key[2] = {uint32_t(p.seed), uint32_t(p.seed >> 32)};
+    double exported = 0;
+    for (uint32_t id = 0; id < p.count; ++id) {
+        uint32_t ctr[4] = {id, iteration, 0x64726f70u, 0u}, o[4];
+        Philox::gen(ctr, key, o);
+        T px = T(double(o[0] >> 8) * (1.0 / 16777216.0)) * T(n - 1);
+        T py = T(double(o[1] >> 8) * (1.0 / 16777216.0)) * T(n - 1);
+        T dx = T(0), dy = T(0), speed = T(p.init_speed), water = T(p.init_water), sed = T(0);
+        bool alive = true;
+        for (uint32_t life = 0; life < p.max_lifetime; ++life) {
+            const int cx = cell(px), cy = cell(py);
+            const T u = px - T(cx), v = py - T(cy);
+            const T h00 = H(cx, cy), h10 = H(cx + 1, cy), h01 = H(cx, cy + 1), h11 = H(cx + 1, cy + 1);
+            const T gx = (h10 - h00) * (T(1) - v) + (h11 - h01) * v;
+            const T gy = (h01 - h00) * (T(1) - u) + (h11 - h10) * u;
+            const T h = h00 * (T(1) - u) * (T(1) - v) + h10 * u * (T(1) - v) + h01 * (T(1) - u) * v + h11 * u * v;
+            dx = dx * T(p.inertia) - gx * (T(1) - T(p.inertia));
+            dy = dy * T(p.inertia) - gy * (T(1) - T(p.inertia));
+            const T len = sqrt(dx * dx + dy * dy);
+            if (val(len) > 0) { dx = dx / len; dy = dy / len; }
+            const T ox = px, oy = py;
+            if (val(dx) == 0 && val(dy) == 0) { deposit(px, py, sed); sed = T(0); alive = false; break; }
+            px = px + dx;
+            py = py + dy;
+            if (!(val(px) >= 0 && val(py) >= 0 && val(px) < n - 1 && val(py) < n - 1)) {
+                exported += double(val(sed));
+                sed = T(0);
+                alive = false;
+                break;
+            }
+            const int nx2 = cell(px), ny2 = cell(py);
+            const T u2 = px - T(nx2), v2 = py - T(ny2);
+            const T hn = H(nx2, ny2) * (T(1) - u2) * (T(1) - v2) + H(nx2 + 1, ny2) * u2 * (T(1) - v2) +
+                         H(nx2, ny2 + 1) * (T(1) - u2) * v2 + H(nx2 + 1, ny2 + 1) * u2 * v2;
+            const T dh = hn - h;
+            const T cap = tmax(-dh, T(p.min_slope)) * speed * water * T(p.capacity);
+            if (val(sed) > val(cap) || val(dh) > 0) {
+                const T amount = val(dh) > 0 ? tmin(dh, sed) : (sed - cap) * T(p.deposition);
+                sed -= amount;
+                deposit(ox, oy, amount);
+            } else {
+                const T amount = tmin((cap - sed) * T(p.erosion), -dh);
+                // interior cells use the normalised weights as they are; at the
+                // map edge they are renormalised over the in-map cells
+                const bool interior = cx >= p.brush_radius && cy >= p.brush_radius &&
+                                      cx < n - p.brush_radius && cy < n - p.brush_radius;
+                T scale = amount;
+                if (!interior) {
+                    T wsum = T(0);
+                    for (const auto& b : brush) {
+                        const int x = cx + b.dx, y = cy + b.dy;
+                        if (x >= 0 && y >= 0 && x < n && y < n) wsum += T(b.w);
+                    }
+                    scale = amount / wsum;
+                }
+                for (const auto& b : brush) {
+                    const int x = cx + b.dx, y = cy + b.dy;
+                    if (!(x >= 0 && y >= 0 && x < n && y < n)) continue;
+                    const size_t i = size_t(y) * n + x;
+                    const T want = scale * T(b.w);
+                    const T td = tmin(tmax(dirt[i], T(0)), want);
+                    const T tr = tmin(tmax(rock[i], T(0)), want - td);
+                    dirt[i] -= td;
+                    rock[i] -= tr;
+                    sed += td + tr;
+                }
+            }
+            speed = sqrt(tmax(T(0), speed * speed - dh * T(p.gravity)));
+            water = water * (T(1) - T(p.evaporation));
+        }
+        if (alive) deposit(px, py, sed);
+    }
+    return exported;
+}
+
 }  // namespace orc

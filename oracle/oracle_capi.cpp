@@ -139,4 +139,28 @@ int orc_pass_f32(float* planes, int nx, int ny, const hg_erosion_settings* s, in
                       pass);
 }
 
+// One droplet iteration (serial, droplets in id order) on rock/dirt planes.
+// Returns the mass exported over the map edge.
+double orc_droplets_f64(double* rock, double* dirt, int n, const hg_droplet_settings* s, uint32_t iteration) {
+    orc::DropletParams p{s->count, s->max_lifetime, s->inertia, s->capacity, s->deposition, s->erosion,
+                         s->evaporation, s->gravity, s->min_slope, s->brush_radius, s->init_water,
+                         s->init_speed, s->seed};
+    std::vector<double> r(rock, rock + size_t(n) * n), d(dirt, dirt + size_t(n) * n);
+    const double ex = orc::droplets<double>(r, d, n, p, iteration);
+    std::memcpy(rock, r.data(), r.size() * sizeof(double));
+    std::memcpy(dirt, d.data(), d.size() * sizeof(double));
+    return ex;
+}
+
+double orc_droplets_f32(float* rock, float* dirt, int n, const hg_droplet_settings* s, uint32_t iteration) {
+    orc::DropletParams p{s->count, s->max_lifetime, s->inertia, s->capacity, s->deposition, s->erosion,
+                         s->evaporation, s->gravity, s->min_slope, s->brush_radius, s->init_water,
+                         s->init_speed, s->seed};
+    std::vector<float> r(rock, rock + size_t(n) * n), d(dirt, dirt + size_t(n) * n);
+    const double ex = orc::droplets<float>(r, d, n, p, iteration);
+    std::memcpy(rock, r.data(), r.size() * sizeof(float));
+    std::memcpy(dirt, d.data(), d.size() * sizeof(float));
+    return ex;
+}
+
 }  // extern "C"

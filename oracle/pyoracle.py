@@ -48,6 +48,36 @@ class FbmParams(ctypes.Structure):
                 ("height_hi", ctypes.c_float)]
 
 
+class DropletSettings(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint32), ("max_lifetime", ctypes.c_uint32),
+                ("inertia", ctypes.c_float), ("capacity", ctypes.c_float),
+                ("deposition", ctypes.c_float), ("erosion", ctypes.c_float),
+                ("evaporation", ctypes.c_float), ("gravity", ctypes.c_float),
+                ("min_slope", ctypes.c_float), ("brush_radius", ctypes.c_int32),
+                ("init_water", ctypes.c_float), ("init_speed", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+def droplet_settings(count=1 << 20, max_lifetime=64, inertia=0.05, capacity=4.0, deposition=0.3,
+                     erosion=0.3, evaporation=0.01, gravity=4.0, min_slope=0.01, brush_radius=3,
+                     init_water=1.0, init_speed=1.0, seed=0) -> DropletSettings:
+    d = DropletSettings()
+    d.count = count; d.max_lifetime = max_lifetime; d.inertia = inertia; d.capacity = capacity
+    d.deposition = deposition; d.erosion = erosion; d.evaporation = evaporation
+    d.gravity = gravity; d.min_slope = min_slope; d.brush_radius = brush_radius
+    d.init_water = init_water; d.init_speed = init_speed; d.seed = seed
+    return d
+
+
+def droplets(rock: np.ndarray, dirt: np.ndarray, s: DropletSettings, iteration: int) -> float:
+    """One serial droplet iteration in place (FP64 or FP32 by dtype); returns
+    the exported mass."""
+    assert rock.dtype == dirt.dtype
+    n = rock.shape[0]
+    fn = lib().orc_droplets_f64 if rock.dtype == np.float64 else lib().orc_droplets_f32
+    return fn(_ptr(rock), _ptr(dirt), n, ctypes.byref(s), iteration)
+
+
 def config_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0, slippage=0.1,
                     G=9.81, dt=0.005, Kconv=0.001, E=1.0) -> ErosionSettings:
     """The configs' scalar coefficients broadcast to both layers; the unnamed
@@ -105,6 +135,12 @@ def lib():
         L.orc_fbm_permutation.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8)]
         L.orc_simplex2_f64.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
         L.orc_simplex2_f64.restype = ctypes.c_double
+        L.orc_droplets_f64.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.c_int, ctypes.POINTER(DropletSettings), ctypes.c_uint32]
+        L.orc_droplets_f64.restype = ctypes.c_double
+        L.orc_droplets_f32.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                       ctypes.c_int, ctypes.POINTER(DropletSettings), ctypes.c_uint32]
+        L.orc_droplets_f32.restype = ctypes.c_double
         for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
             p = ctypes.POINTER(t)
             getattr(L, f"orc_fbm_init_{name}").argtypes = [p, ctypes.c_int, ctypes.c_int,
