@@ -78,27 +78,51 @@ __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x);
 //  wall      : bit0 x==0, bit1 x==N-1, bit2 y==0, bit3 y==N-1
 // Returns the new water d2; writes velocity (vx, vy, vz).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float flux_cell(float f[4], float d1, float w, const float wn[4],
-                                           const float fin[4], unsigned wall, const KParams& k,
-                                           float& vx, float& vy, float& vz) {
+// Intermediates of flux_cell kept for the adjoint.
+struct FluxTape {
+    float f1[4];      // outflow after the max(0, .) and the walls, before K
+    unsigned act;     // bit i: outflow i was positive (not clamped, not a wall)
+    float S;          // sum of f1
+    float q;          // d1 / (S*dt); K = min(1, q)
+    float arg;        // d1 + dt*(sum_in - sum_out), d2 = max(0, arg)
+    float inv;        // 1 / (d1 + d2) when d1 + d2 > 0
+};
+
+__device__ __forceinline__ float flux_cell_t(float f[4], float d1, float w, const float wn[4],
+                                             const float fin[4], unsigned wall, const KParams& k,
+                                             float& vx, float& vy, float& vz, FluxTape& tp) {
     const float fo0 = f[0], fo1 = f[1], fo2 = f[2], fo3 = f[3];
     const float c = k.dt * k.G;  // dt*A*G/L with A = L = 1
+    tp.act = 0u;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) f[i] = fmaxf(0.0f, k.E * f[i] + c * (w - wn[i]));   // :100-107
-    if (wall & 1u) f[0] = 0.0f; else if (wall & 2u) f[1] = 0.0f;                      // :110-119
-    if (wall & 4u) f[3] = 0.0f; else if (wall & 8u) f[2] = 0.0f;
+    for (int i = 0; i < 4; ++i) {                                                     // :100-107
+        const float f0 = k.E * f[i] + c * (w - wn[i]);
+        f[i] = fmaxf(0.0f, f0);
+        if (f0 > 0.0f) tp.act |= 1u << i;
+    }
+    if (wall & 1u) { f[0] = 0.0f; tp.act &= ~1u; }                                   // :110-119
+    else if (wall & 2u) { f[1] = 0.0f; tp.act &= ~2u; }
+    if (wall & 4u) { f[3] = 0.0f; tp.act &= ~8u; }
+    else if (wall & 8u) { f[2] = 0.0f; tp.act &= ~4u; }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tp.f1[i] = f[i];
     const float sum_in = (fin[0] + fin[1]) + (fin[2] + fin[3]);
     float sum_out = (f[0] + f[1]) + (f[2] + f[3]);
+    tp.S = sum_out;
     // K = min(1, b/(sum_out*dt)); sum_out == 0 gives K = 1 (:125)
     const float q = d1 * rcp(sum_out * k.dt);
+    tp.q = q;
     const float K = (q < 1.0f) ? q : 1.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) f[i] *= K;
     sum_out *= K;
-    const float d2 = fmaxf(0.0f, d1 + k.dt * (sum_in - sum_out));                      // :133-134
+    tp.arg = d1 + k.dt * (sum_in - sum_out);
+    const float d2 = fmaxf(0.0f, tp.arg);                                             // :133-134
     vz = d1 + d2;                                                                      // :144
+    tp.inv = 0.0f;
     if (vz > 0.0f) {                                                                   // :146-161
         const float inv = rcp(vz);
+        tp.inv = inv;
         vx = ((fin[0] - fo0) + (fo1 - fin[1])) * inv;
         vy = ((fin[3] - fo3) + (fo2 - fin[2])) * inv;
     } else {
@@ -108,58 +132,101 @@ __device__ __forceinline__ float flux_cell(float f[4], float d1, float w, const 
     return d2;
 }
 
+__device__ __forceinline__ float flux_cell(float f[4], float d1, float w, const float wn[4],
+                                           const float fin[4], unsigned wall, const KParams& k,
+                                           float& vx, float& vy, float& vz) {
+    FluxTape tp;
+    return flux_cell_t(f, d1, w, wn, fin, wall, k, vx, vy, vz, tp);
+}
+
 // ---------------------------------------------------------------------------
 // hydro_erosion (glsl/hydro_erosion.glsl:37-92) for one cell.
 // hl, hr, ht, hb: rock+dirt of the L, R, T, B neighbours (0 out of grid).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void erosion_cell(float& r, float& d, float& s0, float& s1, float vx,
-                                             float vy, float vz, float hl, float hr, float ht,
-                                             float hb, const KParams& k) {
+// Intermediates of erosion_cell kept for the adjoint.  Layer cases:
+// 0 dissolve, 1 dissolve that exhausted the layer, 2 deposit, 3 not reached.
+struct ErosionTape {
+    float ero0, sm, tt, sin_a, g2, dx, dz, cap0;
+    float c1, c0;          // transport capacity seen by dirt and rock
+    float s1c, s0c;        // sediment before the rock->dirt conversion
+    int case1, case0;
+    bool slow, t_inner;    // shallow-water damping applied; its t strictly in (0,1)
+};
+
+__device__ __forceinline__ void erosion_cell_t(float& r, float& d, float& s0, float& s1, float vx,
+                                               float vy, float vz, float hl, float hr, float ht,
+                                               float hb, const KParams& k, ErosionTape& tp) {
     float ero = sqrtf(vx * vx + vy * vy);                                              // :45-52
-    if (vz < 1e-3f) {
+    tp.ero0 = ero;
+    tp.slow = vz < 1e-3f;
+    tp.sm = 0.0f;
+    tp.tt = 0.0f;
+    tp.t_inner = false;
+    if (tp.slow) {
         const float dd = fmaxf(5e-4f, vz);
         float t = (dd - 1e-3f) * (1.0f / (5e-4f - 1e-3f));
+        tp.t_inner = t > 0.0f && t < 1.0f && vz > 5e-4f;
         t = fminf(fmaxf(t, 0.0f), 1.0f);
-        ero = ero * (1.0f - t * t * (3.0f - 2.0f * t));
+        tp.tt = t;
+        tp.sm = t * t * (3.0f - 2.0f * t);
+        ero = ero * (1.0f - tp.sm);
     }
     const float dx = hr - hl, dz = ht - hb;                                            // :23-35
     const float g2 = dx * dx + dz * dz;
     const float sin_a = sqrtf(g2 * rcp(g2 + 4.0f));                                    // :57
+    tp.dx = dx; tp.dz = dz; tp.g2 = g2; tp.sin_a = sin_a;
     const float cap0 = k.Kc * fmaxf(0.02f, sin_a) * ero;
+    tp.cap0 = cap0;
     // layer loop, dirt (1) then rock (0), :58-83
     float cap = 0.0f;
     bool stop = false;
     {
         const float c = fmaxf(0.0f, cap0 - cap);
+        tp.c1 = c;
         if (c > s1) {
             const float old = d;
             const float delta = (k.dt * k.Ks[1]) * (c - s1);
             d -= delta;
             s1 += delta;
-            if (d < 0.0f) { s1 += d; d = 0.0f; cap += old; }
-            else stop = true;
+            if (d < 0.0f) { s1 += d; d = 0.0f; cap += old; tp.case1 = 1; }
+            else { stop = true; tp.case1 = 0; }
         } else {
             const float delta = (k.dt * k.Kd[1]) * (s1 - c);
             d += delta;
             s1 -= delta;
+            tp.case1 = 2;
         }
     }
+    tp.case0 = 3;
+    tp.c0 = 0.0f;
     if (!stop) {
         const float c = fmaxf(0.0f, cap0 - cap);
+        tp.c0 = c;
         if (c > s0) {
             const float delta = (k.dt * k.Ks[0]) * (c - s0);
             r -= delta;
             s0 += delta;
-            if (r < 0.0f) { s0 += r; r = 0.0f; }
+            if (r < 0.0f) { s0 += r; r = 0.0f; tp.case0 = 1; }
+            else tp.case0 = 0;
         } else {
             const float delta = (k.dt * k.Kd[0]) * (s0 - c);
             r += delta;
             s0 -= delta;
+            tp.case0 = 2;
         }
     }
+    tp.s0c = s0;
+    tp.s1c = s1;
     const float conv = s0 * (k.Kconv * k.dt);                                          // :84-88
     s1 += conv;
     s0 -= conv;
+}
+
+__device__ __forceinline__ void erosion_cell(float& r, float& d, float& s0, float& s1, float vx,
+                                             float vy, float vz, float hl, float hr, float ht,
+                                             float hb, const KParams& k) {
+    ErosionTape tp;
+    erosion_cell_t(r, d, s0, s1, vx, vy, vz, hl, hr, ht, hb, k, tp);
 }
 
 // Semi-Lagrangian back-trace (sediment_transport.glsl:70 and :26-30): returns
@@ -182,30 +249,52 @@ __device__ __forceinline__ bool talus_marked(float D, int k, int layer, const KP
     return D > (k < 4 ? p.tan_alpha[layer] : p.thr_diag[layer]);
 }
 
-__device__ __forceinline__ float thermal_g(const float D[8], float h_layer, int layer,
-                                           const KParams& p) {
+// Intermediates of thermal_g kept for the adjoint.
+struct ThermalTape {
+    float bk, mc, md, H, rmax, sharp, S;
+    int kc, kd;            // first index of the largest cross / diagonal D
+    bool h_is_layer;       // H = h_l (else H = max(0, mc, md))
+    bool sharp_active;     // 1 + atan(rmax) - Kalpha > 1
+};
+
+__device__ __forceinline__ float thermal_g_t(const float D[8], float h_layer, int layer,
+                                             const KParams& p, ThermalTape& tp) {
     const float ta = p.tan_alpha[layer], td = p.thr_diag[layer];
     float bk = 0.0f, mc = D[0], md = D[4];
+    int kc = 0, kd = 4;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         bk += D[k] > ta ? D[k] : 0.0f;
-        mc = fmaxf(mc, D[k]);
+        if (D[k] > mc) { mc = D[k]; kc = k; }
     }
 #pragma unroll
     for (int k = 4; k < 8; ++k) {
         bk += D[k] > td ? D[k] : 0.0f;
-        md = fmaxf(md, D[k]);
+        if (D[k] > md) { md = D[k]; kd = k; }
     }
+    tp.bk = bk; tp.mc = mc; tp.md = md; tp.kc = kc; tp.kd = kd;
     if (!(bk > 0.0f)) return 0.0f;
     // H = min(h_l, max(0, max_k D_k))                                                  :46-58
-    const float H = fminf(h_layer, fmaxf(fmaxf(mc, md), 0.0f));
+    const float M = fmaxf(fmaxf(mc, md), 0.0f);
+    tp.h_is_layer = h_layer < M;
+    const float H = fminf(h_layer, M);
     // The largest marked D_k/d_k is the largest D_k/d_k overall when any direction
     // is marked (unmarked cross D <= tan a < marked diagonal D/sqrt2 and vice versa).
     const float rmax = fmaxf(mc, md * 0.70710678118654752f);
-    float sharp = fmaxf(1.0f + atanf(rmax) - p.Kalpha[layer], 1.0f);                   // :75-83
+    const float sharp0 = 1.0f + atanf(rmax) - p.Kalpha[layer];                         // :75-83
+    tp.sharp_active = sharp0 > 1.0f;
+    float sharp = fmaxf(sharp0, 1.0f);
+    tp.sharp = sharp;
     sharp = sharp * (sharp * sharp);                                                   // :92
     const float S = (p.dt * p.Kspeed[layer] * 0.5f) * sharp * H;                       // :95
+    tp.H = H; tp.rmax = rmax; tp.S = S;
     return S * rcp(bk);
+}
+
+__device__ __forceinline__ float thermal_g(const float D[8], float h_layer, int layer,
+                                           const KParams& p) {
+    ThermalTape tp;
+    return thermal_g_t(D, h_layer, layer, p, tp);
 }
 
 // Height change of cell p in direction k (thermal_transport.glsl:146-172):
@@ -220,20 +309,27 @@ __device__ __forceinline__ float thermal_term(float Dk, float g_self, float g_nb
 // ---------------------------------------------------------------------------
 // Smoothing, grid branch (glsl/smoothing.glsl:36-74,97-103), interior cell.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void smooth_cell(float& r, float& d, float lr, float lg, float rr,
-                                            float rg, float tr, float tg, float br, float bg,
-                                            float m) {
+// Extremum tests of the smoothing pass for the rock (er) and surface (eg) channels.
+__device__ __forceinline__ void smooth_flags(float r, float d, float lr, float lg, float rr, float rg,
+                                             float tr, float tg, float br, float bg, bool& er,
+                                             bool& eg) {
     const float dlr = r - lr, dlg = (d - lg) + dlr;
     const float drr = r - rr, drg = (d - rg) + drr;
     const float dtr = r - tr, dtg = (d - tg) + dtr;
     const float dbr = r - br, dbg = (d - bg) + dbr;
     const float gh = fabsf(((dlg + drg) + (dtg + dbg)) * 0.25f);
     const float rh = fabsf(((dlr + drr) + (dtr + dbr)) * 0.25f);
-    float nr = r, ng = d;
-    if (((-dlr > rh || -drr > rh) && dlr * drr > 0.0f) || ((-dtr > rh || -dbr > rh) && dtr * dbr > 0.0f))
-        nr = (r + lr + rr + tr + br) * 0.2f;
-    if (((-dlg > gh || -drg > gh) && dlg * drg > 0.0f) || ((-dtg > gh || -dbg > gh) && dtg * dbg > 0.0f))
-        ng = (d + lg + rg + tg + bg) * 0.2f;
+    er = ((-dlr > rh || -drr > rh) && dlr * drr > 0.0f) || ((-dtr > rh || -dbr > rh) && dtr * dbr > 0.0f);
+    eg = ((-dlg > gh || -drg > gh) && dlg * drg > 0.0f) || ((-dtg > gh || -dbg > gh) && dtg * dbg > 0.0f);
+}
+
+__device__ __forceinline__ void smooth_cell(float& r, float& d, float lr, float lg, float rr,
+                                            float rg, float tr, float tg, float br, float bg,
+                                            float m) {
+    bool er, eg;
+    smooth_flags(r, d, lr, lg, rr, rg, tr, tg, br, bg, er, eg);
+    const float nr = er ? (r + lr + rr + tr + br) * 0.2f : r;
+    const float ng = eg ? (d + lg + rg + tg + bg) * 0.2f : d;
     r = fmaf(m, nr - r, r);
     d = fmaf(m, ng - d, d);
 }

@@ -18,10 +18,9 @@
 #include <utility>
 #include <vector>
 
-#include "hg_kernels.hpp"
+#include "hg_context.hpp"
 
-using hgk::KParams;
-using hgk::Planes;
+using namespace hgctx;
 
 namespace {
 
@@ -33,50 +32,11 @@ std::map<std::pair<int, std::string>, std::pair<int, int>> g_tune_cache;
 constexpr int DEFAULT_HYDRO_VARIANT = 0;
 constexpr int DEFAULT_THERMAL_VARIANT = 1;
 
-struct KernelStat {
-    std::string name;
-    uint64_t launches = 0;
-    double total_ms = 0.0;
-};
-
-struct PendingTiming {
-    hipEvent_t start, end;
-    int stat;
-};
-
 }  // namespace
 
-struct hg_ctx {
-    int device = 0;
-    int n = 0;
-    hipStream_t stream = nullptr;
-    void* block = nullptr;           // one allocation for all plane buffers
-    float* cur[HG_NUM_PLANES] = {};
-    float* alt[HG_NUM_PLANES] = {};
-    unsigned* minmax = nullptr;      // 2 words for the init reduction
-    hg_erosion_settings es{};
-    hg_rain_settings rs{};
-    int rain_mode = HG_RAIN_OFF;
-    uint64_t seed = 0;
-    uint64_t step = 0;
-    KParams kp{};
-    hg_droplet_settings ds{};
-    uint32_t particle_iter = 0;
-    double* dev_exported = nullptr;  // cumulative droplet statistics (device)
-    unsigned long long* dev_dsteps = nullptr;
-    int hydro_variant = 0;           // fused kernel tile variants (see hg_autotune)
-    int thermal_variant = 0;
-    std::string arch;
-    std::string err;
-    // timing
-    bool timing = false;
-    std::vector<KernelStat> stats;
-    std::vector<PendingTiming> pending;
-    std::vector<hipEvent_t> event_pool;
-    std::vector<float> host_tmp;
-};
 
-namespace {
+
+namespace hgctx {
 
 int fail(hg_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg; else g_create_error = msg;
@@ -87,23 +47,6 @@ int hip_fail(hg_ctx* c, hipError_t e, const char* what) {
     return fail(c, HG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HG_HIP(ctx, call)                                       \
-    do {                                                        \
-        hipError_t e_ = (call);                                 \
-        if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
-    } while (0)
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
 
 size_t plane_n(const hg_ctx* c) { return size_t(c->n) * size_t(c->n); }
 
@@ -189,29 +132,6 @@ int resolve_timing(hg_ctx* c) {
         c->event_pool.push_back(p.end);
     }
     c->pending.clear();
-    return HG_OK;
-}
-
-// Launch helper: brackets the launch with events when timing is on.
-template <class F>
-int timed_launch(hg_ctx* c, const char* name, F&& launch) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
-        e0 = take_event(c);
-        e1 = take_event(c);
-        if (!e0 || !e1) return fail(c, HG_ERR_HIP, "hipEventCreate failed");
-        HG_HIP(c, hipEventRecord(e0, c->stream));
-    }
-    const hipError_t e = launch();
-    if (e != hipSuccess) return hip_fail(c, e, name);
-    if (c->timing) {
-        HG_HIP(c, hipEventRecord(e1, c->stream));
-        c->pending.push_back(PendingTiming{e0, e1, stat_index(c, name)});
-        if (c->pending.size() > 4096) {
-            const int rc = resolve_timing(c);
-            if (rc != HG_OK) return rc;
-        }
-    }
     return HG_OK;
 }
 

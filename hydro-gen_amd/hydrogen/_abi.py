@@ -36,7 +36,7 @@ EXPORTS = [
     "hg_set_rain", "hg_init_fbm", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
     "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
-    "hg_set_droplets", "hg_step_particle", "hg_particle_stats",
+    "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_autotune",
     "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
     "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles",
@@ -180,6 +180,8 @@ def load() -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]
     L.hg_reset_timing.argtypes = [vp]
     L.hg_set_droplets.argtypes = [vp, ctypes.POINTER(DropletSettings)]
+    L.hg_adjoint_rollout.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, fp, fp,
+                                     ctypes.POINTER(ctypes.c_double)]
     L.hg_step_particle.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.hg_particle_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
     ip = ctypes.POINTER(ctypes.c_int)
@@ -273,6 +275,16 @@ class Terrain:
     # -- stepping --
     def dispatch_grid(self, nsteps: int = 1, flags: int = STEP_DEFAULT):
         self._check(self.lib.hg_step_grid(self.h, nsteps, flags))
+
+    def adjoint_rollout(self, nsteps: int, k_ckpt: int, dL_dH_final: np.ndarray,
+                        flags: int = STEP_DEFAULT):
+        """Run nsteps from the current state and return (dL/dH0, dL/d(Kc, Ks, Kd, Ke))."""
+        g = np.ascontiguousarray(dL_dH_final, dtype=np.float32)
+        assert g.shape == (self.n, self.n)
+        out = np.empty((self.n, self.n), dtype=np.float32)
+        coef = (ctypes.c_double * 4)()
+        self._check(self.lib.hg_adjoint_rollout(self.h, nsteps, k_ckpt, flags, _fptr(g), _fptr(out), coef))
+        return out, np.array(list(coef))
 
     def set_droplets(self, d: DropletSettings):
         self._check(self.lib.hg_set_droplets(self.h, ctypes.byref(d)))

@@ -259,6 +259,21 @@ int hg_step_particle(hg_ctx* ctx, uint32_t iterations, uint32_t flags);
  * number of droplet steps simulated. */
 int hg_particle_stats(hg_ctx* ctx, double* exported_mass, uint64_t* droplet_steps);
 
+/* ---- adjoint -------------------------------------------------------------- */
+
+/* Gradient of a GRID rollout (no counterpart in the reference).  Runs nsteps
+ * steps (flags: rain / thermal / smoothing) from the context's current state,
+ * saving a checkpoint every k_ckpt steps, then the hand-written reverse sweep.
+ * dL_dH_final (n*n host floats) is the gradient of the caller's loss with
+ * respect to the final terrain height rock + dirt.  Outputs: dL_dH0 (n*n host
+ * floats), the gradient with respect to the initial rock height, and
+ * dL_dcoef[4], with respect to (Kc, Ks, Kd, Ke) (Ks and Kd shared by both
+ * layers).  Clamps, max/min, the talus test and the erosion cases contribute
+ * the derivative of the branch the forward run took.  The context is left at
+ * the final state, its step counter advanced by nsteps. */
+int hg_adjoint_rollout(hg_ctx* ctx, uint32_t nsteps, uint32_t k_ckpt, uint32_t flags,
+                       const float* dL_dH_final, float* dL_dH0, double* dL_dcoef);
+
 /* ---- tile variants and autotuning -------------------------------------- */
 
 /* The two fused kernels of a GRID step are compiled for several tile shapes

@@ -33,6 +33,8 @@ namespace orc {
 // ---------------------------------------------------------------------------
 inline double val(double x) { return x; }
 inline float val(float x) { return x; }
+template <int N> struct Dual;
+template <int N> double val(const Dual<N>& x);
 
 template <class T> inline T tmax(T a, T b) { return val(a) < val(b) ? b : a; }   // GLSL max(a,b)
 template <class T> inline T tmin(T a, T b) { return val(b) < val(a) ? b : a; }   // GLSL min(a,b)
@@ -311,7 +313,7 @@ template <class T> void erosion_cell(T& r, T& d, T& s0, T& s1, T vx, T vy, T vz,
     // :45-52
     T dd = vz;
     T ero_vel = sqrt(vx * vx + vy * vy);
-    if (val(dd) < T(1e-3f)) {
+    if (val(dd) < double(1e-3f)) {
         dd = tmax(T(5e-4f), dd);
         T t = tclamp((dd - T(1e-3f)) / (T(5e-4f) - T(1e-3f)), T(0), T(1));
         T sm = t * t * (T(3) - T(2) * t);

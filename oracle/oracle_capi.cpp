@@ -11,6 +11,7 @@
 #include <cstring>
 #include <vector>
 #include "hg_oracle.hpp"
+#include "hg_dual.hpp"
 #include "../include/hg.h"
 
 static_assert(sizeof(hg_erosion_settings) == 96, "Erosion_data is 96 bytes in std140");
@@ -67,6 +68,32 @@ int run(T* planes, int nx, int ny, const hg_erosion_settings* s, int rain_mode, 
     store(st, planes);
     return 0;
 }
+// Loss of a rollout: L = mean over cells of (rock + dirt - target)^2 after
+// nsteps GRID steps from `planes`, with (Kc, Ks, Kd, Ke) = coef (Ks, Kd for
+// both layers) and the other settings from s.
+template <class T>
+T rollout_loss(orc::State<T>& st, const hg_erosion_settings* s, const T coef[4], int rain_mode,
+               double rain_amount, uint64_t seed, int nsteps, unsigned flags, const double* target) {
+    orc::Settings<T> k = to_settings<T>(s);
+    k.Kc = coef[0];
+    k.Ks[0] = coef[1]; k.Ks[1] = coef[1];
+    k.Kd[0] = coef[2]; k.Kd[1] = coef[2];
+    k.Ke = coef[3];
+    orc::RainCfg rc;
+    rc.mode = rain_mode;
+    rc.amount = rain_amount;
+    rc.seed = seed;
+    for (int i = 0; i < nsteps; ++i) orc::grid_step(st, k, rc, uint64_t(i), flags);
+    const size_t n = size_t(st.nx) * st.ny;
+    T acc = T(0);
+    for (size_t i = 0; i < n; ++i) {
+        const T e = st.r[i] + st.d[i] - T(target[i]);
+        acc += e * e;
+    }
+    return acc / T(double(n));
+}
+
+constexpr int DUAL_N = 36;   // 4 coefficients + up to 32 cells
 }  // namespace
 
 extern "C" {
@@ -161,6 +188,44 @@ double orc_droplets_f32(float* rock, float* dirt, int n, const hg_droplet_settin
     std::memcpy(rock, r.data(), r.size() * sizeof(float));
     std::memcpy(dirt, d.data(), d.size() * sizeof(float));
     return ex;
+}
+
+// FP64 rollout loss (see rollout_loss); planes are the initial state.
+int orc_rollout_loss_f64(const double* planes, int nx, int ny, const hg_erosion_settings* s,
+                         const double coef[4], int rain_mode, float rain_amount, uint64_t seed,
+                         int nsteps, unsigned flags, const double* target, double* loss) {
+    if (!planes || !s || !coef || !target || !loss || nx < 2 || ny < 2) return -1;
+    orc::State<double> st;
+    load(st, planes, nx, ny);
+    *loss = rollout_loss<double>(st, s, coef, rain_mode, double(rain_amount), seed, nsteps, flags, target);
+    return 0;
+}
+
+// Dual-number FP64 rollout: the loss and its derivatives with respect to
+// (Kc, Ks, Kd, Ke) and to the initial rock height at `ncells` cells
+// (grad[0..3], then grad[4 + i] for cells[i]).
+int orc_rollout_grad_dual(const double* planes, int nx, int ny, const hg_erosion_settings* s,
+                          const double coef[4], int rain_mode, float rain_amount, uint64_t seed,
+                          int nsteps, unsigned flags, const double* target, const int* cells,
+                          int ncells, double* loss, double* grad) {
+    using D = orc::Dual<DUAL_N>;
+    if (!planes || !s || !coef || !target || !loss || !grad || ncells < 0 || ncells > DUAL_N - 4) return -1;
+    orc::State<D> st;
+    st.resize(nx, ny);
+    const size_t n = size_t(nx) * ny;
+    auto ps = st.planes();
+    for (int p = 0; p < ORC_NUM_PLANES; ++p)
+        for (size_t i = 0; i < n; ++i) (*ps[p])[i] = D(planes[p * n + i]);
+    for (int c = 0; c < ncells; ++c) {
+        if (cells[c] < 0 || size_t(cells[c]) >= n) return -1;
+        st.r[size_t(cells[c])].d[4 + c] = 1.0;
+    }
+    D cf[4];
+    for (int j = 0; j < 4; ++j) cf[j] = D::seed(coef[j], j);
+    const D L = rollout_loss<D>(st, s, cf, rain_mode, double(rain_amount), seed, nsteps, flags, target);
+    *loss = L.v;
+    for (int j = 0; j < 4 + ncells; ++j) grad[j] = L.d[j];
+    return 0;
 }
 
 }  // extern "C"

@@ -141,6 +141,13 @@ def lib():
         L.orc_droplets_f32.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                        ctypes.c_int, ctypes.POINTER(DropletSettings), ctypes.c_uint32]
         L.orc_droplets_f32.restype = ctypes.c_double
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_rollout_loss_f64.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), dp,
+                                           ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int,
+                                           ctypes.c_uint, dp, dp]
+        L.orc_rollout_grad_dual.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), dp,
+                                            ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int,
+                                            ctypes.c_uint, dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, dp, dp]
         for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
             p = ctypes.POINTER(t)
             getattr(L, f"orc_fbm_init_{name}").argtypes = [p, ctypes.c_int, ctypes.c_int,
@@ -227,3 +234,37 @@ def run_pass(state: np.ndarray, settings: ErosionSettings, pass_id: int, rain_mo
     if rc != 0:
         raise ValueError("orc_pass failed")
     return state
+
+
+def rollout_loss(state: np.ndarray, settings: ErosionSettings, coef, nsteps: int, target: np.ndarray,
+                 flags: int = STEP_DEFAULT, rain_mode: int = RAIN_CONSTANT, rain_amount: float = 0.01,
+                 seed: int = 0) -> float:
+    """FP64 loss mean((rock + dirt - target)^2) after nsteps from `state`, with
+    (Kc, Ks, Kd, Ke) = coef (double precision) and the rest from settings."""
+    _, ny, nx = state.shape
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    tg = np.ascontiguousarray(target, dtype=np.float64)
+    out = ctypes.c_double()
+    if lib().orc_rollout_loss_f64(_ptr(st), nx, ny, ctypes.byref(settings), _ptr(c), rain_mode, rain_amount,
+                                  seed, nsteps, flags, _ptr(tg), ctypes.byref(out)) != 0:
+        raise ValueError("orc_rollout_loss_f64 failed")
+    return out.value
+
+
+def rollout_grad_dual(state: np.ndarray, settings: ErosionSettings, coef, nsteps: int, target: np.ndarray,
+                      cells, flags: int = STEP_DEFAULT, rain_mode: int = RAIN_CONSTANT,
+                      rain_amount: float = 0.01, seed: int = 0):
+    """Dual-number FP64 rollout: (loss, dL/d(Kc,Ks,Kd,Ke), dL/drock0[cells])."""
+    _, ny, nx = state.shape
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    tg = np.ascontiguousarray(target, dtype=np.float64)
+    cl = np.ascontiguousarray(cells, dtype=np.int32)
+    g = np.zeros(4 + len(cl))
+    out = ctypes.c_double()
+    if lib().orc_rollout_grad_dual(_ptr(st), nx, ny, ctypes.byref(settings), _ptr(c), rain_mode, rain_amount,
+                                   seed, nsteps, flags, _ptr(tg), cl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   len(cl), ctypes.byref(out), _ptr(g)) != 0:
+        raise ValueError("orc_rollout_grad_dual failed")
+    return out.value, g[:4], g[4:]

@@ -1,0 +1,90 @@
+"""GPU adjoint (hg_adjoint_rollout) against the dual-number FP64 oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import hydrogen as hg
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+COEF = np.array([0.05, 0.02, 0.02, 0.01])
+
+
+def setup_case(n, seed=2, hi=12.0):
+    h0 = O.fbm_init(n, O.fbm_params(seed=seed, base_freq=1.0 / 12, hi=hi), np.float32)
+    tgt = O.fbm_init(n, O.fbm_params(seed=seed + 1, base_freq=1.0 / 12, hi=hi), np.float32)
+    return h0, tgt
+
+
+def gpu_grad(n, h0, tgt, steps, k_ckpt, flags=hg.STEP_DEFAULT):
+    with hg.Terrain(n) as t:
+        t.set_erosion(hg.erosion_settings(Kc=COEF[0], Ks=COEF[1], Kd=COEF[2], Ke=COEF[3]))
+        t.set_rain(hg.RAIN_CONSTANT, amount=0.01)
+        st = np.zeros((hg.NUM_STATE_PLANES, n, n), np.float32)
+        st[hg.P_ROCK] = h0
+        t.write_state(st)
+        t.step = 0
+        t.dispatch_grid(steps, flags)
+        h = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
+        final = t.read_state(hg.NUM_STATE_PLANES)
+        g = (2.0 * (h - tgt) / h.size).astype(np.float32)
+        t.write_state(st)
+        t.step = 0
+        gh, gc = t.adjoint_rollout(steps, k_ckpt, g, flags)
+        after = t.read_state(hg.NUM_STATE_PLANES)
+        assert t.step == steps
+    return gh, gc, final, after
+
+
+@pytest.mark.parametrize("flags", [hg.STEP_DEFAULT, hg.STEP_RAIN])
+def test_adjoint_matches_dual_oracle(flags):
+    n, steps = 40, 12
+    h0, tgt = setup_case(n)
+    gh, gc, _, _ = gpu_grad(n, h0, tgt, steps, 5, flags)
+    st = O.new_state(n)
+    st[O.ROCK] = h0
+    rng = np.random.default_rng(0)
+    cells = rng.choice(n * n, size=24, replace=False)
+    loss, dc, dh = O.rollout_grad_dual(st, O.config_settings(), COEF, steps, tgt, cells,
+                                       flags=flags, rain_mode=O.RAIN_CONSTANT, rain_amount=0.01)
+    print("coef gpu", gc, "dual", dc)
+    for j in range(4):
+        if abs(dc[j]) < 1e-12:
+            assert abs(gc[j]) < 1e-8, j
+        else:
+            assert abs(gc[j] - dc[j]) <= 2e-3 * abs(dc[j]), (j, gc[j], dc[j])
+    scale = np.abs(gh).max()
+    err = np.abs(gh.flat[cells] - dh)
+    print("cells max err / max|g|", err.max() / scale)
+    assert err.max() <= 2e-3 * scale
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    n, steps = 32, 9
+    h0, tgt = setup_case(n, seed=5)
+    ref_h, ref_c, _, _ = gpu_grad(n, h0, tgt, steps, 1)
+    for k in (2, 4, 9, 20):
+        gh, gc, _, _ = gpu_grad(n, h0, tgt, steps, k)
+        assert np.abs(gh - ref_h).max() <= 1e-5 * np.abs(ref_h).max(), k
+        assert np.allclose(gc, ref_c, rtol=1e-5, atol=1e-12), k
+
+
+def test_adjoint_leaves_the_final_state():
+    n, steps = 32, 7
+    h0, tgt = setup_case(n, seed=7)
+    _, _, final, after = gpu_grad(n, h0, tgt, steps, 3)
+    for p in range(hg.NUM_STATE_PLANES):
+        assert np.array_equal(final[p], after[p]), p
+
+
+def test_adjoint_arguments_rejected():
+    with hg.Terrain(32) as t:
+        g = np.zeros((32, 32), np.float32)
+        with pytest.raises(hg.HGError):
+            t.adjoint_rollout(0, 4, g)
+        with pytest.raises(hg.HGError):
+            t.adjoint_rollout(4, 0, g)
+        with pytest.raises(hg.HGError):
+            t.adjoint_rollout(4, 2, g, hg.STEP_PER_PASS)
