@@ -166,6 +166,75 @@ def bench_particles(iterations: int, oracle_iters: int) -> dict:
     return out
 
 
+def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
+    """Differentiable-rollout config: n^2 fBm terrain (6 octaves, frequency
+    1/256, [0, 64]), `steps` pipe+thermal steps with the plain-config
+    coefficients and rain 0.01/step; loss = MSE between the final rock+dirt and
+    a target made by the forward model from another seed with coefficients
+    scaled by U(0.8, 1.2); gradients w.r.t. the initial height and
+    (Kc, Ks, Kd, Ke) through the adjoint with a checkpoint every k_ckpt steps.
+    Reports t(forward + adjoint) / t(forward)."""
+    coef = dict(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01)
+    rng = np.random.default_rng(9)
+    pert = {k: v * float(rng.uniform(0.8, 1.2)) for k, v in coef.items()}
+    t = hg.Terrain(n)
+    t.set_rain(hg.RAIN_CONSTANT, amount=0.01)
+    # target terrain
+    t.set_erosion(hg.erosion_settings(**pert))
+    t.init_fbm(hg.fbm_params(seed=12, base_freq=1.0 / 256, hi=64.0))
+    t.dispatch_grid(steps)
+    target = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
+    t.set_erosion(hg.erosion_settings(**coef))
+    t.init_fbm(hg.fbm_params(seed=11, base_freq=1.0 / 256, hi=64.0))
+    init = t.read_state(hg.NUM_STATE_PLANES)
+    # forward alone
+    t.dispatch_grid(steps)
+    t.sync()
+    t.write_state(init)
+    t.step = 0
+    t.sync()
+    t0 = time.perf_counter()
+    t.dispatch_grid(steps)
+    t.sync()
+    t_fwd = time.perf_counter() - t0
+    h = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
+    loss = float(((h - target) ** 2).mean())
+    g = (2.0 * (h - target) / h.size).astype(np.float32)
+    # forward + adjoint (the adjoint call runs its own checkpointed forward)
+    t.write_state(init)
+    t.step = 0
+    t.sync()
+    t0 = time.perf_counter()
+    gh, gc = t.adjoint_rollout(steps, k_ckpt, g)
+    t_adj = time.perf_counter() - t0
+    t.close()
+    return {"grid": n, "steps": steps, "k_ckpt": k_ckpt, "loss": loss,
+            "t_forward_ms": t_fwd * 1e3, "t_forward_adjoint_ms": t_adj * 1e3,
+            "cost_ratio": t_adj / t_fwd, "grad_coef": [float(x) for x in gc],
+            "grad_h0_absmax": float(np.abs(gh).max()), "grad_finite": bool(np.isfinite(gh).all())}
+
+
+def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> dict:
+    """A plain device-to-device copy of known size (for scaling the PMC
+    FETCH_SIZE / WRITE_SIZE counters, which a profiler attributes to the copy
+    kernel of the HIP runtime)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t0) / reps
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"bytes": nbytes, "copy_GBps": 2 * nbytes / dt / 1e9}
+
+
 def cpu_baseline(t: hg.Terrain, steps: int) -> dict:
     """FP32 oracle (CPU port of the same passes) on `steps` steps of the same
     4096^2 state; OpenMP threads from OMP_NUM_THREADS."""
@@ -198,6 +267,10 @@ def main() -> int:
     ap.add_argument("--particle-iters", type=int, default=100,
                     help="droplet iterations of the particle workload (0 = skip)")
     ap.add_argument("--particle-oracle-iters", type=int, default=1)
+    ap.add_argument("--adjoint-steps", type=int, default=256,
+                    help="steps of the differentiable-rollout workload (0 = skip)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device copy (PMC counter calibration)")
     ap.add_argument("--tune", action="store_true", default=True)
     ap.add_argument("--no-tune", dest="tune", action="store_false")
     args = ap.parse_args()
@@ -245,6 +318,8 @@ def main() -> int:
     batched = bench_batched(args.batched_steps, 10, tiles) if args.batched_steps > 0 else None
     particles = (bench_particles(args.particle_iters, args.particle_oracle_iters)
                  if args.particle_iters > 0 else None)
+    adjoint = bench_adjoint(steps=args.adjoint_steps) if args.adjoint_steps > 0 else None
+    copy = calibrate_copy() if args.calibrate_copy else None
 
     out = {
         "metric": METRIC,
@@ -270,6 +345,8 @@ def main() -> int:
             "state_finite": finite,
             "batched": batched,
             "particles": particles,
+            "adjoint": adjoint,
+            "copy_calibration": copy,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
