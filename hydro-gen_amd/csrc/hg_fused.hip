@@ -270,11 +270,27 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
     }
 }
 
+// Tile of workgroup b in a 1-D grid of nb.  Workgroups are dealt to the 8 XCDs
+// round robin (b -> XCD b mod 8); with `xcd` set, XCD x gets the contiguous run
+// of tiles starting at x*q + min(x, r) (q = nb/8, r = nb mod 8), so a tile and
+// its neighbours, whose halos overlap, are read through the same L2.
+__device__ __forceinline__ void tile_of(int b, int nb, int gx, int xcd, int& bx, int& by) {
+    int t = b;
+    if (xcd) {
+        const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
+        t = x * q + min(x, r) + j;
+    }
+    by = t / gx;
+    bx = t - by * gx;
+}
+
 template <int TX, int TY, int HA, int NT>
 __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, uint32_t step,
-                                                    uint32_t terrain, int rain, KParams k) {
+                                                    uint32_t terrain, int rain, int xcd, KParams k) {
     __shared__ float lds[6 * HydroTile<TX, TY, HA, NT>::NLDS];
-    hydro_tile<TX, TY, HA, NT>(lds, io, nx, ny, blockIdx.x, blockIdx.y, step, terrain, rain, k);
+    int bx, by;
+    tile_of(blockIdx.x, gridDim.x, (nx + TX - 1) / TX, xcd, bx, by);
+    hydro_tile<TX, TY, HA, NT>(lds, io, nx, ny, bx, by, step, terrain, rain, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -442,9 +458,11 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
 
 template <int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_fused_thermal(
-    ThermIO io, int nx, int ny, int do_thermal, int do_smooth, KParams k) {
+    ThermIO io, int nx, int ny, int do_thermal, int do_smooth, int xcd, KParams k) {
     __shared__ float lds[3 * ThermalTile<TX, TY>::N0];
-    thermal_tile<TX, TY, NT>(lds, io, nx, ny, blockIdx.x, blockIdx.y, do_thermal, do_smooth, k);
+    int bx, by;
+    tile_of(blockIdx.x, gridDim.x, (nx + TX - 1) / TX, xcd, bx, by);
+    thermal_tile<TX, TY, NT>(lds, io, nx, ny, bx, by, do_thermal, do_smooth, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -568,12 +586,13 @@ bool thermal_variant_info(int v, int* tx, int* ty, int* halo, int* nt) {
 }
 
 hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step,
-                              uint32_t terrain, bool rain, const KParams& k, hipStream_t s) {
+                              uint32_t terrain, bool rain, bool xcd, const KParams& k, hipStream_t s) {
     switch (variant) {
 #define X(i, TX_, TY_, HA_, NT_)                                                              \
     case i: {                                                                                 \
-        const dim3 grid((nx + TX_ - 1) / TX_, (ny + TY_ - 1) / TY_);                          \
-        k_fused_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, rain ? 1 : 0, k); \
+        const unsigned grid = unsigned((nx + TX_ - 1) / TX_) * unsigned((ny + TY_ - 1) / TY_); \
+        k_fused_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, rain ? 1 : 0, \
+                                                               xcd ? 1 : 0, k);               \
         break;                                                                                \
     }
         HG_HYDRO_VARIANTS(X)
@@ -656,13 +675,13 @@ int batch_thermal_blocks_per_cu(int variant) {
 }
 
 hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,
-                                bool smooth, const KParams& k, hipStream_t s) {
+                                bool smooth, bool xcd, const KParams& k, hipStream_t s) {
     switch (variant) {
 #define X(i, TX_, TY_, NT_)                                                                  \
     case i: {                                                                                \
-        const dim3 grid((nx + TX_ - 1) / TX_, (ny + TY_ - 1) / TY_);                         \
+        const unsigned grid = unsigned((nx + TX_ - 1) / TX_) * unsigned((ny + TY_ - 1) / TY_); \
         k_fused_thermal<TX_, TY_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, thermal ? 1 : 0,     \
-                                                             smooth ? 1 : 0, k);             \
+                                                             smooth ? 1 : 0, xcd ? 1 : 0, k); \
         break;                                                                               \
     }
         HG_THERMAL_VARIANTS(X)

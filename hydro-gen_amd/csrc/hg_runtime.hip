@@ -188,7 +188,8 @@ int fused_step(hg_ctx* c, bool rain, bool thermal, bool smooth, bool keep_veloci
     io.vz = keep_velocity ? c->alt[HG_P_VEL_DEPTH] : nullptr;
     const uint32_t step = uint32_t(c->step);
     int rc = timed_launch(c, "fused_hydro", [&] {
-        return hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->kp, c->stream);
+        return hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->xcd_order, c->kp,
+                                       c->stream);
     });
     if (rc != HG_OK) return rc;
     swap_planes(c, {HG_P_ROCK, HG_P_DIRT, HG_P_WATER, HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
@@ -199,7 +200,7 @@ int fused_step(hg_ctx* c, bool rain, bool thermal, bool smooth, bool keep_veloci
     tio.r = c->cur[HG_P_ROCK]; tio.d = c->cur[HG_P_DIRT];
     tio.ro = c->alt[HG_P_ROCK]; tio.dout = c->alt[HG_P_DIRT];
     rc = timed_launch(c, "fused_thermal", [&] {
-        return hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->kp,
+        return hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->xcd_order, c->kp,
                                          c->stream);
     });
     if (rc != HG_OK) return rc;
@@ -454,7 +455,7 @@ int hg_step_particle(hg_ctx* c, uint32_t iterations, uint32_t flags) {
             tio.r = c->cur[HG_P_ROCK]; tio.d = c->cur[HG_P_DIRT];
             tio.ro = c->alt[HG_P_ROCK]; tio.dout = c->alt[HG_P_DIRT];
             rc = timed_launch(c, "fused_thermal", [&] {
-                return hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->kp,
+                return hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->xcd_order, c->kp,
                                                  c->stream);
             });
             if (rc != HG_OK) return rc;
@@ -627,6 +628,12 @@ int hg_get_tiles(const hg_ctx* c, int* hydro_variant, int* thermal_variant) {
     if (!c) return HG_ERR_INVALID;
     if (hydro_variant) *hydro_variant = c->hydro_variant;
     if (thermal_variant) *thermal_variant = c->thermal_variant;
+    return HG_OK;
+}
+
+int hg_set_xcd_order(hg_ctx* c, int on) {
+    if (!c) return HG_ERR_INVALID;
+    c->xcd_order = on != 0;
     return HG_OK;
 }
 

@@ -37,7 +37,7 @@ EXPORTS = [
     "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
-    "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_autotune",
+    "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
     "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
     "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles",
     "hg_batch_set_persistent", "hg_batch_set_xcd_order",
@@ -189,6 +189,7 @@ def load() -> ctypes.CDLL:
     L.hg_tile_variant_info.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
     L.hg_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hg_get_tiles.argtypes = [vp, ip, ip]
+    L.hg_set_xcd_order.argtypes = [vp, ctypes.c_int]
     L.hg_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ip, ip]
     u32p = ctypes.POINTER(ctypes.c_uint32)
     L.hg_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, u32p, ctypes.POINTER(vp)]
@@ -384,6 +385,10 @@ class Terrain:
         a, b = ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.hg_get_tiles(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_xcd_order(self, on: bool):
+        """Contiguous tile runs per XCD for the fused kernels (default off)."""
+        self._check(self.lib.hg_set_xcd_order(self.h, 1 if on else 0))
 
     def autotune(self, steps: int = 10, flags: int = STEP_DEFAULT):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -288,6 +288,11 @@ int hg_tile_variant_info(int kernel, int variant, int* tile_x, int* tile_y, int*
 /* -1 keeps the current choice. */
 int hg_set_tiles(hg_ctx* ctx, int hydro_variant, int thermal_variant);
 int hg_get_tiles(const hg_ctx* ctx, int* hydro_variant, int* thermal_variant);
+/* Tile order of the fused kernels' workgroups.  Workgroups are dealt to the
+ * 8 XCDs round robin; on, each XCD gets a contiguous run of tiles, so the
+ * halos that neighbouring tiles share are read through one L2.  Off (the
+ * default, 2-5% faster on a 4096^2 grid): row-major tile order. */
+int hg_set_xcd_order(hg_ctx* ctx, int on);
 /* Times `steps` fused steps per variant (state and step counter are restored
  * afterwards) and selects the fastest of each kernel. */
 int hg_autotune(hg_ctx* ctx, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best);
