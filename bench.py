@@ -39,6 +39,25 @@ ALGO_BYTES = {"fused_hydro": 72, "fused_thermal": 16}
 STEP_ALGO_BYTES = 72         # minimum for a step that materialises the 9 planes once
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_step_4096.md")
+
+
+def profiled_traffic(kernel: str, n: int, variant: tuple) -> int | None:
+    """HBM bytes per launch of `kernel` (tile variant (tx, ty, halo, threads)) on
+    an n x n grid, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary;
+    None when that summary has no figure for this kernel, grid and variant."""
+    key = f"traffic[{kernel} {n} {'x'.join(str(v) for v in variant)}]:"
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                line = line.strip().lstrip("- ")
+                if line.startswith(key):
+                    return int(line[len(key):].strip())
+    except OSError:
+        pass
+    return None
+
+
 def build_terrain(n: int, seed: int = 1) -> hg.Terrain:
     t = hg.Terrain(n, device=0)
     t.set_erosion(hg.erosion_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0,
@@ -307,8 +326,11 @@ def main() -> int:
     roofline = None
     if per_cell is not None:
         ach = per_cell * n * n / dom_avg_s
+        hv, tv = t.get_tiles()
+        variant = hg.tile_variants(0)[hv] if dom == "fused_hydro" else hg.tile_variants(1)[tv]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
+                    "unit": "GB/s", "frac": ach / HBM_PEAK,
+                    "traffic": profiled_traffic(dom, n, variant),
                     "algo_bytes_per_launch": per_cell * n * n}
 
     h = t.read_plane(hg.P_ROCK)
