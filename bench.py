@@ -273,6 +273,11 @@ def cpu_baseline(t: hg.Terrain, steps: int) -> dict:
                       f"warm-up), oracle/liboracle.so, {cores} OpenMP threads, {dt:.1f} s"}
 
 
+def log(msg: str) -> None:
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,7 +285,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--per-pass", action="store_true", help="time the unfused per-pass kernels")
-    ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=96, help="CPU baseline sample (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=1000,
                     help="steps of the 64-terrain batched graph (0 = skip)")
     ap.add_argument("--particle-iters", type=int, default=100,
@@ -333,14 +338,23 @@ def main() -> int:
                     "traffic": profiled_traffic(dom, n, variant),
                     "algo_bytes_per_launch": per_cell * n * n}
 
+    log(f"{args.steps} GRID steps at {n}^2: {wall / args.steps * 1e3:.3f} ms/step")
     h = t.read_plane(hg.P_ROCK)
     finite = bool(np.isfinite(h).all())
     cpu = cpu_baseline(t, args.cpu_steps) if args.cpu_steps > 0 else None
+    if cpu:
+        log(f"CPU baseline: {cpu['value']:.3e} cell-steps/s")
     t.close()
     batched = bench_batched(args.batched_steps, 10, tiles) if args.batched_steps > 0 else None
+    if batched:
+        log("batched terrains done")
     particles = (bench_particles(args.particle_iters, args.particle_oracle_iters)
                  if args.particle_iters > 0 else None)
+    if particles:
+        log("droplets done")
     adjoint = bench_adjoint(steps=args.adjoint_steps) if args.adjoint_steps > 0 else None
+    if adjoint:
+        log("adjoint done")
     copy = calibrate_copy() if args.calibrate_copy else None
 
     out = {
