@@ -94,7 +94,7 @@ def _run_pair(state, steps, flags, rain_mode=hg.RAIN_UNIFORM, settings=None):
     return out
 
 
-@pytest.mark.parametrize("n", [64, 200, 256])
+@pytest.mark.parametrize("n", [8, 24, 64, 200, 256])
 def test_fused_equals_per_pass_bitwise(n):
     st = spun_up_state(n, 30, seed=5)
     print(f"fused vs per-pass n={n}", flush=True)
