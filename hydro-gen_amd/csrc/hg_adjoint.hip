@@ -286,8 +286,11 @@ __global__ __launch_bounds__(BX * BY) void k_adj_transport(Planes x, Planes lo, 
             lbx += l * ((1.0f - fy) * (a10 - a00) + fy * (a11 - a01));
             lby += l * (lerp1(a01, a11, fx) - lerp1(a00, a10, fx));
         }
-        if (bxr > 0.0f && bxr < float(nx - 1)) li.p[HG_P_VEL_U][i] += -k.dt * lbx;
-        if (byr > 0.0f && byr < float(ny - 1)) li.p[HG_P_VEL_V][i] += -k.dt * lby;
+        // first writer of the velocity adjoint in a step (velocity is an
+        // intermediate of the step: nothing after this step reads it)
+        li.p[HG_P_VEL_U][i] = (bxr > 0.0f && bxr < float(nx - 1)) ? -k.dt * lbx : 0.0f;
+        li.p[HG_P_VEL_V][i] = (byr > 0.0f && byr < float(ny - 1)) ? -k.dt * lby : 0.0f;
+        li.p[HG_P_VEL_DEPTH][i] = 0.0f;
         const float lb = lo.p[HG_P_WATER][i];
         li.p[HG_P_WATER][i] = lb * k.evap;                       // b' = b (1 - Ke dt)
         gKe = double(lb) * double(x.p[HG_P_WATER][i]) * double(-k.dt);
@@ -375,7 +378,7 @@ __global__ __launch_bounds__(BX * BY) void k_adj_thermal_transport(Planes lo, Pl
     for (int kk = 0; kk < 8; ++kk) {
         const int xx = px + TDX[kk], yy = py + TDY[kk];
         const bool ok = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
-        li.p[HG_P_THERM_L + kk][i] += (ok ? lo.p[hp][at(xx, yy, nx)] : 0.0f) - lh;
+        li.p[HG_P_THERM_L + kk][i] = (ok ? lo.p[hp][at(xx, yy, nx)] : 0.0f) - lh;   // only reader of T_kk
     }
 }
 
@@ -412,9 +415,16 @@ __global__ __launch_bounds__(BX * BY) void k_adj_smooth(Planes x, Planes lo, Pla
     }
 }
 
-__global__ __launch_bounds__(256) void k_sum_planes(float* dst, const float* a, const float* b, size_t n) {
-    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) dst[i] = a[i] + b[i];
+// Zeroes `count` planes of n floats (n a multiple of 4: n = side^2, side % 8 == 0).
+struct PlaneList {
+    float* p[48];
+};
+__global__ __launch_bounds__(256) void k_zero_planes(PlaneList l, int count, size_t n4) {
+    const size_t total = size_t(count) * n4;
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += size_t(gridDim.x) * 256) {
+        const size_t pl = e / n4, j = e - pl * n4;
+        reinterpret_cast<float4*>(l.p[pl])[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
 }
 
 }  // namespace
@@ -442,6 +452,19 @@ std::vector<int> produced(int pass) {
     }
 }
 
+// Whether the adjoint kernel of `pass` accumulates (with atomics) into the
+// fresh buffer of produced plane `p`, which must then start at zero; the other
+// fresh buffers are written with plain stores or never read.
+bool needs_zero(int pass, int p) {
+    switch (pass) {
+        case HG_PASS_FLUX: return p == HG_P_WATER || (p >= HG_P_FLUX_L && p <= HG_P_FLUX_B);
+        case HG_PASS_EROSION: return p == HG_P_ROCK || p == HG_P_DIRT;
+        case HG_PASS_TRANSPORT: return p == HG_P_SED_ROCK || p == HG_P_SED_DIRT;
+        case HG_PASS_SMOOTH: return true;
+        default: return false;
+    }
+}
+
 // Device buffers of the reverse sweep.
 struct AdjWork {
     size_t pn = 0, pitch = 0;
@@ -450,8 +473,8 @@ struct AdjWork {
     std::vector<float*> seg;       // k_ckpt x 9 planes: each step's start state in a segment
     std::vector<float*> arena;     // per-pass outputs of one recomputed step
     std::vector<float*> pool;      // adjoint planes (free list)
-    float* rcur[HG_NUM_STATE_PLANES] = {};   // buffers of the segment recompute
-    float* ralt[HG_NUM_STATE_PLANES] = {};
+    float* tmp_r = nullptr;        // thermal kernel output of the segment recompute
+    float* tmp_d = nullptr;
     float* zero = nullptr;
     float* lw = nullptr;
     double* gcoef = nullptr;
@@ -461,6 +484,44 @@ float* take(std::vector<float*>& pool) {
     float* p = pool.back();
     pool.pop_back();
     return p;
+}
+
+// One fused GRID step from the 9 planes `in` into `out`; the thermal kernel
+// writes tmp_r/tmp_d, which are then exchanged with out's rock/dirt buffers.
+int step_into(hg_ctx* c, float* const* in, float** out, float*& tmp_r, float*& tmp_d, uint32_t step,
+              bool rain, bool thermal, bool smooth) {
+    hgk::HydroIO io;
+    io.r = in[HG_P_ROCK]; io.d = in[HG_P_DIRT]; io.b = in[HG_P_WATER];
+    io.fl = in[HG_P_FLUX_L]; io.fr = in[HG_P_FLUX_R]; io.ft = in[HG_P_FLUX_T]; io.fb = in[HG_P_FLUX_B];
+    io.s0 = in[HG_P_SED_ROCK]; io.s1 = in[HG_P_SED_DIRT];
+    io.ro = out[HG_P_ROCK]; io.dout = out[HG_P_DIRT]; io.bo = out[HG_P_WATER];
+    io.flo = out[HG_P_FLUX_L]; io.fro = out[HG_P_FLUX_R]; io.fto = out[HG_P_FLUX_T]; io.fbo = out[HG_P_FLUX_B];
+    io.s0o = out[HG_P_SED_ROCK]; io.s1o = out[HG_P_SED_DIRT];
+    io.vx = io.vy = io.vz = nullptr;
+    HG_HIP(c, hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->xcd_order, c->kp,
+                                      c->stream));
+    if (!thermal && !smooth) return HG_OK;
+    hgk::ThermIO tio;
+    tio.r = out[HG_P_ROCK]; tio.d = out[HG_P_DIRT];
+    tio.ro = tmp_r; tio.dout = tmp_d;
+    HG_HIP(c, hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->xcd_order, c->kp,
+                                        c->stream));
+    std::swap(out[HG_P_ROCK], tmp_r);
+    std::swap(out[HG_P_DIRT], tmp_d);
+    return HG_OK;
+}
+
+int zero_planes(hg_ctx* c, const std::vector<float*>& planes, size_t pn) {
+    for (size_t b = 0; b < planes.size(); b += 48) {
+        hgk::PlaneList l{};
+        const int cnt = int(std::min<size_t>(48, planes.size() - b));
+        for (int j = 0; j < cnt; ++j) l.p[j] = planes[b + size_t(j)];
+        const size_t n4 = pn / 4;
+        const unsigned grid = unsigned(std::min<size_t>((size_t(cnt) * n4 + 255) / 256, 8192));
+        hgk::k_zero_planes<<<grid, 256, 0, c->stream>>>(l, cnt, n4);
+        HG_HIP(c, hipGetLastError());
+    }
+    return HG_OK;
 }
 
 }  // namespace
@@ -483,9 +544,9 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     AdjWork w;
     w.pn = pn;
     w.pitch = (pn * sizeof(float) + 255) / 256 * 256;
-    const size_t n_arena = 40, n_pool = 2 * HG_NUM_PLANES + 8;
+    const size_t n_arena = 40, n_pool = 64;
     const size_t nplanes = size_t(nck) * HG_NUM_STATE_PLANES + size_t(k_ckpt) * HG_NUM_STATE_PLANES + n_arena +
-                           n_pool + 2 + 2 * HG_NUM_STATE_PLANES;
+                           n_pool + 4;
     HG_HIP(c, hipMalloc(&w.block, nplanes * w.pitch + 256));
     struct Free {
         void* p;
@@ -498,10 +559,8 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     for (size_t i = 0; i < size_t(k_ckpt) * HG_NUM_STATE_PLANES; ++i) w.seg.push_back(plane());
     for (size_t i = 0; i < n_arena; ++i) w.arena.push_back(plane());
     for (size_t i = 0; i < n_pool; ++i) w.pool.push_back(plane());
-    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) {
-        w.rcur[p] = plane();
-        w.ralt[p] = plane();
-    }
+    w.tmp_r = plane();
+    w.tmp_d = plane();
     w.zero = plane();
     w.lw = plane();
     w.gcoef = reinterpret_cast<double*>(base + used * w.pitch);
@@ -519,61 +578,51 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
         if (rc != HG_OK) return rc;
         c->step += 1;
     }
-    // the context is now at the final state; keep its buffers aside
-    float* saved_cur[HG_NUM_PLANES];
-    float* saved_alt[HG_NUM_PLANES];
-    std::memcpy(saved_cur, c->cur, sizeof(saved_cur));
-    std::memcpy(saved_alt, c->alt, sizeof(saved_alt));
     const uint64_t step_end = c->step;
 
     // ---- adjoint of the final state: dL/dH on rock and dirt
     float* lam[HG_NUM_PLANES];
-    for (int p = 0; p < HG_NUM_PLANES; ++p) {
-        lam[p] = take(w.pool);
-        HG_HIP(c, hipMemsetAsync(lam[p], 0, pn * sizeof(float), c->stream));
+    {
+        std::vector<float*> z;
+        for (int p = 0; p < HG_NUM_PLANES; ++p) z.push_back(lam[p] = take(w.pool));
+        const int rz = zero_planes(c, z, pn);
+        if (rz != HG_OK) return rz;
     }
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_ROCK], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_DIRT], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
 
+    std::vector<int> order;
+    if (rain) order.push_back(HG_PASS_RAIN);
+    order.push_back(HG_PASS_FLUX);
+    order.push_back(HG_PASS_EROSION);
+    order.push_back(HG_PASS_TRANSPORT);
+    if (thermal)
+        for (int p = HG_PASS_THERMAL_FLUX_0; p <= HG_PASS_THERMAL_TRANSPORT_1; ++p) order.push_back(p);
+    if (smooth) order.push_back(HG_PASS_SMOOTH);
+
     const dim3 blk(BX, BY), grd((n + BX - 1) / BX, (n + BY - 1) / BY);
     const unsigned g1 = unsigned((pn + 255) / 256);
-    int rc = HG_OK;
-    for (int seg = int(nck) - 1; seg >= 0 && rc == HG_OK; --seg) {
+    for (int seg = int(nck) - 1; seg >= 0; --seg) {
         const uint32_t s_begin = uint32_t(seg) * k_ckpt;
         const uint32_t s_end = std::min(nsteps, s_begin + k_ckpt);
-        // recompute the segment (fused) in the work buffers, saving every
-        // step's start state; the context's own buffers keep the final state
-        for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) {
-            c->cur[p] = w.rcur[p];
-            c->alt[p] = w.ralt[p];
-            HG_HIP(c, hipMemcpyAsync(c->cur[p], w.ckpt[size_t(seg) * HG_NUM_STATE_PLANES + p], pn * sizeof(float),
-                                     hipMemcpyDeviceToDevice, c->stream));
-        }
-        c->step = step0 + s_begin;
-        for (uint32_t s = s_begin; s < s_end; ++s) {
-            for (int p = 0; p < HG_NUM_STATE_PLANES; ++p)
-                HG_HIP(c, hipMemcpyAsync(w.seg[size_t(s - s_begin) * HG_NUM_STATE_PLANES + p], c->cur[p],
-                                         pn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-            if (s + 1 < s_end) {
-                rc = fused_step(c, rain, thermal, smooth, false);
-                if (rc != HG_OK) break;
-                c->step += 1;
-            }
+        // recompute the segment from its checkpoint straight into the slots of
+        // w.seg: slot 0 is the checkpoint itself, slot j the state after j steps
+        // (step_into permutes the rock/dirt buffers of slots >= 1 with
+        // w.tmp_r/w.tmp_d, so the slot table is kept across segments)
+        std::vector<float*>& slots = w.seg;
+        for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) slots[size_t(p)] = w.ckpt[size_t(seg) * HG_NUM_STATE_PLANES + p];
+        for (uint32_t s = s_begin; s + 1 < s_end; ++s) {
+            const size_t j = s - s_begin;
+            const int rc = step_into(c, &slots[j * HG_NUM_STATE_PLANES], &slots[(j + 1) * HG_NUM_STATE_PLANES],
+                                     w.tmp_r, w.tmp_d, uint32_t(step0 + s), rain, thermal, smooth);
+            if (rc != HG_OK) return rc;
         }
         // reverse the steps of the segment
-        for (int s = int(s_end) - 1; s >= int(s_begin) && rc == HG_OK; --s) {
+        for (int s = int(s_end) - 1; s >= int(s_begin); --s) {
             // recompute step s pass by pass into the arena, recording pass inputs
             float* cur[HG_NUM_PLANES];
             for (int p = 0; p < HG_NUM_PLANES; ++p)
-                cur[p] = p < HG_NUM_STATE_PLANES ? w.seg[size_t(s - int(s_begin)) * HG_NUM_STATE_PLANES + p] : w.zero;
-            std::vector<int> order;
-            if (rain) order.push_back(HG_PASS_RAIN);
-            order.push_back(HG_PASS_FLUX);
-            order.push_back(HG_PASS_EROSION);
-            order.push_back(HG_PASS_TRANSPORT);
-            if (thermal)
-                for (int p = HG_PASS_THERMAL_FLUX_0; p <= HG_PASS_THERMAL_TRANSPORT_1; ++p) order.push_back(p);
-            if (smooth) order.push_back(HG_PASS_SMOOTH);
+                cur[p] = p < HG_NUM_STATE_PLANES ? slots[size_t(s - int(s_begin)) * HG_NUM_STATE_PLANES + p] : w.zero;
             std::vector<Planes> xin;
             size_t ai = 0;
             for (int pass : order) {
@@ -583,6 +632,21 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                 xin.push_back(in);
                 for (int p : produced(pass)) cur[p] = out.p[p];
             }
+            // fresh adjoint buffers of every pass output, taken up front; the
+            // ones that accumulate are zeroed by one launch
+            std::vector<std::vector<float*>> fresh_of(order.size());
+            std::vector<float*> to_zero;
+            for (size_t oi = 0; oi < order.size(); ++oi) {
+                if (order[oi] == HG_PASS_RAIN) continue;
+                for (int p : produced(order[oi])) {
+                    float* f = take(w.pool);
+                    fresh_of[oi].push_back(f);
+                    if (needs_zero(order[oi], p)) to_zero.push_back(f);
+                }
+                if (order[oi] == HG_PASS_FLUX) to_zero.push_back(w.lw);
+            }
+            int rc = zero_planes(c, to_zero, pn);
+            if (rc != HG_OK) return rc;
             // pass adjoints in reverse order
             for (int oi = int(order.size()) - 1; oi >= 0; --oi) {
                 const int pass = order[oi];
@@ -592,14 +656,10 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                 float* fresh[HG_NUM_PLANES];
                 std::memcpy(fresh, lam, sizeof(fresh));
                 const std::vector<int> prod = produced(pass);
-                for (int p : prod) {
-                    fresh[p] = take(w.pool);
-                    HG_HIP(c, hipMemsetAsync(fresh[p], 0, pn * sizeof(float), c->stream));
-                }
+                for (size_t q = 0; q < prod.size(); ++q) fresh[prod[q]] = fresh_of[oi][q];
                 const Planes li = planes_of(fresh);
                 switch (pass) {
                     case HG_PASS_FLUX:
-                        HG_HIP(c, hipMemsetAsync(w.lw, 0, pn * sizeof(float), c->stream));
                         k_adj_flux<<<grd, blk, 0, c->stream>>>(X, lo, li, w.lw, n, n, c->kp);
                         HG_HIP(c, hipGetLastError());
                         k_adj_total<<<g1, 256, 0, c->stream>>>(w.lw, fresh[HG_P_ROCK], fresh[HG_P_DIRT],
@@ -634,11 +694,7 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
             }
         }
     }
-    // restore the context's final state
-    std::memcpy(c->cur, saved_cur, sizeof(saved_cur));
-    std::memcpy(c->alt, saved_alt, sizeof(saved_alt));
     c->step = step_end;
-    if (rc != HG_OK) return rc;
     HG_HIP(c, hipMemcpyAsync(dL_dH0, lam[HG_P_ROCK], pn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HG_HIP(c, hipMemcpyAsync(dL_dcoef, w.gcoef, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HG_HIP(c, hipStreamSynchronize(c->stream));
