@@ -470,15 +470,16 @@ struct AdjWork {
     size_t pn = 0, pitch = 0;
     void* block = nullptr;
     std::vector<float*> ckpt;      // (#checkpoints) x 9 planes
-    std::vector<float*> seg;       // k_ckpt x 9 planes: each step's start state in a segment
-    std::vector<float*> arena;     // per-pass outputs of one recomputed step
+    std::vector<float*> seg;       // (k_ckpt + 1) x 9 planes: each step's start state in a segment
+    std::vector<float*> tape;      // k_ckpt x NTAPE planes: each step's intermediates
     std::vector<float*> pool;      // adjoint planes (free list)
-    float* tmp_r = nullptr;        // thermal kernel output of the segment recompute
-    float* tmp_d = nullptr;
     float* zero = nullptr;
     float* lw = nullptr;
     double* gcoef = nullptr;
 };
+
+// Intermediates of one step recorded by the fused kernels.
+enum { T_B1, T_B2, T_S0, T_S1, T_VX, T_VY, T_VZ, T_R3, T_D3, T_R5, T_D6, NTAPE };
 
 float* take(std::vector<float*>& pool) {
     float* p = pool.back();
@@ -486,28 +487,32 @@ float* take(std::vector<float*>& pool) {
     return p;
 }
 
-// One fused GRID step from the 9 planes `in` into `out`; the thermal kernel
-// writes tmp_r/tmp_d, which are then exchanged with out's rock/dirt buffers.
-int step_into(hg_ctx* c, float* const* in, float** out, float*& tmp_r, float*& tmp_d, uint32_t step,
-              bool rain, bool thermal, bool smooth) {
+// One fused GRID step from the 9 planes `in` into `out`, recording the step's
+// intermediates in `tp`: hydro writes rock/dirt after erosion to tp[T_R3/T_D3]
+// (the thermal kernel then produces out's rock/dirt from them).
+int step_into(hg_ctx* c, float* const* in, float* const* out, float* const* tp, uint32_t step, bool rain,
+              bool thermal, bool smooth) {
+    const bool th = thermal || smooth;
     hgk::HydroIO io;
     io.r = in[HG_P_ROCK]; io.d = in[HG_P_DIRT]; io.b = in[HG_P_WATER];
     io.fl = in[HG_P_FLUX_L]; io.fr = in[HG_P_FLUX_R]; io.ft = in[HG_P_FLUX_T]; io.fb = in[HG_P_FLUX_B];
     io.s0 = in[HG_P_SED_ROCK]; io.s1 = in[HG_P_SED_DIRT];
-    io.ro = out[HG_P_ROCK]; io.dout = out[HG_P_DIRT]; io.bo = out[HG_P_WATER];
+    io.ro = th ? tp[T_R3] : out[HG_P_ROCK];
+    io.dout = th ? tp[T_D3] : out[HG_P_DIRT];
+    io.bo = out[HG_P_WATER];
     io.flo = out[HG_P_FLUX_L]; io.fro = out[HG_P_FLUX_R]; io.fto = out[HG_P_FLUX_T]; io.fbo = out[HG_P_FLUX_B];
     io.s0o = out[HG_P_SED_ROCK]; io.s1o = out[HG_P_SED_DIRT];
-    io.vx = io.vy = io.vz = nullptr;
+    io.vx = tp[T_VX]; io.vy = tp[T_VY]; io.vz = tp[T_VZ];
+    io.tb1 = tp[T_B1]; io.tb2 = tp[T_B2]; io.ts0 = tp[T_S0]; io.ts1 = tp[T_S1];
     HG_HIP(c, hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->xcd_order, c->kp,
                                       c->stream));
-    if (!thermal && !smooth) return HG_OK;
+    if (!th) return HG_OK;
     hgk::ThermIO tio;
-    tio.r = out[HG_P_ROCK]; tio.d = out[HG_P_DIRT];
-    tio.ro = tmp_r; tio.dout = tmp_d;
+    tio.r = tp[T_R3]; tio.d = tp[T_D3];
+    tio.ro = out[HG_P_ROCK]; tio.dout = out[HG_P_DIRT];
+    tio.tr5 = tp[T_R5]; tio.td6 = tp[T_D6];
     HG_HIP(c, hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->xcd_order, c->kp,
                                         c->stream));
-    std::swap(out[HG_P_ROCK], tmp_r);
-    std::swap(out[HG_P_DIRT], tmp_d);
     return HG_OK;
 }
 
@@ -544,9 +549,9 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     AdjWork w;
     w.pn = pn;
     w.pitch = (pn * sizeof(float) + 255) / 256 * 256;
-    const size_t n_arena = 40, n_pool = 64;
-    const size_t nplanes = size_t(nck) * HG_NUM_STATE_PLANES + size_t(k_ckpt) * HG_NUM_STATE_PLANES + n_arena +
-                           n_pool + 4;
+    const size_t n_pool = 64;
+    const size_t nplanes = size_t(nck) * HG_NUM_STATE_PLANES + size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES +
+                           size_t(k_ckpt) * NTAPE + n_pool + 2;
     HG_HIP(c, hipMalloc(&w.block, nplanes * w.pitch + 256));
     struct Free {
         void* p;
@@ -556,11 +561,9 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     size_t used = 0;
     auto plane = [&]() { float* p = reinterpret_cast<float*>(base + used * w.pitch); ++used; return p; };
     for (size_t i = 0; i < size_t(nck) * HG_NUM_STATE_PLANES; ++i) w.ckpt.push_back(plane());
-    for (size_t i = 0; i < size_t(k_ckpt) * HG_NUM_STATE_PLANES; ++i) w.seg.push_back(plane());
-    for (size_t i = 0; i < n_arena; ++i) w.arena.push_back(plane());
+    for (size_t i = 0; i < size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES; ++i) w.seg.push_back(plane());
+    for (size_t i = 0; i < size_t(k_ckpt) * NTAPE; ++i) w.tape.push_back(plane());
     for (size_t i = 0; i < n_pool; ++i) w.pool.push_back(plane());
-    w.tmp_r = plane();
-    w.tmp_d = plane();
     w.zero = plane();
     w.lw = plane();
     w.gcoef = reinterpret_cast<double*>(base + used * w.pitch);
@@ -592,7 +595,6 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_DIRT], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
 
     std::vector<int> order;
-    if (rain) order.push_back(HG_PASS_RAIN);
     order.push_back(HG_PASS_FLUX);
     order.push_back(HG_PASS_EROSION);
     order.push_back(HG_PASS_TRANSPORT);
@@ -605,39 +607,56 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     for (int seg = int(nck) - 1; seg >= 0; --seg) {
         const uint32_t s_begin = uint32_t(seg) * k_ckpt;
         const uint32_t s_end = std::min(nsteps, s_begin + k_ckpt);
-        // recompute the segment from its checkpoint straight into the slots of
-        // w.seg: slot 0 is the checkpoint itself, slot j the state after j steps
-        // (step_into permutes the rock/dirt buffers of slots >= 1 with
-        // w.tmp_r/w.tmp_d, so the slot table is kept across segments)
+        // recompute the segment from its checkpoint, recording every step's
+        // start state (slot 0 is the checkpoint itself) and intermediates
         std::vector<float*>& slots = w.seg;
         for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) slots[size_t(p)] = w.ckpt[size_t(seg) * HG_NUM_STATE_PLANES + p];
-        for (uint32_t s = s_begin; s + 1 < s_end; ++s) {
+        for (uint32_t s = s_begin; s < s_end; ++s) {
             const size_t j = s - s_begin;
             const int rc = step_into(c, &slots[j * HG_NUM_STATE_PLANES], &slots[(j + 1) * HG_NUM_STATE_PLANES],
-                                     w.tmp_r, w.tmp_d, uint32_t(step0 + s), rain, thermal, smooth);
+                                     &w.tape[j * NTAPE], uint32_t(step0 + s), rain, thermal, smooth);
             if (rc != HG_OK) return rc;
         }
         // reverse the steps of the segment
         for (int s = int(s_end) - 1; s >= int(s_begin); --s) {
-            // recompute step s pass by pass into the arena, recording pass inputs
-            float* cur[HG_NUM_PLANES];
-            for (int p = 0; p < HG_NUM_PLANES; ++p)
-                cur[p] = p < HG_NUM_STATE_PLANES ? slots[size_t(s - int(s_begin)) * HG_NUM_STATE_PLANES + p] : w.zero;
-            std::vector<Planes> xin;
-            size_t ai = 0;
-            for (int pass : order) {
-                Planes in = planes_of(cur), out = planes_of(cur);
-                for (int p : produced(pass)) out.p[p] = w.arena[ai++];
-                HG_HIP(c, launch_pass(pass, in, out, n, n, uint32_t(step0 + uint32_t(s)), 0u, c->kp, c->stream));
-                xin.push_back(in);
-                for (int p : produced(pass)) cur[p] = out.p[p];
-            }
+            float* const* in = &slots[size_t(s - int(s_begin)) * HG_NUM_STATE_PLANES];
+            float* const* tp = &w.tape[size_t(s - int(s_begin)) * NTAPE];
+            // pass inputs, from the step's start state and its tape
+            auto x_of = [&](int pass) {
+                Planes X;
+                for (int p = 0; p < HG_NUM_PLANES; ++p) X.p[p] = w.zero;
+                switch (pass) {
+                    case HG_PASS_FLUX:
+                        X.p[HG_P_ROCK] = in[HG_P_ROCK]; X.p[HG_P_DIRT] = in[HG_P_DIRT]; X.p[HG_P_WATER] = tp[T_B1];
+                        for (int p = HG_P_FLUX_L; p <= HG_P_FLUX_B; ++p) X.p[p] = in[p];
+                        break;
+                    case HG_PASS_EROSION:
+                        X.p[HG_P_ROCK] = in[HG_P_ROCK]; X.p[HG_P_DIRT] = in[HG_P_DIRT];
+                        X.p[HG_P_SED_ROCK] = in[HG_P_SED_ROCK]; X.p[HG_P_SED_DIRT] = in[HG_P_SED_DIRT];
+                        X.p[HG_P_VEL_U] = tp[T_VX]; X.p[HG_P_VEL_V] = tp[T_VY]; X.p[HG_P_VEL_DEPTH] = tp[T_VZ];
+                        break;
+                    case HG_PASS_TRANSPORT:
+                        X.p[HG_P_SED_ROCK] = tp[T_S0]; X.p[HG_P_SED_DIRT] = tp[T_S1]; X.p[HG_P_WATER] = tp[T_B2];
+                        X.p[HG_P_VEL_U] = tp[T_VX]; X.p[HG_P_VEL_V] = tp[T_VY]; X.p[HG_P_VEL_DEPTH] = tp[T_VZ];
+                        break;
+                    case HG_PASS_THERMAL_FLUX_0:
+                        X.p[HG_P_ROCK] = tp[T_R3]; X.p[HG_P_DIRT] = tp[T_D3];
+                        break;
+                    case HG_PASS_THERMAL_FLUX_1:
+                        X.p[HG_P_ROCK] = tp[T_R5]; X.p[HG_P_DIRT] = tp[T_D3];
+                        break;
+                    case HG_PASS_SMOOTH:
+                        X.p[HG_P_ROCK] = tp[T_R5]; X.p[HG_P_DIRT] = tp[T_D6];
+                        break;
+                    default: break;
+                }
+                return X;
+            };
             // fresh adjoint buffers of every pass output, taken up front; the
             // ones that accumulate are zeroed by one launch
             std::vector<std::vector<float*>> fresh_of(order.size());
             std::vector<float*> to_zero;
             for (size_t oi = 0; oi < order.size(); ++oi) {
-                if (order[oi] == HG_PASS_RAIN) continue;
                 for (int p : produced(order[oi])) {
                     float* f = take(w.pool);
                     fresh_of[oi].push_back(f);
@@ -647,11 +666,10 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
             }
             int rc = zero_planes(c, to_zero, pn);
             if (rc != HG_OK) return rc;
-            // pass adjoints in reverse order
+            // pass adjoints in reverse order (rain, water += rain, is the identity)
             for (int oi = int(order.size()) - 1; oi >= 0; --oi) {
                 const int pass = order[oi];
-                if (pass == HG_PASS_RAIN) continue;   // water += rain: identity
-                const Planes& X = xin[oi];
+                const Planes X = x_of(pass);
                 const Planes lo = planes_of(lam);
                 float* fresh[HG_NUM_PLANES];
                 std::memcpy(fresh, lam, sizeof(fresh));

@@ -215,6 +215,12 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
                 io.vy[g] = vy;
                 io.vz[g] = vz;
             }
+            if (io.tb1) {
+                io.tb1[g] = b;
+                io.tb2[g] = d2;
+                io.ts0[g] = s0;
+                io.ts1[g] = s1;
+            }
         }
     }
     __syncthreads();   // all reads of the staged inputs are done
@@ -447,10 +453,14 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
         if (gx >= nx || gy >= ny) return;
         const int li = ly * W0 + lx;
         float r = s_r[li], d = s_d[li];
+        const size_t g = at(gx, gy, nx);
+        if (io.tr5) {
+            io.tr5[g] = r;
+            io.td6[g] = d;
+        }
         if (do_smooth && gx > 0 && gy > 0 && gx < nx - 1 && gy < ny - 1)
             smooth_cell(r, d, s_r[li - 1], s_d[li - 1], s_r[li + 1], s_d[li + 1], s_r[li + W0],
                         s_d[li + W0], s_r[li - W0], s_d[li - W0], k.smooth_m);
-        const size_t g = at(gx, gy, nx);
         io.ro[g] = r;
         io.dout[g] = d;
     });

@@ -38,10 +38,15 @@ struct HydroIO {
     const float *r, *d, *b, *fl, *fr, *ft, *fb, *s0, *s1;   // step input
     float *ro, *dout, *bo, *flo, *fro, *fto, *fbo, *s0o, *s1o; // step output
     float *vx, *vy, *vz;                                     // velocity out (null = not kept)
+    // optional tape of intermediates for the adjoint (all set or all null):
+    // water after rain, water before evaporation, eroded sediment before transport
+    float *tb1 = nullptr, *tb2 = nullptr, *ts0 = nullptr, *ts1 = nullptr;
 };
 struct ThermIO {
     const float *r, *d;
     float *ro, *dout;
+    // optional tape: rock after thermal layer 0, dirt after layer 1 (before smoothing)
+    float *tr5 = nullptr, *td6 = nullptr;
 };
 int hydro_variant_count();
 int thermal_variant_count();
