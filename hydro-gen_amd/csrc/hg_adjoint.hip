@@ -9,18 +9,21 @@
 //  * forward: the ordinary fused steps, saving the 9 state planes every
 //    k_ckpt steps;
 //  * reverse, one segment of k_ckpt steps at a time: recompute the segment
-//    from its checkpoint (fused, saving every step's start state), then for
-//    each step from last to first recompute it pass by pass with the per-pass
-//    kernels into fresh buffers (so every pass input stays available) and
-//    apply the pass adjoints in reverse order.
+//    from its checkpoint with the fused kernels, which also record each
+//    step's start state and its intermediates (water after rain and before
+//    evaporation, velocity, eroded sediment, rock/dirt after erosion and after
+//    each thermal layer); then, for each step from last to first, apply the
+//    pass adjoints in reverse order to those recorded inputs.
 //  * each pass adjoint is a vector-Jacobian product of that pass: a thread
 //    recomputes its cell's forward arithmetic through the same inline code as
 //    the forward kernels (hg_cell.hpp, with a tape of the branch decisions), so
 //    the reverse sweep differentiates exactly the branches the forward took;
 //    clamps, max/min, the talus test and the erosion cases contribute the
 //    derivative of the branch taken (a subgradient at switching points).
-//    Contributions to neighbouring cells and to the semi-Lagrangian taps are
-//    scattered with float atomic adds.
+//  * contributions a cell sends to its neighbours are gathered, not
+//    scattered: each tile evaluates its cells and a one-cell ring into LDS and
+//    every output is written once.  Only semi-Lagrangian back-traces longer
+//    than one cell scatter, with float atomic adds, after the gather.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -56,15 +59,42 @@ __device__ void reduce_coef(double* gcoef, double a, double b, double c, double 
             if (sh[j][0] != 0.0) unsafeAtomicAdd(gcoef + j, sh[j][0]);
 }
 
+// ---------------------------------------------------------------------------
+// Tiled gather.  A pass whose adjoint sends contributions to the 4 or 8
+// neighbours of a cell is evaluated on a TX x TY tile plus a one-cell ring:
+// phase 1 computes every ring and tile cell's contributions into LDS, phase 2
+// sums, for each tile cell, its own part and what its neighbours send it, and
+// writes each output once.  No atomics and no zeroed buffers.
+// ---------------------------------------------------------------------------
+constexpr int GTX = 64, GTY = 8, GNT = 256;
+constexpr int GRW = GTX + 2, GRH = GTY + 2, GNR = GRW * GRH;
+
+template <class Cell, class Gather>
+__device__ __forceinline__ void tile_gather(int nx, int ny, Cell&& cell, Gather&& gather) {
+    const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
+    for (int e = threadIdx.x; e < GNR; e += GNT) {
+        const int ey = e / GRW, ex = e - ey * GRW;
+        const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
+        const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+        const bool tile = ex >= 1 && ex <= GTX && ey >= 1 && ey <= GTY;
+        cell(e, gx, gy, in, tile);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < GTX * GTY; t += GNT) {
+        const int ty = t / GTX, tx = t - ty * GTX;
+        const int gx = x0 + tx, gy = y0 + ty;
+        if (gx < nx && gy < ny) gather((ty + 1) * GRW + tx + 1, gx, gy);
+    }
+}
+
 // ---- hydro_flux ----------------------------------------------------------
-// x: flux input state; lo: adjoint of the flux outputs (water, flux, velocity);
-// li: adjoint of the inputs (fresh zeroed water/flux/velocity planes, rock,
-// dirt and sediment accumulated in place); lw: adjoint of the total height.
-__global__ __launch_bounds__(BX * BY) void k_adj_flux(Planes x, Planes lo, Planes li, float* lw, int nx,
-                                                      int ny, KParams k) {
-    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
-    if (px >= nx || py >= ny) return;
-    const size_t i = at(px, py, nx);
+// x: flux inputs (rock, dirt, water after rain, old flux); lo: adjoint of the
+// flux outputs (water, flux, velocity).  Writes li's water and flux (fresh)
+// and adds the total-height adjoint to li's rock and dirt.
+__global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
+    // per ring/tile cell: outflow adj (4), inflow adj sent to neighbours (4),
+    // total-height adj (own, 4 sent), water adj
+    __shared__ float s_fo[4][GNR], s_fin[4][GNR], s_wn[4][GNR], s_wp[GNR], s_d1[GNR];
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
     const float* B = x.p[HG_P_WATER];
@@ -73,218 +103,275 @@ __global__ __launch_bounds__(BX * BY) void k_adj_flux(Planes x, Planes lo, Plane
         const size_t j = at(xx, yy, nx);
         return R[j] + D[j] + B[j];
     };
-    float f[4] = {x.p[HG_P_FLUX_L][i], x.p[HG_P_FLUX_R][i], x.p[HG_P_FLUX_T][i], x.p[HG_P_FLUX_B][i]};
-    const float w = R[i] + D[i] + B[i];
-    const float wn[4] = {total(px - 1, py), total(px + 1, py), total(px, py + 1), total(px, py - 1)};
-    const float fin[4] = {px > 0 ? x.p[HG_P_FLUX_R][i - 1] : 0.0f, px < nx - 1 ? x.p[HG_P_FLUX_L][i + 1] : 0.0f,
-                          py < ny - 1 ? x.p[HG_P_FLUX_B][i + nx] : 0.0f, py > 0 ? x.p[HG_P_FLUX_T][i - nx] : 0.0f};
-    const unsigned wall = (px == 0 ? 1u : 0u) | (px == nx - 1 ? 2u : 0u) | (py == 0 ? 4u : 0u) |
-                          (py == ny - 1 ? 8u : 0u);
-    FluxTape tp;
-    float vx, vy, vz;
-    (void)flux_cell_t(f, B[i], w, wn, fin, wall, k, vx, vy, vz, tp);
-
-    const float lf[4] = {lo.p[HG_P_FLUX_L][i], lo.p[HG_P_FLUX_R][i], lo.p[HG_P_FLUX_T][i], lo.p[HG_P_FLUX_B][i]};
-    float ld2 = lo.p[HG_P_WATER][i];
-    const float lvx = lo.p[HG_P_VEL_U][i], lvy = lo.p[HG_P_VEL_V][i];
-    float lvz = lo.p[HG_P_VEL_DEPTH][i];
-    float lfo[4] = {0, 0, 0, 0}, lfin[4] = {0, 0, 0, 0}, lwn[4] = {0, 0, 0, 0};
-    float ld1 = 0.0f, lwp = 0.0f;
-    if (vz > 0.0f) {                              // v = n * inv, inv = 1 / vz
-        const float lnx = lvx * tp.inv, lny = lvy * tp.inv;
-        lvz -= (lvx * vx + lvy * vy) * tp.inv;
-        lfin[0] += lnx; lfo[0] -= lnx; lfo[1] += lnx; lfin[1] -= lnx;
-        lfin[3] += lny; lfo[3] -= lny; lfo[2] += lny; lfin[2] -= lny;
-    }
-    ld1 += lvz;                                   // vz = d1 + d2
-    ld2 += lvz;
-    float lsum_out = 0.0f;
-    if (tp.arg > 0.0f) {                          // d2 = max(0, d1 + dt (sum_in - sum_out))
-        ld1 += ld2;
-        const float ls = k.dt * ld2;
-        for (int j = 0; j < 4; ++j) lfin[j] += ls;
-        lsum_out = -ls;
-    }
-    const float K = tp.q < 1.0f ? tp.q : 1.0f;
-    float lK = lsum_out * tp.S;                   // f = K f1, sum_out = K S
-    float lf1[4];
-    for (int j = 0; j < 4; ++j) {
-        lK += lf[j] * tp.f1[j];
-        lf1[j] = (lf[j] + lsum_out) * K;
-    }
-    if (tp.q < 1.0f) {                            // K = q = d1 / (S dt)
-        ld1 += lK * rcp(tp.S * k.dt);
-        const float lS = -lK * tp.q * rcp(tp.S);
-        for (int j = 0; j < 4; ++j) lf1[j] += lS;
-    }
-    const float c = k.dt * k.G;                   // f1 = max(0, E f + c (w - wn))
-    for (int j = 0; j < 4; ++j)
-        if ((tp.act >> j) & 1u) {
-            lfo[j] += k.E * lf1[j];
-            lwp += c * lf1[j];
-            lwn[j] -= c * lf1[j];
-        }
-    aadd(li.p[HG_P_FLUX_L] + i, lfo[0]);
-    aadd(li.p[HG_P_FLUX_R] + i, lfo[1]);
-    aadd(li.p[HG_P_FLUX_T] + i, lfo[2]);
-    aadd(li.p[HG_P_FLUX_B] + i, lfo[3]);
-    if (px > 0) aadd(li.p[HG_P_FLUX_R] + i - 1, lfin[0]);
-    if (px < nx - 1) aadd(li.p[HG_P_FLUX_L] + i + 1, lfin[1]);
-    if (py < ny - 1) aadd(li.p[HG_P_FLUX_B] + i + nx, lfin[2]);
-    if (py > 0) aadd(li.p[HG_P_FLUX_T] + i - nx, lfin[3]);
-    aadd(lw + i, lwp);
-    if (px > 0) aadd(lw + i - 1, lwn[0]);
-    if (px < nx - 1) aadd(lw + i + 1, lwn[1]);
-    if (py < ny - 1) aadd(lw + i + nx, lwn[2]);
-    if (py > 0) aadd(lw + i - nx, lwn[3]);
-    aadd(li.p[HG_P_WATER] + i, ld1);
-}
-
-// w = rock + dirt + water: hand the total-height adjoint to the three planes.
-__global__ __launch_bounds__(256) void k_adj_total(const float* lw, float* lr, float* ld, float* lb, size_t n) {
-    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float v = lw[i];
-    if (v != 0.0f) {
-        aadd(lr + i, v);
-        aadd(ld + i, v);
-        aadd(lb + i, v);
-    }
+    tile_gather(nx, ny,
+        [&](int e, int px, int py, bool in, bool) {
+            float lfo[4] = {0, 0, 0, 0}, lfin[4] = {0, 0, 0, 0}, lwn[4] = {0, 0, 0, 0};
+            float ld1 = 0.0f, lwp = 0.0f;
+            if (in) {
+                const size_t i = at(px, py, nx);
+                float f[4] = {x.p[HG_P_FLUX_L][i], x.p[HG_P_FLUX_R][i], x.p[HG_P_FLUX_T][i], x.p[HG_P_FLUX_B][i]};
+                const float w = R[i] + D[i] + B[i];
+                const float wn[4] = {total(px - 1, py), total(px + 1, py), total(px, py + 1), total(px, py - 1)};
+                const float fin[4] = {px > 0 ? x.p[HG_P_FLUX_R][i - 1] : 0.0f,
+                                      px < nx - 1 ? x.p[HG_P_FLUX_L][i + 1] : 0.0f,
+                                      py < ny - 1 ? x.p[HG_P_FLUX_B][i + nx] : 0.0f,
+                                      py > 0 ? x.p[HG_P_FLUX_T][i - nx] : 0.0f};
+                const unsigned wall = (px == 0 ? 1u : 0u) | (px == nx - 1 ? 2u : 0u) | (py == 0 ? 4u : 0u) |
+                                      (py == ny - 1 ? 8u : 0u);
+                FluxTape tp;
+                float vx, vy, vz;
+                (void)flux_cell_t(f, B[i], w, wn, fin, wall, k, vx, vy, vz, tp);
+                const float lf[4] = {lo.p[HG_P_FLUX_L][i], lo.p[HG_P_FLUX_R][i], lo.p[HG_P_FLUX_T][i],
+                                     lo.p[HG_P_FLUX_B][i]};
+                float ld2 = lo.p[HG_P_WATER][i];
+                const float lvx = lo.p[HG_P_VEL_U][i], lvy = lo.p[HG_P_VEL_V][i];
+                float lvz = lo.p[HG_P_VEL_DEPTH][i];
+                if (vz > 0.0f) {                              // v = n * inv, inv = 1 / vz
+                    const float lnx = lvx * tp.inv, lny = lvy * tp.inv;
+                    lvz -= (lvx * vx + lvy * vy) * tp.inv;
+                    lfin[0] += lnx; lfo[0] -= lnx; lfo[1] += lnx; lfin[1] -= lnx;
+                    lfin[3] += lny; lfo[3] -= lny; lfo[2] += lny; lfin[2] -= lny;
+                }
+                ld1 += lvz;                                   // vz = d1 + d2
+                ld2 += lvz;
+                float lsum_out = 0.0f;
+                if (tp.arg > 0.0f) {                          // d2 = max(0, d1 + dt (sum_in - sum_out))
+                    ld1 += ld2;
+                    const float ls = k.dt * ld2;
+                    for (int j = 0; j < 4; ++j) lfin[j] += ls;
+                    lsum_out = -ls;
+                }
+                const float K = tp.q < 1.0f ? tp.q : 1.0f;
+                float lK = lsum_out * tp.S;                   // f = K f1, sum_out = K S
+                float lf1[4];
+                for (int j = 0; j < 4; ++j) {
+                    lK += lf[j] * tp.f1[j];
+                    lf1[j] = (lf[j] + lsum_out) * K;
+                }
+                if (tp.q < 1.0f) {                            // K = q = d1 / (S dt)
+                    ld1 += lK * rcp(tp.S * k.dt);
+                    const float lS = -lK * tp.q * rcp(tp.S);
+                    for (int j = 0; j < 4; ++j) lf1[j] += lS;
+                }
+                const float c = k.dt * k.G;                   // f1 = max(0, E f + c (w - wn))
+                for (int j = 0; j < 4; ++j)
+                    if ((tp.act >> j) & 1u) {
+                        lfo[j] += k.E * lf1[j];
+                        lwp += c * lf1[j];
+                        lwn[j] -= c * lf1[j];
+                    }
+                // contributions to neighbours outside the grid are dropped
+                if (px == 0) { lfin[0] = 0.0f; lwn[0] = 0.0f; }
+                if (px == nx - 1) { lfin[1] = 0.0f; lwn[1] = 0.0f; }
+                if (py == ny - 1) { lfin[2] = 0.0f; lwn[2] = 0.0f; }
+                if (py == 0) { lfin[3] = 0.0f; lwn[3] = 0.0f; }
+            }
+            for (int j = 0; j < 4; ++j) {
+                s_fo[j][e] = lfo[j];
+                s_fin[j][e] = lfin[j];
+                s_wn[j][e] = lwn[j];
+            }
+            s_wp[e] = lwp;
+            s_d1[e] = ld1;
+        },
+        [&](int e, int px, int py) {
+            const size_t i = at(px, py, nx);
+            // neighbour q = p - 1 sends its R-inflow adjoint (index 1) to p's L
+            // outflow, q = p + 1 its index 0 to p's R, q = p - nx its index 2
+            // to p's B, q = p + nx its index 3 to p's T
+            li.p[HG_P_FLUX_L][i] = s_fo[0][e] + s_fin[1][e - 1];
+            li.p[HG_P_FLUX_R][i] = s_fo[1][e] + s_fin[0][e + 1];
+            li.p[HG_P_FLUX_T][i] = s_fo[2][e] + s_fin[3][e + GRW];
+            li.p[HG_P_FLUX_B][i] = s_fo[3][e] + s_fin[2][e - GRW];
+            const float lw = s_wp[e] + s_wn[0][e + 1] + s_wn[1][e - 1] + s_wn[2][e - GRW] + s_wn[3][e + GRW];
+            li.p[HG_P_WATER][i] = s_d1[e] + lw;                // w = rock + dirt + water
+            li.p[HG_P_ROCK][i] += lw;
+            li.p[HG_P_DIRT][i] += lw;
+        });
 }
 
 // ---- hydro_erosion -------------------------------------------------------
-__global__ __launch_bounds__(BX * BY) void k_adj_erosion(Planes x, Planes lo, Planes li, int nx, int ny,
-                                                         KParams k, double* gcoef) {
-    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
+// Writes li's rock, dirt and sediment (fresh) and adds to li's velocity.
+__global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
+                                                     double* gcoef) {
+    __shared__ float s_dx[GNR], s_dz[GNR], s_r[GNR], s_d[GNR];
+    const float* R = x.p[HG_P_ROCK];
+    const float* D = x.p[HG_P_DIRT];
+    auto rd = [&](int xx, int yy) -> float {
+        if (xx < 0 || yy < 0 || xx >= nx || yy >= ny) return 0.0f;
+        const size_t j = at(xx, yy, nx);
+        return R[j] + D[j];
+    };
     double gKc = 0, gKs = 0, gKd = 0;
-    if (px < nx && py < ny) {
-        const size_t i = at(px, py, nx);
-        const float* R = x.p[HG_P_ROCK];
-        const float* D = x.p[HG_P_DIRT];
-        auto rd = [&](int xx, int yy) -> float {
-            if (xx < 0 || yy < 0 || xx >= nx || yy >= ny) return 0.0f;
-            const size_t j = at(xx, yy, nx);
-            return R[j] + D[j];
-        };
-        const float r_in = R[i], d_in = D[i], s0_in = x.p[HG_P_SED_ROCK][i], s1_in = x.p[HG_P_SED_DIRT][i];
-        const float vx = x.p[HG_P_VEL_U][i], vy = x.p[HG_P_VEL_V][i], vz = x.p[HG_P_VEL_DEPTH][i];
-        float r = r_in, d = d_in, s0 = s0_in, s1 = s1_in;
-        ErosionTape tp;
-        erosion_cell_t(r, d, s0, s1, vx, vy, vz, rd(px - 1, py), rd(px + 1, py), rd(px, py + 1), rd(px, py - 1),
-                       k, tp);
-        const float lr = lo.p[HG_P_ROCK][i], ld = lo.p[HG_P_DIRT][i];
-        const float ls0 = lo.p[HG_P_SED_ROCK][i], ls1 = lo.p[HG_P_SED_DIRT][i];
-        // conversion s1 += s0 kc, s0 -= s0 kc
-        const float kc = k.Kconv * k.dt;
-        const float ls0c = ls0 * (1.0f - kc) + ls1 * kc, ls1c = ls1;
-        // rock layer: (r, s0, c0) -> (r', s0c)
-        float lr_in = lr, ls0_in = ls0c, lc0 = 0.0f, lcap0 = 0.0f, lcap = 0.0f;
-        if (tp.case0 == 0 || tp.case0 == 2) {
-            const float a = k.dt * (tp.case0 == 0 ? k.Ks[0] : k.Kd[0]);
-            ls0_in = lr * a + ls0c * (1.0f - a);
-            lc0 = (ls0c - lr) * a;
-            if (tp.case0 == 0) gKs += double(k.dt) * double(tp.c0 - s0_in) * double(ls0c - lr);
-            else gKd += double(k.dt) * double(s0_in - tp.c0) * double(lr - ls0c);
-        } else if (tp.case0 == 1) {               // rock exhausted: r' = 0, s0c = s0 + r
-            lr_in = ls0c;
-            ls0_in = ls0c;
-        }
-        if (tp.case0 != 3 && tp.c0 > 0.0f) {      // c0 = max(0, cap0 - cap)
-            lcap0 += lc0;
-            lcap -= lc0;
-        }
-        // dirt layer: (d, s1, c1) -> (d', s1c); cap = d when exhausted
-        float ld_in = ld, ls1_in = ls1c, lc1 = 0.0f;
-        if (tp.case1 == 0 || tp.case1 == 2) {
-            const float a = k.dt * (tp.case1 == 0 ? k.Ks[1] : k.Kd[1]);
-            ls1_in = ld * a + ls1c * (1.0f - a);
-            lc1 = (ls1c - ld) * a;
-            if (tp.case1 == 0) gKs += double(k.dt) * double(tp.c1 - s1_in) * double(ls1c - ld);
-            else gKd += double(k.dt) * double(s1_in - tp.c1) * double(ld - ls1c);
-        } else {                                  // exhausted: d' = 0, s1c = s1 + d, cap = d
-            ld_in = ls1c + lcap;
-            ls1_in = ls1c;
-        }
-        if (tp.c1 > 0.0f) lcap0 += lc1;           // c1 = max(0, cap0)
-        // cap0 = Kc * max(0.02, sin_a) * ero
-        const float m = fmaxf(0.02f, tp.sin_a);
-        const float ero = tp.slow ? tp.ero0 * (1.0f - tp.sm) : tp.ero0;
-        gKc += double(lcap0) * double(m) * double(ero);
-        const float lm = lcap0 * k.Kc * ero, lero = lcap0 * k.Kc * m;
-        float lhl = 0.0f, lhr = 0.0f, lht = 0.0f, lhb = 0.0f;
-        if (tp.sin_a > 0.02f && tp.sin_a > 0.0f) {  // sin_a = sqrt(g2 / (g2 + 4))
-            const float q = tp.g2 + 4.0f;
-            const float lg2 = lm * 2.0f * rcp(tp.sin_a * q * q);
-            const float ldx = 2.0f * tp.dx * lg2, ldz = 2.0f * tp.dz * lg2;
-            lhr = ldx; lhl = -ldx; lht = ldz; lhb = -ldz;
-        }
-        float lero0 = lero, lvz = 0.0f;
-        if (tp.slow) {
-            lero0 = lero * (1.0f - tp.sm);
-            if (tp.t_inner) {                     // sm = t^2 (3 - 2t), t = (vz - 1e-3) / (5e-4 - 1e-3)
-                const float lsm = -lero * tp.ero0;
-                lvz = lsm * 6.0f * tp.tt * (1.0f - tp.tt) * (1.0f / (5e-4f - 1e-3f));
+    tile_gather(nx, ny,
+        [&](int e, int px, int py, bool in, bool tile) {
+            float lr_in = 0.0f, ld_in = 0.0f, ldx = 0.0f, ldz = 0.0f;
+            if (in) {
+                const size_t i = at(px, py, nx);
+                const float r_in = R[i], d_in = D[i], s0_in = x.p[HG_P_SED_ROCK][i], s1_in = x.p[HG_P_SED_DIRT][i];
+                const float vx = x.p[HG_P_VEL_U][i], vy = x.p[HG_P_VEL_V][i], vz = x.p[HG_P_VEL_DEPTH][i];
+                float r = r_in, d = d_in, s0 = s0_in, s1 = s1_in;
+                ErosionTape tp;
+                erosion_cell_t(r, d, s0, s1, vx, vy, vz, rd(px - 1, py), rd(px + 1, py), rd(px, py + 1),
+                               rd(px, py - 1), k, tp);
+                const float lr = lo.p[HG_P_ROCK][i], ld = lo.p[HG_P_DIRT][i];
+                const float ls0 = lo.p[HG_P_SED_ROCK][i], ls1 = lo.p[HG_P_SED_DIRT][i];
+                double cKc = 0, cKs = 0, cKd = 0;
+                // conversion s1 += s0 kc, s0 -= s0 kc
+                const float kc = k.Kconv * k.dt;
+                const float ls0c = ls0 * (1.0f - kc) + ls1 * kc, ls1c = ls1;
+                // rock layer: (r, s0, c0) -> (r', s0c)
+                float ls0_in = ls0c, lc0 = 0.0f, lcap0 = 0.0f, lcap = 0.0f;
+                lr_in = lr;
+                if (tp.case0 == 0 || tp.case0 == 2) {
+                    const float a = k.dt * (tp.case0 == 0 ? k.Ks[0] : k.Kd[0]);
+                    ls0_in = lr * a + ls0c * (1.0f - a);
+                    lc0 = (ls0c - lr) * a;
+                    if (tp.case0 == 0) cKs += double(k.dt) * double(tp.c0 - s0_in) * double(ls0c - lr);
+                    else cKd += double(k.dt) * double(s0_in - tp.c0) * double(lr - ls0c);
+                } else if (tp.case0 == 1) {               // rock exhausted: r' = 0, s0c = s0 + r
+                    lr_in = ls0c;
+                    ls0_in = ls0c;
+                }
+                if (tp.case0 != 3 && tp.c0 > 0.0f) {      // c0 = max(0, cap0 - cap)
+                    lcap0 += lc0;
+                    lcap -= lc0;
+                }
+                // dirt layer: (d, s1, c1) -> (d', s1c); cap = d when exhausted
+                float ls1_in = ls1c, lc1 = 0.0f;
+                ld_in = ld;
+                if (tp.case1 == 0 || tp.case1 == 2) {
+                    const float a = k.dt * (tp.case1 == 0 ? k.Ks[1] : k.Kd[1]);
+                    ls1_in = ld * a + ls1c * (1.0f - a);
+                    lc1 = (ls1c - ld) * a;
+                    if (tp.case1 == 0) cKs += double(k.dt) * double(tp.c1 - s1_in) * double(ls1c - ld);
+                    else cKd += double(k.dt) * double(s1_in - tp.c1) * double(ld - ls1c);
+                } else {                                  // exhausted: d' = 0, s1c = s1 + d, cap = d
+                    ld_in = ls1c + lcap;
+                    ls1_in = ls1c;
+                }
+                if (tp.c1 > 0.0f) lcap0 += lc1;           // c1 = max(0, cap0)
+                // cap0 = Kc * max(0.02, sin_a) * ero
+                const float m = fmaxf(0.02f, tp.sin_a);
+                const float ero = tp.slow ? tp.ero0 * (1.0f - tp.sm) : tp.ero0;
+                cKc += double(lcap0) * double(m) * double(ero);
+                const float lm = lcap0 * k.Kc * ero, lero = lcap0 * k.Kc * m;
+                if (tp.sin_a > 0.02f && tp.sin_a > 0.0f) {  // sin_a = sqrt(g2 / (g2 + 4))
+                    const float q = tp.g2 + 4.0f;
+                    const float lg2 = lm * 2.0f * rcp(tp.sin_a * q * q);
+                    ldx = 2.0f * tp.dx * lg2;             // +ldx to p+1, -ldx to p-1
+                    ldz = 2.0f * tp.dz * lg2;             // +ldz to p+nx, -ldz to p-nx
+                }
+                float lero0 = lero, lvz = 0.0f;
+                if (tp.slow) {
+                    lero0 = lero * (1.0f - tp.sm);
+                    if (tp.t_inner) {                     // sm = t^2 (3 - 2t), t = (vz - 1e-3) / (5e-4 - 1e-3)
+                        const float lsm = -lero * tp.ero0;
+                        lvz = lsm * 6.0f * tp.tt * (1.0f - tp.tt) * (1.0f / (5e-4f - 1e-3f));
+                    }
+                }
+                float lvx = 0.0f, lvy = 0.0f;
+                if (tp.ero0 > 0.0f) {
+                    const float inv = rcp(tp.ero0);
+                    lvx = lero0 * vx * inv;
+                    lvy = lero0 * vy * inv;
+                }
+                if (tile) {                               // own outputs, counted once
+                    li.p[HG_P_SED_ROCK][i] = ls0_in;
+                    li.p[HG_P_SED_DIRT][i] = ls1_in;
+                    li.p[HG_P_VEL_U][i] += lvx;
+                    li.p[HG_P_VEL_V][i] += lvy;
+                    li.p[HG_P_VEL_DEPTH][i] += lvz;
+                    gKc += cKc; gKs += cKs; gKd += cKd;
+                }
             }
-        }
-        float lvx = 0.0f, lvy = 0.0f;
-        if (tp.ero0 > 0.0f) {
-            const float inv = rcp(tp.ero0);
-            lvx = lero0 * vx * inv;
-            lvy = lero0 * vy * inv;
-        }
-        aadd(li.p[HG_P_ROCK] + i, lr_in);
-        aadd(li.p[HG_P_DIRT] + i, ld_in);
-        li.p[HG_P_SED_ROCK][i] = ls0_in;
-        li.p[HG_P_SED_DIRT][i] = ls1_in;
-        li.p[HG_P_VEL_U][i] += lvx;
-        li.p[HG_P_VEL_V][i] += lvy;
-        li.p[HG_P_VEL_DEPTH][i] += lvz;
-        auto nb = [&](int xx, int yy, float v) {
-            if (xx < 0 || yy < 0 || xx >= nx || yy >= ny || v == 0.0f) return;
-            const size_t j = at(xx, yy, nx);
-            aadd(li.p[HG_P_ROCK] + j, v);
-            aadd(li.p[HG_P_DIRT] + j, v);
-        };
-        nb(px - 1, py, lhl);
-        nb(px + 1, py, lhr);
-        nb(px, py + 1, lht);
-        nb(px, py - 1, lhb);
-    }
+            s_dx[e] = ldx;
+            s_dz[e] = ldz;
+            s_r[e] = lr_in;
+            s_d[e] = ld_in;
+        },
+        [&](int e, int px, int py) {
+            const size_t i = at(px, py, nx);
+            const float nb = (s_dx[e - 1] - s_dx[e + 1]) + (s_dz[e - GRW] - s_dz[e + GRW]);
+            li.p[HG_P_ROCK][i] = s_r[e] + nb;
+            li.p[HG_P_DIRT][i] = s_d[e] + nb;
+        });
     reduce_coef(gcoef, gKc, gKs, gKd, 0.0);
 }
 
 // ---- sediment_transport --------------------------------------------------
+// The bilinear gather's adjoint is a scatter to the four taps of each cell's
+// back-trace.  A cell whose taps lie within one cell of it ("near") is
+// gathered by the taps themselves (k_adj_transport); the few others scatter
+// with atomics afterwards (k_adj_transport_far).
+struct Taps {
+    int ix, iy, ix1, iy1;
+    float w00, w10, w01, w11;
+    bool near;
+};
+__device__ __forceinline__ Taps taps_of(int qx, int qy, float vx, float vy, int nx, int ny, float dt) {
+    Taps t;
+    float bx, by;
+    back_trace(qx, qy, vx, vy, dt, nx, ny, bx, by);
+    t.ix = int(bx); t.iy = int(by);
+    const float fx = bx - float(t.ix), fy = by - float(t.iy);
+    t.ix1 = min(t.ix + 1, nx - 1); t.iy1 = min(t.iy + 1, ny - 1);
+    t.w00 = (1.0f - fx) * (1.0f - fy); t.w10 = fx * (1.0f - fy); t.w01 = (1.0f - fx) * fy; t.w11 = fx * fy;
+    t.near = t.ix >= qx - 1 && t.ix1 <= qx + 1 && t.iy >= qy - 1 && t.iy1 <= qy + 1;
+    return t;
+}
+
 __global__ __launch_bounds__(BX * BY) void k_adj_transport(Planes x, Planes lo, Planes li, int nx, int ny,
                                                            KParams k, double* gcoef) {
     const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
     double gKe = 0;
     if (px < nx && py < ny) {
         const size_t i = at(px, py, nx);
-        const float bxr = float(px) - x.p[HG_P_VEL_U][i] * k.dt;
-        const float byr = float(py) - x.p[HG_P_VEL_V][i] * k.dt;
+        const float* VX = x.p[HG_P_VEL_U];
+        const float* VY = x.p[HG_P_VEL_V];
+        // sediment: gather from the near cells whose taps include p
+        float a0 = 0.0f, a1 = 0.0f;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int qx = px + dx, qy = py + dy;
+                if (qx < 0 || qy < 0 || qx >= nx || qy >= ny) continue;
+                const size_t q = at(qx, qy, nx);
+                const Taps t = taps_of(qx, qy, VX[q], VY[q], nx, ny, k.dt);
+                if (!t.near) continue;
+                float w = 0.0f;
+                if (t.iy == py) {
+                    if (t.ix == px) w += t.w00;
+                    if (t.ix1 == px) w += t.w10;
+                }
+                if (t.iy1 == py) {
+                    if (t.ix == px) w += t.w01;
+                    if (t.ix1 == px) w += t.w11;
+                }
+                if (w != 0.0f) {
+                    a0 += lo.p[HG_P_SED_ROCK][q] * w;
+                    a1 += lo.p[HG_P_SED_DIRT][q] * w;
+                }
+            }
+        li.p[HG_P_SED_ROCK][i] = a0;
+        li.p[HG_P_SED_DIRT][i] = a1;
+        // own velocity and water
+        const float vx = VX[i], vy = VY[i];
+        const float bxr = float(px) - vx * k.dt;
+        const float byr = float(py) - vy * k.dt;
+        const Taps t = taps_of(px, py, vx, vy, nx, ny, k.dt);
+        const size_t j00 = at(t.ix, t.iy, nx), j10 = at(t.ix1, t.iy, nx), j01 = at(t.ix, t.iy1, nx),
+                     j11 = at(t.ix1, t.iy1, nx);
         float bx, by;
-        back_trace(px, py, x.p[HG_P_VEL_U][i], x.p[HG_P_VEL_V][i], k.dt, nx, ny, bx, by);
-        const int ix = int(bx), iy = int(by);
-        const float fx = bx - float(ix), fy = by - float(iy);
-        const int ix1 = min(ix + 1, nx - 1), iy1 = min(iy + 1, ny - 1);
-        const size_t j00 = at(ix, iy, nx), j10 = at(ix1, iy, nx), j01 = at(ix, iy1, nx), j11 = at(ix1, iy1, nx);
+        back_trace(px, py, vx, vy, k.dt, nx, ny, bx, by);
+        const float fx = bx - float(t.ix), fy = by - float(t.iy);
         const float l0 = lo.p[HG_P_SED_ROCK][i], l1 = lo.p[HG_P_SED_DIRT][i];
-        const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy), w01 = (1.0f - fx) * fy, w11 = fx * fy;
-        aadd(li.p[HG_P_SED_ROCK] + j00, l0 * w00);
-        aadd(li.p[HG_P_SED_ROCK] + j10, l0 * w10);
-        aadd(li.p[HG_P_SED_ROCK] + j01, l0 * w01);
-        aadd(li.p[HG_P_SED_ROCK] + j11, l0 * w11);
-        aadd(li.p[HG_P_SED_DIRT] + j00, l1 * w00);
-        aadd(li.p[HG_P_SED_DIRT] + j10, l1 * w10);
-        aadd(li.p[HG_P_SED_DIRT] + j01, l1 * w01);
-        aadd(li.p[HG_P_SED_DIRT] + j11, l1 * w11);
         float lbx = 0.0f, lby = 0.0f;
         for (int c = 0; c < 2; ++c) {
             const float* S = x.p[c == 0 ? HG_P_SED_ROCK : HG_P_SED_DIRT];
-            const float a00 = S[j00], a10 = S[j10], a01 = S[j01], a11 = S[j11];
+            const float s00 = S[j00], s10 = S[j10], s01 = S[j01], s11 = S[j11];
             const float l = c == 0 ? l0 : l1;
-            lbx += l * ((1.0f - fy) * (a10 - a00) + fy * (a11 - a01));
-            lby += l * (lerp1(a01, a11, fx) - lerp1(a00, a10, fx));
+            lbx += l * ((1.0f - fy) * (s10 - s00) + fy * (s11 - s01));
+            lby += l * (lerp1(s01, s11, fx) - lerp1(s00, s10, fx));
         }
         // first writer of the velocity adjoint in a step (velocity is an
         // intermediate of the step: nothing after this step reads it)
@@ -298,121 +385,153 @@ __global__ __launch_bounds__(BX * BY) void k_adj_transport(Planes x, Planes lo, 
     reduce_coef(gcoef, 0.0, 0.0, 0.0, gKe);
 }
 
+__global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, Planes li, int nx, int ny,
+                                                           KParams k) {
+    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
+    if (px >= nx || py >= ny) return;
+    const size_t i = at(px, py, nx);
+    const Taps t = taps_of(px, py, x.p[HG_P_VEL_U][i], x.p[HG_P_VEL_V][i], nx, ny, k.dt);
+    if (t.near) return;
+    const float l0 = lo.p[HG_P_SED_ROCK][i], l1 = lo.p[HG_P_SED_DIRT][i];
+    const size_t j00 = at(t.ix, t.iy, nx), j10 = at(t.ix1, t.iy, nx), j01 = at(t.ix, t.iy1, nx),
+                 j11 = at(t.ix1, t.iy1, nx);
+    aadd(li.p[HG_P_SED_ROCK] + j00, l0 * t.w00);
+    aadd(li.p[HG_P_SED_ROCK] + j10, l0 * t.w10);
+    aadd(li.p[HG_P_SED_ROCK] + j01, l0 * t.w01);
+    aadd(li.p[HG_P_SED_ROCK] + j11, l0 * t.w11);
+    aadd(li.p[HG_P_SED_DIRT] + j00, l1 * t.w00);
+    aadd(li.p[HG_P_SED_DIRT] + j10, l1 * t.w10);
+    aadd(li.p[HG_P_SED_DIRT] + j01, l1 * t.w01);
+    aadd(li.p[HG_P_SED_DIRT] + j11, l1 * t.w11);
+}
+
 __device__ const int TDX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
 __device__ const int TDY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
 
-// ---- thermal_erosion (outflow), layer `layer` ----------------------------
-__global__ __launch_bounds__(BX * BY) void k_adj_thermal_flux(Planes x, Planes lo, Planes li, int nx, int ny,
-                                                              int layer, KParams k) {
-    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
-    if (px >= nx || py >= ny) return;
-    const size_t i = at(px, py, nx);
+// ---- thermal layer `layer`: thermal_erosion then thermal_transport --------
+// Forward: T_k = outflow(h_in); h_out(p) = h_in(p) + sum_k (T_opp(k)(p+k) - T_k(p)).
+// Adjoint: dL/dT_k(p) = lh(p+k) - lh(p) (lh = dL/dh_out of the layer), then the
+// outflow's vector-Jacobian product sends -dL/dD_k to neighbour p+k and the
+// sum to p (both rock and dirt for layer 1, whose D_k spans both layers).
+// x: the layer's input rock and dirt.  Writes li's layer plane (fresh) and,
+// for layer 1, adds the rock part to li's rock in place.
+__global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes li, int nx, int ny, int layer,
+                                                     KParams k) {
+    __shared__ float s_lD[8][GNR], s_self[GNR], s_hl[GNR];
     const float* R = x.p[HG_P_ROCK];
     const float* Dt = x.p[HG_P_DIRT];
-    const float r = R[i], d = Dt[i];
-    float Dk[8];
-    bool ok[8];
-    size_t jn[8];
+    const float* LH = lo.p[layer == 0 ? HG_P_ROCK : HG_P_DIRT];
+    tile_gather(nx, ny,
+        [&](int e, int px, int py, bool in, bool) {
+            float lD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            float lself = 0.0f, lh_layer = 0.0f;
+            if (in) {
+                const size_t i = at(px, py, nx);
+                const float r = R[i], d = Dt[i];
+                float Dk[8];
+                bool ok[8];
+                size_t jn[8];
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-        const int xx = px + TDX[kk], yy = py + TDY[kk];
-        ok[kk] = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
-        jn[kk] = ok[kk] ? at(xx, yy, nx) : 0;
-        const float rn = ok[kk] ? R[jn[kk]] : HG_OOB_HEIGHT;
-        float v = r - rn;
-        if (layer == 1) v = v + (d - (ok[kk] ? Dt[jn[kk]] : HG_OOB_HEIGHT));
-        Dk[kk] = v;
-    }
-    ThermalTape tp;
-    const float g = thermal_g_t(Dk, layer == 0 ? r : d, layer, k, tp);
-    if (!(tp.bk > 0.0f)) return;                  // no outflow: outputs are constant 0
-    float lD[8], lg = 0.0f;
+                for (int kk = 0; kk < 8; ++kk) {
+                    const int xx = px + TDX[kk], yy = py + TDY[kk];
+                    ok[kk] = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
+                    jn[kk] = ok[kk] ? at(xx, yy, nx) : 0;
+                    const float rn = ok[kk] ? R[jn[kk]] : HG_OOB_HEIGHT;
+                    float v = r - rn;
+                    if (layer == 1) v = v + (d - (ok[kk] ? Dt[jn[kk]] : HG_OOB_HEIGHT));
+                    Dk[kk] = v;
+                }
+                ThermalTape tp;
+                const float g = thermal_g_t(Dk, layer == 0 ? r : d, layer, k, tp);
+                if (tp.bk > 0.0f) {
+                    const float lh = LH[i];
+                    float lg = 0.0f;
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-        const bool m = talus_marked(Dk[kk], kk, layer, k);
-        const float lout = lo.p[HG_P_THERM_L + kk][i];
-        lD[kk] = m ? lout * g : 0.0f;
-        if (m) lg += lout * Dk[kk];
-    }
-    const float rb = rcp(tp.bk);                  // g = S / bk
-    const float lS = lg * rb, lbk = -lg * g * rb;
+                    for (int kk = 0; kk < 8; ++kk) {
+                        const bool m = talus_marked(Dk[kk], kk, layer, k);
+                        const float lout = (ok[kk] ? LH[jn[kk]] : 0.0f) - lh;   // dL/dT_k
+                        lD[kk] = m ? lout * g : 0.0f;
+                        if (m) lg += lout * Dk[kk];
+                    }
+                    const float rb = rcp(tp.bk);              // g = S / bk
+                    const float lS = lg * rb, lbk = -lg * g * rb;
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk)
-        if (talus_marked(Dk[kk], kk, layer, k)) lD[kk] += lbk;
-    const float A = k.dt * k.Kspeed[layer] * 0.5f;  // S = A sharp^3 H
-    const float sh3 = tp.sharp * (tp.sharp * tp.sharp);
-    const float lsh3 = lS * A * tp.H, lH = lS * A * sh3;
-    if (tp.sharp_active) {                        // sharp = 1 + atan(rmax) - Kalpha
-        const float lrmax = lsh3 * 3.0f * tp.sharp * tp.sharp * rcp(1.0f + tp.rmax * tp.rmax);
-        if (tp.mc >= tp.md * 0.70710678118654752f) lD[tp.kc] += lrmax;
-        else lD[tp.kd] += lrmax * 0.70710678118654752f;
-    }
-    float lh_layer = 0.0f;
-    if (tp.h_is_layer) lh_layer = lH;             // H = min(h_l, max(0, mc, md))
-    else if (fmaxf(tp.mc, tp.md) > 0.0f) lD[tp.mc >= tp.md ? tp.kc : tp.kd] += lH;
-    float lself = 0.0f;
+                    for (int kk = 0; kk < 8; ++kk)
+                        if (talus_marked(Dk[kk], kk, layer, k)) lD[kk] += lbk;
+                    const float A = k.dt * k.Kspeed[layer] * 0.5f;  // S = A sharp^3 H
+                    const float sh3 = tp.sharp * (tp.sharp * tp.sharp);
+                    const float lsh3 = lS * A * tp.H, lH = lS * A * sh3;
+                    if (tp.sharp_active) {                    // sharp = 1 + atan(rmax) - Kalpha
+                        const float lrmax = lsh3 * 3.0f * tp.sharp * tp.sharp * rcp(1.0f + tp.rmax * tp.rmax);
+                        if (tp.mc >= tp.md * 0.70710678118654752f) lD[tp.kc] += lrmax;
+                        else lD[tp.kd] += lrmax * 0.70710678118654752f;
+                    }
+                    if (tp.h_is_layer) lh_layer = lH;         // H = min(h_l, max(0, mc, md))
+                    else if (fmaxf(tp.mc, tp.md) > 0.0f) lD[tp.mc >= tp.md ? tp.kc : tp.kd] += lH;
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-        lself += lD[kk];
-        if (ok[kk] && lD[kk] != 0.0f) {
-            aadd(li.p[HG_P_ROCK] + jn[kk], -lD[kk]);
-            if (layer == 1) aadd(li.p[HG_P_DIRT] + jn[kk], -lD[kk]);
-        }
-    }
-    aadd(li.p[HG_P_ROCK] + i, lself + (layer == 0 ? lh_layer : 0.0f));
-    if (layer == 1) aadd(li.p[HG_P_DIRT] + i, lself + lh_layer);
-}
-
-// ---- thermal_transport, layer `layer` (gather) ---------------------------
-// h'(p) = h(p) + sum_k (T_opp(k)(p+k) - T_k(p)): T_j(q) enters q's own change
-// with -1 and that of q + dir_j with +1.
-__global__ __launch_bounds__(BX * BY) void k_adj_thermal_transport(Planes lo, Planes li, int nx, int ny,
-                                                                   int layer) {
-    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
-    if (px >= nx || py >= ny) return;
-    const size_t i = at(px, py, nx);
-    const int hp = layer == 0 ? HG_P_ROCK : HG_P_DIRT;
-    const float lh = lo.p[hp][i];
-    li.p[hp][i] = lh;
+                    for (int kk = 0; kk < 8; ++kk) {
+                        lself += lD[kk];
+                        if (!ok[kk]) lD[kk] = 0.0f;           // nothing is sent off the grid
+                    }
+                }
+            }
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-        const int xx = px + TDX[kk], yy = py + TDY[kk];
-        const bool ok = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
-        li.p[HG_P_THERM_L + kk][i] = (ok ? lo.p[hp][at(xx, yy, nx)] : 0.0f) - lh;   // only reader of T_kk
-    }
+            for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = lD[kk];
+            s_self[e] = lself;
+            s_hl[e] = lh_layer;
+        },
+        [&](int e, int px, int py) {
+            const size_t i = at(px, py, nx);
+            float recv = 0.0f;                                // from q = p - dir_k: -dL/dD_k(q)
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) recv -= s_lD[kk][e - (TDX[kk] + TDY[kk] * GRW)];
+            const float shared = s_self[e] + recv;
+            if (layer == 0) {
+                li.p[HG_P_ROCK][i] = LH[i] + shared + s_hl[e];
+            } else {
+                li.p[HG_P_DIRT][i] = LH[i] + shared + s_hl[e];
+                li.p[HG_P_ROCK][i] += shared;
+            }
+        });
 }
 
 // ---- smoothing -----------------------------------------------------------
+// y = x + m (mean5 - x) at an extremum.  Gather: p keeps its own share and
+// receives 0.2 m dL/dy(q) from each interior neighbour q that is an extremum.
+// Writes li's rock and dirt (fresh).
 __global__ __launch_bounds__(BX * BY) void k_adj_smooth(Planes x, Planes lo, Planes li, int nx, int ny,
                                                         KParams k) {
     const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
     if (px >= nx || py >= ny) return;
     const size_t i = at(px, py, nx);
-    const float lr = lo.p[HG_P_ROCK][i], ld = lo.p[HG_P_DIRT][i];
-    if (!(px > 0 && py > 0 && px < nx - 1 && py < ny - 1)) {
-        aadd(li.p[HG_P_ROCK] + i, lr);
-        aadd(li.p[HG_P_DIRT] + i, ld);
-        return;
-    }
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
-    bool er, eg;
-    smooth_flags(R[i], D[i], R[i - 1], D[i - 1], R[i + 1], D[i + 1], R[i + nx], D[i + nx], R[i - nx], D[i - nx],
-                 er, eg);
-    const float m = k.smooth_m;                   // y = x + m (avg - x) when an extremum
-    const size_t nbs[4] = {i - 1, i + 1, i + nx, i - nx};
-    if (er) {
-        aadd(li.p[HG_P_ROCK] + i, lr * (1.0f - 0.8f * m));
-        for (int j = 0; j < 4; ++j) aadd(li.p[HG_P_ROCK] + nbs[j], lr * 0.2f * m);
-    } else {
-        aadd(li.p[HG_P_ROCK] + i, lr);
+    const float m = k.smooth_m;
+    auto interior = [&](int xx, int yy) { return xx > 0 && yy > 0 && xx < nx - 1 && yy < ny - 1; };
+    auto flags = [&](size_t j, bool& er, bool& eg) {
+        smooth_flags(R[j], D[j], R[j - 1], D[j - 1], R[j + 1], D[j + 1], R[j + nx], D[j + nx], R[j - nx],
+                     D[j - nx], er, eg);
+    };
+    const float lr = lo.p[HG_P_ROCK][i], ld = lo.p[HG_P_DIRT][i];
+    float ar = lr, ad = ld;
+    if (interior(px, py)) {
+        bool er, eg;
+        flags(i, er, eg);
+        if (er) ar = lr * (1.0f - 0.8f * m);
+        if (eg) ad = ld * (1.0f - 0.8f * m);
     }
-    if (eg) {
-        aadd(li.p[HG_P_DIRT] + i, ld * (1.0f - 0.8f * m));
-        for (int j = 0; j < 4; ++j) aadd(li.p[HG_P_DIRT] + nbs[j], ld * 0.2f * m);
-    } else {
-        aadd(li.p[HG_P_DIRT] + i, ld);
+    const int NXo[4] = {-1, 1, 0, 0}, NYo[4] = {0, 0, 1, -1};
+    for (int j = 0; j < 4; ++j) {
+        const int qx = px + NXo[j], qy = py + NYo[j];
+        if (!interior(qx, qy)) continue;
+        const size_t q = at(qx, qy, nx);
+        bool er, eg;
+        flags(q, er, eg);
+        if (er) ar += lo.p[HG_P_ROCK][q] * 0.2f * m;
+        if (eg) ad += lo.p[HG_P_DIRT][q] * 0.2f * m;
     }
+    li.p[HG_P_ROCK][i] = ar;
+    li.p[HG_P_DIRT][i] = ad;
 }
 
 // Zeroes `count` planes of n floats (n a multiple of 4: n = side^2, side % 8 == 0).
@@ -432,8 +551,9 @@ __global__ __launch_bounds__(256) void k_zero_planes(PlaneList l, int count, siz
 
 namespace {
 
-// Planes each reference pass produces (the adjoint of their input value is
-// zero; their input adjoint starts from a fresh zeroed buffer).
+// Planes each reverse unit produces: their adjoint before the unit is written
+// in full into a fresh buffer (the thermal units stand for outflow + transport
+// of a layer).
 std::vector<int> produced(int pass) {
     switch (pass) {
         case HG_PASS_RAIN: return {HG_P_WATER};
@@ -441,27 +561,10 @@ std::vector<int> produced(int pass) {
                                    HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH};
         case HG_PASS_EROSION: return {HG_P_ROCK, HG_P_DIRT, HG_P_SED_ROCK, HG_P_SED_DIRT};
         case HG_PASS_TRANSPORT: return {HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER};
-        case HG_PASS_THERMAL_FLUX_0:
-        case HG_PASS_THERMAL_FLUX_1:
-            return {HG_P_THERM_L, HG_P_THERM_R, HG_P_THERM_T, HG_P_THERM_B,
-                    HG_P_THERM_LT, HG_P_THERM_RT, HG_P_THERM_LB, HG_P_THERM_RB};
         case HG_PASS_THERMAL_TRANSPORT_0: return {HG_P_ROCK};
         case HG_PASS_THERMAL_TRANSPORT_1: return {HG_P_DIRT};
         case HG_PASS_SMOOTH: return {HG_P_ROCK, HG_P_DIRT};
         default: return {};
-    }
-}
-
-// Whether the adjoint kernel of `pass` accumulates (with atomics) into the
-// fresh buffer of produced plane `p`, which must then start at zero; the other
-// fresh buffers are written with plain stores or never read.
-bool needs_zero(int pass, int p) {
-    switch (pass) {
-        case HG_PASS_FLUX: return p == HG_P_WATER || (p >= HG_P_FLUX_L && p <= HG_P_FLUX_B);
-        case HG_PASS_EROSION: return p == HG_P_ROCK || p == HG_P_DIRT;
-        case HG_PASS_TRANSPORT: return p == HG_P_SED_ROCK || p == HG_P_SED_DIRT;
-        case HG_PASS_SMOOTH: return true;
-        default: return false;
     }
 }
 
@@ -474,7 +577,6 @@ struct AdjWork {
     std::vector<float*> tape;      // k_ckpt x NTAPE planes: each step's intermediates
     std::vector<float*> pool;      // adjoint planes (free list)
     float* zero = nullptr;
-    float* lw = nullptr;
     double* gcoef = nullptr;
 };
 
@@ -565,7 +667,6 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     for (size_t i = 0; i < size_t(k_ckpt) * NTAPE; ++i) w.tape.push_back(plane());
     for (size_t i = 0; i < n_pool; ++i) w.pool.push_back(plane());
     w.zero = plane();
-    w.lw = plane();
     w.gcoef = reinterpret_cast<double*>(base + used * w.pitch);
     HG_HIP(c, hipMemsetAsync(w.zero, 0, pn * sizeof(float), c->stream));
     HG_HIP(c, hipMemsetAsync(w.gcoef, 0, 4 * sizeof(double), c->stream));
@@ -594,16 +695,20 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_ROCK], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_DIRT], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
 
+    // reverse units: a thermal layer (outflow then transport) is one unit,
+    // named by its transport pass
     std::vector<int> order;
     order.push_back(HG_PASS_FLUX);
     order.push_back(HG_PASS_EROSION);
     order.push_back(HG_PASS_TRANSPORT);
-    if (thermal)
-        for (int p = HG_PASS_THERMAL_FLUX_0; p <= HG_PASS_THERMAL_TRANSPORT_1; ++p) order.push_back(p);
+    if (thermal) {
+        order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
+        order.push_back(HG_PASS_THERMAL_TRANSPORT_1);
+    }
     if (smooth) order.push_back(HG_PASS_SMOOTH);
 
     const dim3 blk(BX, BY), grd((n + BX - 1) / BX, (n + BY - 1) / BY);
-    const unsigned g1 = unsigned((pn + 255) / 256);
+    const dim3 gtile((n + GTX - 1) / GTX, (n + GTY - 1) / GTY);
     for (int seg = int(nck) - 1; seg >= 0; --seg) {
         const uint32_t s_begin = uint32_t(seg) * k_ckpt;
         const uint32_t s_end = std::min(nsteps, s_begin + k_ckpt);
@@ -639,10 +744,10 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         X.p[HG_P_SED_ROCK] = tp[T_S0]; X.p[HG_P_SED_DIRT] = tp[T_S1]; X.p[HG_P_WATER] = tp[T_B2];
                         X.p[HG_P_VEL_U] = tp[T_VX]; X.p[HG_P_VEL_V] = tp[T_VY]; X.p[HG_P_VEL_DEPTH] = tp[T_VZ];
                         break;
-                    case HG_PASS_THERMAL_FLUX_0:
+                    case HG_PASS_THERMAL_TRANSPORT_0:
                         X.p[HG_P_ROCK] = tp[T_R3]; X.p[HG_P_DIRT] = tp[T_D3];
                         break;
-                    case HG_PASS_THERMAL_FLUX_1:
+                    case HG_PASS_THERMAL_TRANSPORT_1:
                         X.p[HG_P_ROCK] = tp[T_R5]; X.p[HG_P_DIRT] = tp[T_D3];
                         break;
                     case HG_PASS_SMOOTH:
@@ -652,20 +757,11 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                 }
                 return X;
             };
-            // fresh adjoint buffers of every pass output, taken up front; the
-            // ones that accumulate are zeroed by one launch
+            // fresh adjoint buffers of every unit's outputs (each is written
+            // in full by its kernel: no zeroing)
             std::vector<std::vector<float*>> fresh_of(order.size());
-            std::vector<float*> to_zero;
-            for (size_t oi = 0; oi < order.size(); ++oi) {
-                for (int p : produced(order[oi])) {
-                    float* f = take(w.pool);
-                    fresh_of[oi].push_back(f);
-                    if (needs_zero(order[oi], p)) to_zero.push_back(f);
-                }
-                if (order[oi] == HG_PASS_FLUX) to_zero.push_back(w.lw);
-            }
-            int rc = zero_planes(c, to_zero, pn);
-            if (rc != HG_OK) return rc;
+            for (size_t oi = 0; oi < order.size(); ++oi)
+                for (size_t q = 0; q < produced(order[oi]).size(); ++q) fresh_of[oi].push_back(take(w.pool));
             // pass adjoints in reverse order (rain, water += rain, is the identity)
             for (int oi = int(order.size()) - 1; oi >= 0; --oi) {
                 const int pass = order[oi];
@@ -678,26 +774,20 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                 const Planes li = planes_of(fresh);
                 switch (pass) {
                     case HG_PASS_FLUX:
-                        k_adj_flux<<<grd, blk, 0, c->stream>>>(X, lo, li, w.lw, n, n, c->kp);
-                        HG_HIP(c, hipGetLastError());
-                        k_adj_total<<<g1, 256, 0, c->stream>>>(w.lw, fresh[HG_P_ROCK], fresh[HG_P_DIRT],
-                                                                fresh[HG_P_WATER], pn);
+                        k_adj_flux<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
                     case HG_PASS_EROSION:
-                        k_adj_erosion<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
+                        k_adj_erosion<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
                         break;
                     case HG_PASS_TRANSPORT:
                         k_adj_transport<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
-                        break;
-                    case HG_PASS_THERMAL_FLUX_0:
-                    case HG_PASS_THERMAL_FLUX_1:
-                        k_adj_thermal_flux<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n,
-                                                                        pass == HG_PASS_THERMAL_FLUX_0 ? 0 : 1, c->kp);
+                        HG_HIP(c, hipGetLastError());
+                        k_adj_transport_far<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
                     case HG_PASS_THERMAL_TRANSPORT_0:
                     case HG_PASS_THERMAL_TRANSPORT_1:
-                        k_adj_thermal_transport<<<grd, blk, 0, c->stream>>>(lo, li, n, n,
-                                                                             pass == HG_PASS_THERMAL_TRANSPORT_0 ? 0 : 1);
+                        k_adj_thermal<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n,
+                                                                     pass == HG_PASS_THERMAL_TRANSPORT_0 ? 0 : 1, c->kp);
                         break;
                     case HG_PASS_SMOOTH:
                         k_adj_smooth<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp);

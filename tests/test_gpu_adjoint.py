@@ -88,3 +88,43 @@ def test_adjoint_arguments_rejected():
             t.adjoint_rollout(4, 0, g)
         with pytest.raises(hg.HGError):
             t.adjoint_rollout(4, 2, g, hg.STEP_PER_PASS)
+
+
+def test_adjoint_with_back_traces_beyond_one_cell():
+    """Thin water under large old fluxes: back-traces of several cells, so the
+    transport adjoint's scatter path (taps more than one cell away) runs."""
+    n, steps = 40, 2
+    h0, tgt = setup_case(n, seed=4)
+    rng = np.random.default_rng(8)
+    st = np.zeros((hg.NUM_STATE_PLANES, n, n), np.float32)
+    st[hg.P_ROCK] = h0
+    st[hg.P_WATER] = rng.uniform(0.0, 0.02, (n, n))
+    for p in (hg.P_FLUX_L, hg.P_FLUX_R, hg.P_FLUX_T, hg.P_FLUX_B):
+        st[p] = rng.uniform(0.0, 60.0, (n, n))
+    st[hg.P_SED_ROCK] = rng.uniform(0, 1, (n, n))
+    st[hg.P_SED_DIRT] = rng.uniform(0, 1, (n, n))
+    with hg.Terrain(n) as t:
+        t.set_erosion(hg.erosion_settings(Kc=COEF[0], Ks=COEF[1], Kd=COEF[2], Ke=COEF[3]))
+        t.set_rain(hg.RAIN_CONSTANT, amount=0.01)
+        t.write_state(st)
+        t.run_pass(hg.PASS_FLUX)
+        vdt = np.abs(t.read_plane(hg.P_VEL_U)) * 0.005
+        assert (vdt > 1.0).mean() > 0.2        # many taps are more than a cell away
+        t.write_state(st)
+        t.step = 0
+        t.dispatch_grid(steps)
+        h = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
+        g = (2.0 * (h - tgt) / h.size).astype(np.float32)
+        t.write_state(st)
+        t.step = 0
+        gh, gc = t.adjoint_rollout(steps, 1, g)
+    ost = O.new_state(n)
+    ost[: hg.NUM_STATE_PLANES] = st
+    cells = rng.choice(n * n, size=24, replace=False)
+    _, dc, dh = O.rollout_grad_dual(ost, O.config_settings(), COEF, steps, tgt, cells,
+                                    rain_mode=O.RAIN_CONSTANT, rain_amount=0.01)
+    print("coef gpu", gc, "dual", dc)
+    for j in range(4):
+        assert abs(gc[j] - dc[j]) <= 2e-3 * max(abs(dc[j]), 1e-9), (j, gc[j], dc[j])
+    err = np.abs(gh.flat[cells] - dh)
+    assert err.max() <= 2e-3 * np.abs(gh).max()
