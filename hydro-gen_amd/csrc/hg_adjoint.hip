@@ -69,10 +69,16 @@ __device__ void reduce_coef(double* gcoef, double a, double b, double c, double 
 constexpr int GTX = 64, GTY = 8, GNT = 256;
 constexpr int GRW = GTX + 2, GRH = GTY + 2, GNR = GRW * GRH;
 
+// Phase 1 must not store to global memory: its loads, unrolled over the
+// ring/tile cells of a thread, are then free to be issued together.
 template <class Cell, class Gather>
 __device__ __forceinline__ void tile_gather(int nx, int ny, Cell&& cell, Gather&& gather) {
     const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
-    for (int e = threadIdx.x; e < GNR; e += GNT) {
+    constexpr int ITER = (GNR + GNT - 1) / GNT;
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+        const int e = threadIdx.x + it * GNT;
+        if (GNR % GNT != 0 && e >= GNR) break;
         const int ey = e / GRW, ex = e - ey * GRW;
         const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
         const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
@@ -194,7 +200,8 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
 // Writes li's rock, dirt and sediment (fresh) and adds to li's velocity.
 __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
                                                      double* gcoef) {
-    __shared__ float s_dx[GNR], s_dz[GNR], s_r[GNR], s_d[GNR];
+    __shared__ float s_dx[GNR], s_dz[GNR], s_r[GNR], s_d[GNR], s_s0[GNR], s_s1[GNR], s_vx[GNR], s_vy[GNR],
+        s_vz[GNR];
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
     auto rd = [&](int xx, int yy) -> float {
@@ -206,6 +213,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
     tile_gather(nx, ny,
         [&](int e, int px, int py, bool in, bool tile) {
             float lr_in = 0.0f, ld_in = 0.0f, ldx = 0.0f, ldz = 0.0f;
+            float ls0_in = 0.0f, ls1_in = 0.0f, lvx = 0.0f, lvy = 0.0f, lvz = 0.0f;
             if (in) {
                 const size_t i = at(px, py, nx);
                 const float r_in = R[i], d_in = D[i], s0_in = x.p[HG_P_SED_ROCK][i], s1_in = x.p[HG_P_SED_DIRT][i];
@@ -221,7 +229,8 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
                 const float kc = k.Kconv * k.dt;
                 const float ls0c = ls0 * (1.0f - kc) + ls1 * kc, ls1c = ls1;
                 // rock layer: (r, s0, c0) -> (r', s0c)
-                float ls0_in = ls0c, lc0 = 0.0f, lcap0 = 0.0f, lcap = 0.0f;
+                float lc0 = 0.0f, lcap0 = 0.0f, lcap = 0.0f;
+                ls0_in = ls0c;
                 lr_in = lr;
                 if (tp.case0 == 0 || tp.case0 == 2) {
                     const float a = k.dt * (tp.case0 == 0 ? k.Ks[0] : k.Kd[0]);
@@ -238,7 +247,8 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
                     lcap -= lc0;
                 }
                 // dirt layer: (d, s1, c1) -> (d', s1c); cap = d when exhausted
-                float ls1_in = ls1c, lc1 = 0.0f;
+                float lc1 = 0.0f;
+                ls1_in = ls1c;
                 ld_in = ld;
                 if (tp.case1 == 0 || tp.case1 == 2) {
                     const float a = k.dt * (tp.case1 == 0 ? k.Ks[1] : k.Kd[1]);
@@ -262,7 +272,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
                     ldx = 2.0f * tp.dx * lg2;             // +ldx to p+1, -ldx to p-1
                     ldz = 2.0f * tp.dz * lg2;             // +ldz to p+nx, -ldz to p-nx
                 }
-                float lero0 = lero, lvz = 0.0f;
+                float lero0 = lero;
                 if (tp.slow) {
                     lero0 = lero * (1.0f - tp.sm);
                     if (tp.t_inner) {                     // sm = t^2 (3 - 2t), t = (vz - 1e-3) / (5e-4 - 1e-3)
@@ -270,18 +280,12 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
                         lvz = lsm * 6.0f * tp.tt * (1.0f - tp.tt) * (1.0f / (5e-4f - 1e-3f));
                     }
                 }
-                float lvx = 0.0f, lvy = 0.0f;
                 if (tp.ero0 > 0.0f) {
                     const float inv = rcp(tp.ero0);
                     lvx = lero0 * vx * inv;
                     lvy = lero0 * vy * inv;
                 }
-                if (tile) {                               // own outputs, counted once
-                    li.p[HG_P_SED_ROCK][i] = ls0_in;
-                    li.p[HG_P_SED_DIRT][i] = ls1_in;
-                    li.p[HG_P_VEL_U][i] += lvx;
-                    li.p[HG_P_VEL_V][i] += lvy;
-                    li.p[HG_P_VEL_DEPTH][i] += lvz;
+                if (tile) {                               // coefficient terms counted once
                     gKc += cKc; gKs += cKs; gKd += cKd;
                 }
             }
@@ -289,12 +293,22 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
             s_dz[e] = ldz;
             s_r[e] = lr_in;
             s_d[e] = ld_in;
+            s_s0[e] = ls0_in;
+            s_s1[e] = ls1_in;
+            s_vx[e] = lvx;
+            s_vy[e] = lvy;
+            s_vz[e] = lvz;
         },
         [&](int e, int px, int py) {
             const size_t i = at(px, py, nx);
             const float nb = (s_dx[e - 1] - s_dx[e + 1]) + (s_dz[e - GRW] - s_dz[e + GRW]);
             li.p[HG_P_ROCK][i] = s_r[e] + nb;
             li.p[HG_P_DIRT][i] = s_d[e] + nb;
+            li.p[HG_P_SED_ROCK][i] = s_s0[e];
+            li.p[HG_P_SED_DIRT][i] = s_s1[e];
+            li.p[HG_P_VEL_U][i] += s_vx[e];
+            li.p[HG_P_VEL_V][i] += s_vy[e];
+            li.p[HG_P_VEL_DEPTH][i] += s_vz[e];
         });
     reduce_coef(gcoef, gKc, gKs, gKd, 0.0);
 }
@@ -321,67 +335,75 @@ __device__ __forceinline__ Taps taps_of(int qx, int qy, float vx, float vy, int 
     return t;
 }
 
-__global__ __launch_bounds__(BX * BY) void k_adj_transport(Planes x, Planes lo, Planes li, int nx, int ny,
-                                                           KParams k, double* gcoef) {
-    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
+__global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
+                                                       double* gcoef) {
+    __shared__ float s_vx[GNR], s_vy[GNR], s_l0[GNR], s_l1[GNR];
+    const float* VX = x.p[HG_P_VEL_U];
+    const float* VY = x.p[HG_P_VEL_V];
     double gKe = 0;
-    if (px < nx && py < ny) {
-        const size_t i = at(px, py, nx);
-        const float* VX = x.p[HG_P_VEL_U];
-        const float* VY = x.p[HG_P_VEL_V];
-        // sediment: gather from the near cells whose taps include p
-        float a0 = 0.0f, a1 = 0.0f;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int qx = px + dx, qy = py + dy;
-                if (qx < 0 || qy < 0 || qx >= nx || qy >= ny) continue;
-                const size_t q = at(qx, qy, nx);
-                const Taps t = taps_of(qx, qy, VX[q], VY[q], nx, ny, k.dt);
-                if (!t.near) continue;
-                float w = 0.0f;
-                if (t.iy == py) {
-                    if (t.ix == px) w += t.w00;
-                    if (t.ix1 == px) w += t.w10;
-                }
-                if (t.iy1 == py) {
-                    if (t.ix == px) w += t.w01;
-                    if (t.ix1 == px) w += t.w11;
-                }
-                if (w != 0.0f) {
-                    a0 += lo.p[HG_P_SED_ROCK][q] * w;
-                    a1 += lo.p[HG_P_SED_DIRT][q] * w;
-                }
+    tile_gather(nx, ny,
+        [&](int e, int px, int py, bool in, bool) {
+            float vx = 0.0f, vy = 0.0f, l0 = 0.0f, l1 = 0.0f;
+            if (in) {
+                const size_t i = at(px, py, nx);
+                vx = VX[i]; vy = VY[i];
+                l0 = lo.p[HG_P_SED_ROCK][i]; l1 = lo.p[HG_P_SED_DIRT][i];
             }
-        li.p[HG_P_SED_ROCK][i] = a0;
-        li.p[HG_P_SED_DIRT][i] = a1;
-        // own velocity and water
-        const float vx = VX[i], vy = VY[i];
-        const float bxr = float(px) - vx * k.dt;
-        const float byr = float(py) - vy * k.dt;
-        const Taps t = taps_of(px, py, vx, vy, nx, ny, k.dt);
-        const size_t j00 = at(t.ix, t.iy, nx), j10 = at(t.ix1, t.iy, nx), j01 = at(t.ix, t.iy1, nx),
-                     j11 = at(t.ix1, t.iy1, nx);
-        float bx, by;
-        back_trace(px, py, vx, vy, k.dt, nx, ny, bx, by);
-        const float fx = bx - float(t.ix), fy = by - float(t.iy);
-        const float l0 = lo.p[HG_P_SED_ROCK][i], l1 = lo.p[HG_P_SED_DIRT][i];
-        float lbx = 0.0f, lby = 0.0f;
-        for (int c = 0; c < 2; ++c) {
-            const float* S = x.p[c == 0 ? HG_P_SED_ROCK : HG_P_SED_DIRT];
-            const float s00 = S[j00], s10 = S[j10], s01 = S[j01], s11 = S[j11];
-            const float l = c == 0 ? l0 : l1;
-            lbx += l * ((1.0f - fy) * (s10 - s00) + fy * (s11 - s01));
-            lby += l * (lerp1(s01, s11, fx) - lerp1(s00, s10, fx));
-        }
-        // first writer of the velocity adjoint in a step (velocity is an
-        // intermediate of the step: nothing after this step reads it)
-        li.p[HG_P_VEL_U][i] = (bxr > 0.0f && bxr < float(nx - 1)) ? -k.dt * lbx : 0.0f;
-        li.p[HG_P_VEL_V][i] = (byr > 0.0f && byr < float(ny - 1)) ? -k.dt * lby : 0.0f;
-        li.p[HG_P_VEL_DEPTH][i] = 0.0f;
-        const float lb = lo.p[HG_P_WATER][i];
-        li.p[HG_P_WATER][i] = lb * k.evap;                       // b' = b (1 - Ke dt)
-        gKe = double(lb) * double(x.p[HG_P_WATER][i]) * double(-k.dt);
-    }
+            s_vx[e] = vx; s_vy[e] = vy; s_l0[e] = l0; s_l1[e] = l1;
+        },
+        [&](int e, int px, int py) {
+            const size_t i = at(px, py, nx);
+            // sediment: gather from the near cells whose taps include p
+            float a0 = 0.0f, a1 = 0.0f;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int qx = px + dx, qy = py + dy;
+                    if (qx < 0 || qy < 0 || qx >= nx || qy >= ny) continue;
+                    const int eq = e + dx + dy * GRW;
+                    const Taps t = taps_of(qx, qy, s_vx[eq], s_vy[eq], nx, ny, k.dt);
+                    if (!t.near) continue;
+                    float w = 0.0f;
+                    if (t.iy == py) {
+                        if (t.ix == px) w += t.w00;
+                        if (t.ix1 == px) w += t.w10;
+                    }
+                    if (t.iy1 == py) {
+                        if (t.ix == px) w += t.w01;
+                        if (t.ix1 == px) w += t.w11;
+                    }
+                    a0 += s_l0[eq] * w;
+                    a1 += s_l1[eq] * w;
+                }
+            li.p[HG_P_SED_ROCK][i] = a0;
+            li.p[HG_P_SED_DIRT][i] = a1;
+            // own velocity and water
+            const float vx = s_vx[e], vy = s_vy[e];
+            const float bxr = float(px) - vx * k.dt;
+            const float byr = float(py) - vy * k.dt;
+            const Taps t = taps_of(px, py, vx, vy, nx, ny, k.dt);
+            const size_t j00 = at(t.ix, t.iy, nx), j10 = at(t.ix1, t.iy, nx), j01 = at(t.ix, t.iy1, nx),
+                         j11 = at(t.ix1, t.iy1, nx);
+            float bx, by;
+            back_trace(px, py, vx, vy, k.dt, nx, ny, bx, by);
+            const float fx = bx - float(t.ix), fy = by - float(t.iy);
+            const float l0 = s_l0[e], l1 = s_l1[e];
+            float lbx = 0.0f, lby = 0.0f;
+            for (int c = 0; c < 2; ++c) {
+                const float* S = x.p[c == 0 ? HG_P_SED_ROCK : HG_P_SED_DIRT];
+                const float s00 = S[j00], s10 = S[j10], s01 = S[j01], s11 = S[j11];
+                const float l = c == 0 ? l0 : l1;
+                lbx += l * ((1.0f - fy) * (s10 - s00) + fy * (s11 - s01));
+                lby += l * (lerp1(s01, s11, fx) - lerp1(s00, s10, fx));
+            }
+            // first writer of the velocity adjoint in a step (velocity is an
+            // intermediate of the step: nothing after this step reads it)
+            li.p[HG_P_VEL_U][i] = (bxr > 0.0f && bxr < float(nx - 1)) ? -k.dt * lbx : 0.0f;
+            li.p[HG_P_VEL_V][i] = (byr > 0.0f && byr < float(ny - 1)) ? -k.dt * lby : 0.0f;
+            li.p[HG_P_VEL_DEPTH][i] = 0.0f;
+            const float lb = lo.p[HG_P_WATER][i];
+            li.p[HG_P_WATER][i] = lb * k.evap;                   // b' = b (1 - Ke dt)
+            gKe += double(lb) * double(x.p[HG_P_WATER][i]) * double(-k.dt);
+        });
     reduce_coef(gcoef, 0.0, 0.0, 0.0, gKe);
 }
 
@@ -780,7 +802,7 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         k_adj_erosion<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
                         break;
                     case HG_PASS_TRANSPORT:
-                        k_adj_transport<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
+                        k_adj_transport<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
                         HG_HIP(c, hipGetLastError());
                         k_adj_transport_far<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
