@@ -39,6 +39,15 @@ namespace {
 constexpr int BX = 64, BY = 4;
 
 __device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * size_t(nx) + size_t(x); }
+
+// p[x, y], or `fill` off the grid.  The load itself is unconditional (from the
+// clamped cell), so a kernel's neighbour loads issue together instead of each
+// waiting behind a branch.
+__device__ __forceinline__ float ldc(const float* p, int x, int y, int nx, int ny, float fill) {
+    const int cx = min(max(x, 0), nx - 1), cy = min(max(y, 0), ny - 1);
+    const float v = p[at(cx, cy, nx)];
+    return (cx == x && cy == y) ? v : fill;
+}
 __device__ __forceinline__ void aadd(float* p, float v) {
     if (v != 0.0f) unsafeAtomicAdd(p, v);
 }
@@ -105,9 +114,8 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
     const float* D = x.p[HG_P_DIRT];
     const float* B = x.p[HG_P_WATER];
     auto total = [&](int xx, int yy) -> float {
-        if (xx < 0 || yy < 0 || xx >= nx || yy >= ny) return HG_OOB_HEIGHT;
-        const size_t j = at(xx, yy, nx);
-        return R[j] + D[j] + B[j];
+        const float t = ldc(R, xx, yy, nx, ny, 0.0f) + ldc(D, xx, yy, nx, ny, 0.0f) + ldc(B, xx, yy, nx, ny, 0.0f);
+        return (xx < 0 || yy < 0 || xx >= nx || yy >= ny) ? HG_OOB_HEIGHT : t;
     };
     tile_gather(nx, ny,
         [&](int e, int px, int py, bool in, bool) {
@@ -118,10 +126,10 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
                 float f[4] = {x.p[HG_P_FLUX_L][i], x.p[HG_P_FLUX_R][i], x.p[HG_P_FLUX_T][i], x.p[HG_P_FLUX_B][i]};
                 const float w = R[i] + D[i] + B[i];
                 const float wn[4] = {total(px - 1, py), total(px + 1, py), total(px, py + 1), total(px, py - 1)};
-                const float fin[4] = {px > 0 ? x.p[HG_P_FLUX_R][i - 1] : 0.0f,
-                                      px < nx - 1 ? x.p[HG_P_FLUX_L][i + 1] : 0.0f,
-                                      py < ny - 1 ? x.p[HG_P_FLUX_B][i + nx] : 0.0f,
-                                      py > 0 ? x.p[HG_P_FLUX_T][i - nx] : 0.0f};
+                const float fin[4] = {ldc(x.p[HG_P_FLUX_R], px - 1, py, nx, ny, 0.0f),
+                                      ldc(x.p[HG_P_FLUX_L], px + 1, py, nx, ny, 0.0f),
+                                      ldc(x.p[HG_P_FLUX_B], px, py + 1, nx, ny, 0.0f),
+                                      ldc(x.p[HG_P_FLUX_T], px, py - 1, nx, ny, 0.0f)};
                 const unsigned wall = (px == 0 ? 1u : 0u) | (px == nx - 1 ? 2u : 0u) | (py == 0 ? 4u : 0u) |
                                       (py == ny - 1 ? 8u : 0u);
                 FluxTape tp;
@@ -205,9 +213,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
     auto rd = [&](int xx, int yy) -> float {
-        if (xx < 0 || yy < 0 || xx >= nx || yy >= ny) return 0.0f;
-        const size_t j = at(xx, yy, nx);
-        return R[j] + D[j];
+        return ldc(R, xx, yy, nx, ny, 0.0f) + ldc(D, xx, yy, nx, ny, 0.0f);
     };
     double gKc = 0, gKs = 0, gKd = 0;
     tile_gather(nx, ny,
@@ -450,18 +456,17 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
             if (in) {
                 const size_t i = at(px, py, nx);
                 const float r = R[i], d = Dt[i];
-                float Dk[8];
+                float Dk[8], lhn[8];
                 bool ok[8];
-                size_t jn[8];
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
                     const int xx = px + TDX[kk], yy = py + TDY[kk];
                     ok[kk] = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
-                    jn[kk] = ok[kk] ? at(xx, yy, nx) : 0;
-                    const float rn = ok[kk] ? R[jn[kk]] : HG_OOB_HEIGHT;
+                    const float rn = ldc(R, xx, yy, nx, ny, HG_OOB_HEIGHT);
                     float v = r - rn;
-                    if (layer == 1) v = v + (d - (ok[kk] ? Dt[jn[kk]] : HG_OOB_HEIGHT));
+                    if (layer == 1) v = v + (d - ldc(Dt, xx, yy, nx, ny, HG_OOB_HEIGHT));
                     Dk[kk] = v;
+                    lhn[kk] = ldc(LH, xx, yy, nx, ny, 0.0f);
                 }
                 ThermalTape tp;
                 const float g = thermal_g_t(Dk, layer == 0 ? r : d, layer, k, tp);
@@ -471,7 +476,7 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
 #pragma unroll
                     for (int kk = 0; kk < 8; ++kk) {
                         const bool m = talus_marked(Dk[kk], kk, layer, k);
-                        const float lout = (ok[kk] ? LH[jn[kk]] : 0.0f) - lh;   // dL/dT_k
+                        const float lout = lhn[kk] - lh;          // dL/dT_k
                         lD[kk] = m ? lout * g : 0.0f;
                         if (m) lg += lout * Dk[kk];
                     }
@@ -543,14 +548,19 @@ __global__ __launch_bounds__(BX * BY) void k_adj_smooth(Planes x, Planes lo, Pla
         if (eg) ad = ld * (1.0f - 0.8f * m);
     }
     const int NXo[4] = {-1, 1, 0, 0}, NYo[4] = {0, 0, 1, -1};
+#pragma unroll
     for (int j = 0; j < 4; ++j) {
+        // flags of the neighbour, evaluated at a clamped interior cell so the
+        // loads are unconditional; used only when the neighbour is interior
         const int qx = px + NXo[j], qy = py + NYo[j];
-        if (!interior(qx, qy)) continue;
-        const size_t q = at(qx, qy, nx);
+        const int cx = min(max(qx, 1), nx - 2), cy = min(max(qy, 1), ny - 2);
+        const size_t q = at(cx, cy, nx);
         bool er, eg;
         flags(q, er, eg);
-        if (er) ar += lo.p[HG_P_ROCK][q] * 0.2f * m;
-        if (eg) ad += lo.p[HG_P_DIRT][q] * 0.2f * m;
+        const bool use = interior(qx, qy);
+        const float lrq = lo.p[HG_P_ROCK][q], ldq = lo.p[HG_P_DIRT][q];
+        if (use && er) ar += lrq * 0.2f * m;
+        if (use && eg) ad += ldq * 0.2f * m;
     }
     li.p[HG_P_ROCK][i] = ar;
     li.p[HG_P_DIRT][i] = ad;
