@@ -45,7 +45,7 @@ struct hg_ctx {
     unsigned long long* dev_dsteps = nullptr;
     int hydro_variant = 0;           // fused kernel tile variants (see hg_autotune)
     int thermal_variant = 0;
-    bool xcd_order = false;          // fused kernels: contiguous tile runs per XCD
+    int xcd_order = 0;               // fused kernels' tile order (hg_set_xcd_order)
     std::string arch;
     std::string err;
     // timing

@@ -95,7 +95,10 @@ struct HydroTile {
 
 // One output tile (bx, by) of the hydro kernel; `lds` holds 6*NLDS floats.
 // Ends with LDS still in use: callers looping over tiles must barrier first.
-template <int TX, int TY, int HA, int NT>
+// EXTRA: also write the optional outputs (velocity, adjoint tape) that are
+// not null; without it their pointers are never read, which keeps them out
+// of the scalar registers of the common case.
+template <int TX, int TY, int HA, int NT, bool EXTRA = false>
 __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx, int ny, int bx,
                                            int by, uint32_t step, uint32_t terrain, int rain,
                                            const KParams& k) {
@@ -210,12 +213,12 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
             io.fro[g] = f[1];
             io.fto[g] = f[2];
             io.fbo[g] = f[3];
-            if (io.vx) {
+            if (EXTRA && io.vx) {
                 io.vx[g] = vx;
                 io.vy[g] = vy;
                 io.vz[g] = vz;
             }
-            if (io.tb1) {
+            if (EXTRA && io.tb1) {
                 io.tb1[g] = b;
                 io.tb2[g] = d2;
                 io.ts0[g] = s0;
@@ -281,8 +284,17 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
 // of tiles starting at x*q + min(x, r) (q = nb/8, r = nb mod 8), so a tile and
 // its neighbours, whose halos overlap, are read through the same L2.
 __device__ __forceinline__ void tile_of(int b, int nb, int gx, int xcd, int& bx, int& by) {
+    if (xcd == 2 && (gx & 7) == 0 && (nb & 7) == 0) {
+        // XCD x owns the column strip of tiles [x*sw, x*sw + sw) and walks it
+        // row by row: its L2 sees both the horizontal and the vertical
+        // neighbours of a tile, and all XCDs advance through the same rows
+        const int sw = gx >> 3, x = b & 7, j = b >> 3;
+        by = j / sw;
+        bx = x * sw + (j - by * sw);
+        return;
+    }
     int t = b;
-    if (xcd) {
+    if (xcd == 1) {
         const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
         t = x * q + min(x, r) + j;
     }
@@ -290,13 +302,13 @@ __device__ __forceinline__ void tile_of(int b, int nb, int gx, int xcd, int& bx,
     bx = t - by * gx;
 }
 
-template <int TX, int TY, int HA, int NT>
+template <int TX, int TY, int HA, int NT, bool EXTRA>
 __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, uint32_t step,
                                                     uint32_t terrain, int rain, int xcd, KParams k) {
     __shared__ float lds[6 * HydroTile<TX, TY, HA, NT>::NLDS];
     int bx, by;
     tile_of(blockIdx.x, gridDim.x, (nx + TX - 1) / TX, xcd, bx, by);
-    hydro_tile<TX, TY, HA, NT>(lds, io, nx, ny, bx, by, step, terrain, rain, k);
+    hydro_tile<TX, TY, HA, NT, EXTRA>(lds, io, nx, ny, bx, by, step, terrain, rain, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,7 +349,7 @@ struct ThermalTile {
 };
 
 // One output tile (bx, by) of the thermal kernel; `lds` holds 3*N0 floats.
-template <int TX, int TY, int NT>
+template <int TX, int TY, int NT, bool EXTRA = false>
 __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int nx, int ny, int bx,
                                              int by, int do_thermal, int do_smooth, const KParams& k) {
     constexpr int W0 = ThermalTile<TX, TY>::W0, N0 = ThermalTile<TX, TY>::N0;
@@ -454,7 +466,7 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
         const int li = ly * W0 + lx;
         float r = s_r[li], d = s_d[li];
         const size_t g = at(gx, gy, nx);
-        if (io.tr5) {
+        if (EXTRA && io.tr5) {
             io.tr5[g] = r;
             io.td6[g] = d;
         }
@@ -466,13 +478,13 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
     });
 }
 
-template <int TX, int TY, int NT>
+template <int TX, int TY, int NT, bool EXTRA>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_fused_thermal(
     ThermIO io, int nx, int ny, int do_thermal, int do_smooth, int xcd, KParams k) {
     __shared__ float lds[3 * ThermalTile<TX, TY>::N0];
     int bx, by;
     tile_of(blockIdx.x, gridDim.x, (nx + TX - 1) / TX, xcd, bx, by);
-    thermal_tile<TX, TY, NT>(lds, io, nx, ny, bx, by, do_thermal, do_smooth, k);
+    thermal_tile<TX, TY, NT, EXTRA>(lds, io, nx, ny, bx, by, do_thermal, do_smooth, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -550,7 +562,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     X(2, 128, 16, 2, 512)    \
     X(3,  64,  8, 2, 256)    \
     X(4,  64, 16, 3, 256)    \
-    X(5, 128,  8, 2, 256)
+    X(5, 128,  8, 2, 256)    \
+    X(6,  64, 16, 2, 512)    \
+    X(7, 128, 16, 2, 1024)
 //                        TX   TY   NT
 #define HG_THERMAL_VARIANTS(X) \
     X(0,  64, 16, 256)         \
@@ -596,13 +610,17 @@ bool thermal_variant_info(int v, int* tx, int* ty, int* halo, int* nt) {
 }
 
 hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step,
-                              uint32_t terrain, bool rain, bool xcd, const KParams& k, hipStream_t s) {
+                              uint32_t terrain, bool rain, int xcd, const KParams& k, hipStream_t s) {
     switch (variant) {
 #define X(i, TX_, TY_, HA_, NT_)                                                              \
     case i: {                                                                                 \
         const unsigned grid = unsigned((nx + TX_ - 1) / TX_) * unsigned((ny + TY_ - 1) / TY_); \
-        k_fused_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, rain ? 1 : 0, \
-                                                               xcd ? 1 : 0, k);               \
+        if (io.vx || io.tb1)                                                                  \
+            k_fused_hydro<TX_, TY_, HA_, NT_, true><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, \
+                                                                         rain ? 1 : 0, xcd, k);   \
+        else                                                                                  \
+            k_fused_hydro<TX_, TY_, HA_, NT_, false><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, \
+                                                                          rain ? 1 : 0, xcd, k);  \
         break;                                                                                \
     }
         HG_HYDRO_VARIANTS(X)
@@ -685,13 +703,17 @@ int batch_thermal_blocks_per_cu(int variant) {
 }
 
 hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,
-                                bool smooth, bool xcd, const KParams& k, hipStream_t s) {
+                                bool smooth, int xcd, const KParams& k, hipStream_t s) {
     switch (variant) {
 #define X(i, TX_, TY_, NT_)                                                                  \
     case i: {                                                                                \
         const unsigned grid = unsigned((nx + TX_ - 1) / TX_) * unsigned((ny + TY_ - 1) / TY_); \
-        k_fused_thermal<TX_, TY_, NT_><<<grid, NT_, 0, s>>>(io, nx, ny, thermal ? 1 : 0,     \
-                                                             smooth ? 1 : 0, xcd ? 1 : 0, k); \
+        if (io.tr5)                                                                          \
+            k_fused_thermal<TX_, TY_, NT_, true><<<grid, NT_, 0, s>>>(io, nx, ny, thermal ? 1 : 0, \
+                                                                      smooth ? 1 : 0, xcd, k); \
+        else                                                                                 \
+            k_fused_thermal<TX_, TY_, NT_, false><<<grid, NT_, 0, s>>>(io, nx, ny, thermal ? 1 : 0, \
+                                                                       smooth ? 1 : 0, xcd, k); \
         break;                                                                               \
     }
         HG_THERMAL_VARIANTS(X)

@@ -53,9 +53,9 @@ int thermal_variant_count();
 bool hydro_variant_info(int v, int* tx, int* ty, int* halo, int* nt);
 bool thermal_variant_info(int v, int* tx, int* ty, int* halo, int* nt);
 hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, uint32_t step,
-                              uint32_t terrain, bool rain, bool xcd, const KParams& k, hipStream_t s);
+                              uint32_t terrain, bool rain, int xcd, const KParams& k, hipStream_t s);
 hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,
-                                bool smooth, bool xcd, const KParams& k, hipStream_t s);
+                                bool smooth, int xcd, const KParams& k, hipStream_t s);
 
 // batched persistent kernels (hg_fused.hip)
 struct BatchTerrain {

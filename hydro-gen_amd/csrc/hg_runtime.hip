@@ -631,9 +631,10 @@ int hg_get_tiles(const hg_ctx* c, int* hydro_variant, int* thermal_variant) {
     return HG_OK;
 }
 
-int hg_set_xcd_order(hg_ctx* c, int on) {
+int hg_set_xcd_order(hg_ctx* c, int mode) {
     if (!c) return HG_ERR_INVALID;
-    c->xcd_order = on != 0;
+    if (mode < 0 || mode > 2) return fail(c, HG_ERR_INVALID, "hg_set_xcd_order: mode is 0, 1 or 2");
+    c->xcd_order = mode;
     return HG_OK;
 }
 

@@ -386,9 +386,10 @@ class Terrain:
         self._check(self.lib.hg_get_tiles(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    def set_xcd_order(self, on: bool):
-        """Contiguous tile runs per XCD for the fused kernels (default off)."""
-        self._check(self.lib.hg_set_xcd_order(self.h, 1 if on else 0))
+    def set_xcd_order(self, mode: int):
+        """Workgroup-to-tile order of the fused kernels (hg_set_xcd_order):
+        0 row-major, 1 contiguous runs per XCD, 2 column strips per XCD."""
+        self._check(self.lib.hg_set_xcd_order(self.h, int(mode)))
 
     def autotune(self, steps: int = 10, flags: int = STEP_DEFAULT):
         a, b = ctypes.c_int(), ctypes.c_int()

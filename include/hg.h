@@ -288,11 +288,13 @@ int hg_tile_variant_info(int kernel, int variant, int* tile_x, int* tile_y, int*
 /* -1 keeps the current choice. */
 int hg_set_tiles(hg_ctx* ctx, int hydro_variant, int thermal_variant);
 int hg_get_tiles(const hg_ctx* ctx, int* hydro_variant, int* thermal_variant);
-/* Tile order of the fused kernels' workgroups.  Workgroups are dealt to the
- * 8 XCDs round robin; on, each XCD gets a contiguous run of tiles, so the
- * halos that neighbouring tiles share are read through one L2.  Off (the
- * default, 2-5% faster on a 4096^2 grid): row-major tile order. */
-int hg_set_xcd_order(hg_ctx* ctx, int on);
+/* Tile order of the fused kernels' workgroups, which the hardware deals to
+ * the 8 XCDs round robin.  0: row-major tile order.  1: each XCD gets a
+ * contiguous run of tiles.  2: each XCD gets a column strip of tiles (when
+ * the tile columns divide by 8) and walks it row by row, so the halos a tile
+ * shares with its horizontal and vertical neighbours are read through one
+ * L2.  Returns HG_ERR_INVALID for other modes. */
+int hg_set_xcd_order(hg_ctx* ctx, int mode);
 /* Times `steps` fused steps per variant (state and step counter are restored
  * afterwards) and selects the fastest of each kernel. */
 int hg_autotune(hg_ctx* ctx, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best);
