@@ -261,21 +261,35 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
         // region-local coordinates of the taps
         const int ex0 = ix - (x0 - HA), ey0 = iy - (y0 - HA);
         const int ex1 = ix1 - (x0 - HA), ey1 = iy1 - (y0 - HA);
-        float a0[4], a1[4];
+        float o0, o1;
         if (ex0 >= 0 && ey0 >= 0 && ex1 < T::WE && ey1 < T::HE) {
             const int e00 = ey0 * T::WE + ex0, e10 = ey0 * T::WE + ex1;
             const int e01 = ey1 * T::WE + ex0, e11 = ey1 * T::WE + ex1;
-            a0[0] = s_s0[e00]; a0[1] = s_s0[e10]; a0[2] = s_s0[e01]; a0[3] = s_s0[e11];
-            a1[0] = s_s1[e00]; a1[1] = s_s1[e10]; a1[2] = s_s1[e01]; a1[3] = s_s1[e11];
+            o0 = lerp1(lerp1(s_s0[e00], s_s0[e10], fx), lerp1(s_s0[e01], s_s0[e11], fx), fy);
+            o1 = lerp1(lerp1(s_s1[e00], s_s1[e10], fx), lerp1(s_s1[e01], s_s1[e11], fx), fy);
         } else {
-            eroded_sediment_global(io, ix, iy, nx, ny, step, terrain, rain != 0, k, a0[0], a1[0]);
-            eroded_sediment_global(io, ix1, iy, nx, ny, step, terrain, rain != 0, k, a0[1], a1[1]);
-            eroded_sediment_global(io, ix, iy1, nx, ny, step, terrain, rain != 0, k, a0[2], a1[2]);
-            eroded_sediment_global(io, ix1, iy1, nx, ny, step, terrain, rain != 0, k, a0[3], a1[3]);
+            // taps outside the halo (rare): one copy of the recompute, looped
+            // over the taps (00, 10, 01, 11) with the same lerp order
+            float p0 = 0.0f, p1 = 0.0f, t0 = 0.0f, t1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
+#pragma unroll 1
+            for (int tap = 0; tap < 4; ++tap) {
+                const int qx = (tap & 1) ? ix1 : ix, qy = (tap & 2) ? iy1 : iy;
+                float v0, v1;
+                eroded_sediment_global(io, qx, qy, nx, ny, step, terrain, rain != 0, k, v0, v1);
+                if ((tap & 1) == 0) {
+                    p0 = v0; p1 = v1;
+                } else if (tap == 1) {
+                    t0 = lerp1(p0, v0, fx); t1 = lerp1(p1, v1, fx);
+                } else {
+                    b0 = lerp1(p0, v0, fx); b1 = lerp1(p1, v1, fx);
+                }
+            }
+            o0 = lerp1(t0, b0, fy);
+            o1 = lerp1(t1, b1, fy);
         }
         const size_t g = at(gx, gy, nx);
-        io.s0o[g] = lerp1(lerp1(a0[0], a0[1], fx), lerp1(a0[2], a0[3], fx), fy);
-        io.s1o[g] = lerp1(lerp1(a1[0], a1[1], fx), lerp1(a1[2], a1[3], fx), fy);
+        io.s0o[g] = o0;
+        io.s1o[g] = o1;
     }
 }
 
