@@ -674,6 +674,8 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
         return fail(c, HG_ERR_INVALID, "hg_adjoint_rollout: need nsteps > 0 and 0 < k_ckpt <= 1024");
     if (flags & ~(HG_STEP_RAIN | HG_STEP_THERMAL | HG_STEP_SMOOTH))
         return fail(c, HG_ERR_INVALID, "hg_adjoint_rollout: only rain/thermal/smooth flags");
+    if ((flags & HG_STEP_RAIN) && c->rain_mode == HG_RAIN_REFERENCE)
+        return fail(c, HG_ERR_UNSUPPORTED, "hg_adjoint_rollout: reference rain is not differentiated");
     DeviceGuard guard(c->device);
     const bool rain = (flags & HG_STEP_RAIN) && c->rain_mode != HG_RAIN_OFF;
     const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;

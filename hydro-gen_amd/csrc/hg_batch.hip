@@ -226,7 +226,7 @@ int per_terrain_steps(hg_batch* b, uint32_t nsteps, uint32_t flags, bool per_pas
                     *out[p] = ptr(p, c ^ 1);
                 }
                 io.vx = io.vy = io.vz = nullptr;
-                BH(b, hgk::launch_fused_hydro(b->hydro_variant, io, n, n, step, uint32_t(t), rain, false, k,
+                BH(b, hgk::launch_fused_hydro(b->hydro_variant, io, n, n, step, uint32_t(t), rain, 2, k,
                                               b->stream));
                 cur ^= (1u << HG_NUM_STATE_PLANES) - 1u;
                 if (thermal || smooth) {
@@ -234,7 +234,7 @@ int per_terrain_steps(hg_batch* b, uint32_t nsteps, uint32_t flags, bool per_pas
                     const int cr = cur & 1, cd = (cur >> 1) & 1;
                     tio.r = ptr(HG_P_ROCK, cr); tio.ro = ptr(HG_P_ROCK, cr ^ 1);
                     tio.d = ptr(HG_P_DIRT, cd); tio.dout = ptr(HG_P_DIRT, cd ^ 1);
-                    BH(b, hgk::launch_fused_thermal(b->thermal_variant, tio, n, n, thermal, smooth, false, k,
+                    BH(b, hgk::launch_fused_thermal(b->thermal_variant, tio, n, n, thermal, smooth, 2, k,
                                                     b->stream));
                     cur ^= (1u << HG_P_ROCK) | (1u << HG_P_DIRT);
                 }

@@ -41,7 +41,71 @@ struct KParams {
     float rain_amount;
     int rain_mode;            // HG_RAIN_*
     uint32_t seed_lo, seed_hi;
+    // reference rain (HG_RAIN_REFERENCE): Rain_data fields, the explicit time
+    // seed and the map's max height
+    float rr_thresh, rr_multip, rr_drops, rr_time, rr_max_h;
 };
+
+// ---------------------------------------------------------------------------
+// Reference rain (glsl/rain.glsl:32-56): simplex noise gln_simplex
+// (glsl/simplex_noise.glsl:377-402), its fBm gln_sfbm (:419-456) and the rain
+// increment with the mountain bonus.  FP32 with the GLSL definitions
+// mod(x, y) = x - y floor(x / y) and fract(x) = x - floor(x); pow(result, 1)
+// of gln_sfbm is taken as result.  The order of every operation follows the
+// shader, so host (FP32) and device agree bit for bit.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ float glsl_mod(float x, float y) { return x - y * floorf(x / y); }
+__host__ __device__ __forceinline__ float glsl_fract(float x) { return x - floorf(x); }
+
+__device__ __forceinline__ float ref_simplex(float vx, float vy) {
+    const float Cx = 0.211324865405187f, Cy = 0.366025403784439f, Cz = -0.577350269189626f,
+                Cw = 0.024390243902439f;
+    const float d = vx * Cy + vy * Cy;
+    float ix = floorf(vx + d), iy = floorf(vy + d);
+    const float di = ix * Cx + iy * Cx;
+    const float x0x = (vx - ix) + di, x0y = (vy - iy) + di;
+    const float i1x = x0x > x0y ? 1.0f : 0.0f, i1y = 1.0f - i1x;
+    const float x1x = (x0x + Cx) - i1x, x1y = (x0y + Cx) - i1y;
+    const float x2x = x0x + Cz, x2y = x0y + Cz;
+    ix = glsl_mod(ix, 289.0f);
+    iy = glsl_mod(iy, 289.0f);
+    auto rnd = [](float q) { return glsl_mod((q * 34.0f + 1.0f) * q, 289.0f); };
+    const float p[3] = {rnd(rnd(iy) + ix), rnd((rnd(iy + i1y) + ix) + i1x), rnd((rnd(iy + 1.0f) + ix) + 1.0f)};
+    const float gx[3] = {x0x, x1x, x2x}, gy[3] = {x0y, x1y, x2y};
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float q = 0.5f - (gx[c] * gx[c] + gy[c] * gy[c]);
+        float m = q < 0.0f ? 0.0f : q;
+        m = m * m;
+        m = m * m;
+        const float x = 2.0f * glsl_fract(p[c] * Cw) - 1.0f;
+        const float h = fabsf(x) - 0.5f;
+        const float a0 = x - floorf(x + 0.5f);
+        m = m * (1.79284291400159f - 0.85373472095314f * (a0 * a0 + h * h));
+        acc = acc + m * (a0 * gx[c] + h * gy[c]);
+    }
+    return 130.0f * acc;
+}
+
+// Rain increment of cell (x, y) with total height w (rock + dirt + water).
+__device__ __forceinline__ float ref_rain_incr(int x, int y, float w, const KParams& k) {
+    const float seed = glsl_fract(k.rr_time * 1.372914227e3f) * 1000.0f;
+    float vx = float(x) + seed * 100.0f, vy = float(y) + seed * 100.0f;
+    float result = 0.0f, amplitude = 1.0f, frequency = 1.0f, maximum = amplitude;
+    for (int o = 0; o < 8; ++o) {
+        result = result + ref_simplex((vx * frequency) * k.rr_drops, (vy * frequency) * k.rr_drops) * amplitude;
+        frequency = frequency * 2.0f;
+        amplitude = amplitude * 0.5f;
+        maximum = maximum + amplitude;
+    }
+    const float sf = result / maximum;
+    const float r = sf < 0.0f ? 0.0f : sf;
+    float incr = k.rain_amount * r;
+    const float mountain = w - k.rr_max_h * k.rr_thresh;
+    if (mountain > 0.0f) incr = incr + ((mountain * k.rr_multip) * r) / ((1.0f - k.rr_thresh) * k.rr_max_h);
+    return incr;
+}
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).

@@ -45,7 +45,9 @@ struct hg_ctx {
     unsigned long long* dev_dsteps = nullptr;
     int hydro_variant = 0;           // fused kernel tile variants (see hg_autotune)
     int thermal_variant = 0;
-    int xcd_order = 0;               // fused kernels' tile order (hg_set_xcd_order)
+    int xcd_order = 2;               // fused kernels' tile order (hg_set_xcd_order)
+    float rain_time = 0.0f;          // HG_RAIN_REFERENCE time seed
+    float map_max_height = 256.0f;   // HG_RAIN_REFERENCE mountain scale (src/settings.hpp:30)
     std::string arch;
     std::string err;
     // timing

@@ -23,6 +23,16 @@ __global__ __launch_bounds__(BX * BY) void k_rain(Planes in, Planes out, int nx,
     out.p[HG_P_WATER][i] = in.p[HG_P_WATER][i] + rain_incr(x, y, step, terrain, k);
 }
 
+// glsl/rain.glsl:32-56 (HG_RAIN_REFERENCE)
+__global__ __launch_bounds__(BX * BY) void k_rain_reference(Planes in, Planes out, int nx, int ny, KParams k) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+    if (x >= nx || y >= ny) return;
+    const size_t i = at(x, y, nx);
+    const float b = in.p[HG_P_WATER][i];
+    const float w = in.p[HG_P_ROCK][i] + in.p[HG_P_DIRT][i] + b;
+    out.p[HG_P_WATER][i] = b + ref_rain_incr(x, y, w, k);
+}
+
 // glsl/hydro_flux.glsl:77-166
 __global__ __launch_bounds__(BX * BY) void k_flux(Planes in, Planes out, int nx, int ny, KParams k) {
     const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
@@ -168,7 +178,10 @@ hipError_t launch_pass(int pass, const Planes& in, const Planes& out, int nx, in
                        uint32_t terrain, const KParams& k, hipStream_t s) {
     const dim3 block(BX, BY), grid((nx + BX - 1) / BX, (ny + BY - 1) / BY);
     switch (pass) {
-        case HG_PASS_RAIN: k_rain<<<grid, block, 0, s>>>(in, out, nx, ny, step, terrain, k); break;
+        case HG_PASS_RAIN:
+            if (k.rain_mode == HG_RAIN_REFERENCE) k_rain_reference<<<grid, block, 0, s>>>(in, out, nx, ny, k);
+            else k_rain<<<grid, block, 0, s>>>(in, out, nx, ny, step, terrain, k);
+            break;
         case HG_PASS_FLUX: k_flux<<<grid, block, 0, s>>>(in, out, nx, ny, k); break;
         case HG_PASS_EROSION: k_erosion<<<grid, block, 0, s>>>(in, out, nx, ny, k); break;
         case HG_PASS_TRANSPORT: k_transport<<<grid, block, 0, s>>>(in, out, nx, ny, k); break;

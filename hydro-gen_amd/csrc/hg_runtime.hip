@@ -98,6 +98,11 @@ void update_kparams(hg_ctx* c) {
     k.evap = 1.0f - s.Ke * s.d_t;                             // sediment_transport.glsl:75
     k.rain_amount = c->rs.amount;
     k.rain_mode = c->rain_mode;
+    k.rr_thresh = c->rs.mountain_thresh;
+    k.rr_multip = c->rs.mountain_multip;
+    k.rr_drops = c->rs.drops;
+    k.rr_time = c->rain_time;
+    k.rr_max_h = c->map_max_height;
     k.seed_lo = uint32_t(c->seed);
     k.seed_hi = uint32_t(c->seed >> 32);
 }
@@ -339,12 +344,26 @@ int hg_get_erosion(const hg_ctx* c, hg_erosion_settings* s) {
 int hg_set_rain(hg_ctx* c, const hg_rain_settings* s, int mode, uint64_t seed) {
     if (!c) return HG_ERR_INVALID;
     if (!s) return fail(c, HG_ERR_INVALID, "hg_set_rain: settings is NULL");
-    if (mode < HG_RAIN_OFF || mode > HG_RAIN_UNIFORM) return fail(c, HG_ERR_INVALID, "hg_set_rain: bad mode");
+    if (mode < HG_RAIN_OFF || mode > HG_RAIN_REFERENCE) return fail(c, HG_ERR_INVALID, "hg_set_rain: bad mode");
     if (!std::isfinite(s->amount) || s->amount < 0.0f)
         return fail(c, HG_ERR_INVALID, "hg_set_rain: amount must be finite and >= 0");
+    if (mode == HG_RAIN_REFERENCE &&
+        (!std::isfinite(s->drops) || !std::isfinite(s->mountain_multip) || !(s->mountain_thresh >= 0.0f) ||
+         !(s->mountain_thresh < 1.0f)))
+        return fail(c, HG_ERR_INVALID, "hg_set_rain: reference rain needs finite drops/multip, 0 <= thresh < 1");
     c->rs = *s;
     c->rain_mode = mode;
     c->seed = seed;
+    update_kparams(c);
+    return HG_OK;
+}
+
+int hg_set_rain_time(hg_ctx* c, float time, float max_height) {
+    if (!c) return HG_ERR_INVALID;
+    if (!std::isfinite(time) || !std::isfinite(max_height) || !(max_height > 0.0f))
+        return fail(c, HG_ERR_INVALID, "hg_set_rain_time: need a finite time and max_height > 0");
+    c->rain_time = time;
+    c->map_max_height = max_height;
     update_kparams(c);
     return HG_OK;
 }
@@ -394,11 +413,19 @@ int hg_step_grid(hg_ctx* c, uint32_t nsteps, uint32_t flags) {
                            HG_STEP_KEEP_VELOCITY;
     if (flags & ~known) return fail(c, HG_ERR_INVALID, "hg_step_grid: unknown flag bits");
     DeviceGuard guard(c->device);
-    const bool rain = (flags & HG_STEP_RAIN) && c->rain_mode != HG_RAIN_OFF;
+    // config rain: every step, inside the step; reference rain: its own pass
+    // before step s when (s + 1) % period == 0 (src/main.cpp:313-319)
+    const bool rain = (flags & HG_STEP_RAIN) && (c->rain_mode == HG_RAIN_CONSTANT ||
+                                                  c->rain_mode == HG_RAIN_UNIFORM);
+    const bool ref_rain = (flags & HG_STEP_RAIN) && c->rain_mode == HG_RAIN_REFERENCE && c->rs.period > 0;
     const bool thermal = (flags & HG_STEP_THERMAL) != 0;
     const bool smooth = (flags & HG_STEP_SMOOTH) != 0;
     for (uint32_t s = 0; s < nsteps; ++s) {
         int rc = HG_OK;
+        if (ref_rain && (c->step + 1) % uint64_t(c->rs.period) == 0) {
+            rc = run_pass(c, HG_PASS_RAIN);
+            if (rc != HG_OK) return rc;
+        }
         if (flags & HG_STEP_PER_PASS) {
             if (rain) rc = run_pass(c, HG_PASS_RAIN);
             for (int p = HG_PASS_FLUX; rc == HG_OK && p <= HG_PASS_TRANSPORT; ++p) rc = run_pass(c, p);
@@ -651,7 +678,8 @@ int hg_autotune(hg_ctx* c, uint32_t steps, uint32_t flags, int* hydro_best, int*
         HG_HIP(c, hipMemcpyAsync(bk + p * pn, c->cur[p], pn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     const uint64_t step0 = c->step;
     const bool saved_timing = c->timing;
-    const bool rain = (flags & HG_STEP_RAIN) && c->rain_mode != HG_RAIN_OFF;
+    const bool rain = (flags & HG_STEP_RAIN) && (c->rain_mode == HG_RAIN_CONSTANT ||
+                                                  c->rain_mode == HG_RAIN_UNIFORM);
     const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
     int rc = resolve_timing(c);
     std::vector<KernelStat> saved_stats = c->stats;

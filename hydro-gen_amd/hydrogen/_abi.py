@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "libhydrogen.so")
 
 # ---- enums mirrored from include/hg.h -------------------------------------
 HG_OK, HG_ERR_INVALID, HG_ERR_HIP, HG_ERR_OOM, HG_ERR_STATE, HG_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
-RAIN_OFF, RAIN_CONSTANT, RAIN_UNIFORM = 0, 1, 2
+RAIN_OFF, RAIN_CONSTANT, RAIN_UNIFORM, RAIN_REFERENCE = 0, 1, 2, 3
 CH_HEIGHT, CH_FLUX, CH_VELOCITY, CH_SEDIMENT = 0, 1, 2, 3
 (P_ROCK, P_DIRT, P_WATER, P_FLUX_L, P_FLUX_R, P_FLUX_T, P_FLUX_B, P_SED_ROCK, P_SED_DIRT,
  P_VEL_U, P_VEL_V, P_VEL_DEPTH, P_THERM_L, P_THERM_R, P_THERM_T, P_THERM_B, P_THERM_LT,
@@ -33,7 +33,7 @@ NUM_PASSES = 9
 # Every exported symbol of include/hg.h (checked by the ABI test).
 EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_size", "hg_set_erosion", "hg_get_erosion",
-    "hg_set_rain", "hg_init_fbm", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
+    "hg_set_rain", "hg_set_rain_time", "hg_init_fbm", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
     "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
@@ -94,13 +94,15 @@ def erosion_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0, slippag
     return s
 
 
-def rain_settings(amount=0.01) -> RainSettings:
+def rain_settings(amount=0.01, mountain_thresh=0.55, mountain_multip=0.05, period=512,
+                  drops=0.02) -> RainSettings:
+    """Rain_data; the defaults are the reference's (src/settings.hpp:15-21)."""
     r = RainSettings()
     r.amount = amount
-    r.mountain_thresh = 0.55
-    r.mountain_multip = 0.05
-    r.period = 512
-    r.drops = 0.02
+    r.mountain_thresh = mountain_thresh
+    r.mountain_multip = mountain_multip
+    r.period = period
+    r.drops = drops
     return r
 
 
@@ -162,6 +164,7 @@ def load() -> ctypes.CDLL:
     L.hg_set_erosion.argtypes = [vp, ctypes.POINTER(ErosionSettings)]
     L.hg_get_erosion.argtypes = [vp, ctypes.POINTER(ErosionSettings)]
     L.hg_set_rain.argtypes = [vp, ctypes.POINTER(RainSettings), ctypes.c_int, ctypes.c_uint64]
+    L.hg_set_rain_time.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     L.hg_init_fbm.argtypes = [vp, ctypes.POINTER(FbmParams)]
     L.hg_rain.argtypes = [vp]
     L.hg_step_grid.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
@@ -269,6 +272,10 @@ class Terrain:
     def set_rain(self, mode: int, amount: float = 0.01, seed: int = 0, settings: RainSettings | None = None):
         r = settings if settings is not None else rain_settings(amount)
         self._check(self.lib.hg_set_rain(self.h, ctypes.byref(r), mode, seed))
+
+    def set_rain_time(self, time: float, max_height: float = 256.0):
+        """Time seed and map max height of RAIN_REFERENCE (hg_set_rain_time)."""
+        self._check(self.lib.hg_set_rain_time(self.h, time, max_height))
 
     def init_fbm(self, p: FbmParams):
         self._check(self.lib.hg_init_fbm(self.h, ctypes.byref(p)))

@@ -73,13 +73,18 @@ typedef struct hg_rain_settings {
     uint32_t _pad[3];        /* 20 */
 } hg_rain_settings;
 
-/* Rain modes.  The reference's own rain (glsl/rain.glsl:32-56) is seeded from
- * the wall clock; these modes are reproducible replacements.
- *  CONSTANT: water += amount on every cell.
- *  UNIFORM:  water += amount * u, u = (Philox4x32-10(ctr, key) >> 8) * 2^-24 with
- *            ctr = {x >> 2, y, step, terrain}, key = {seed lo, seed hi}, output
- *            word (x & 3).  Resampled every step; bit-identical host/device. */
-enum { HG_RAIN_OFF = 0, HG_RAIN_CONSTANT = 1, HG_RAIN_UNIFORM = 2 };
+/* Rain modes.
+ *  CONSTANT:  water += amount on every cell, every step.
+ *  UNIFORM:   water += amount * u, u = (Philox4x32-10(ctr, key) >> 8) * 2^-24 with
+ *             ctr = {x >> 2, y, step, terrain}, key = {seed lo, seed hi}, output
+ *             word (x & 3).  Resampled every step; bit-identical host/device.
+ *  REFERENCE: the reference's own rain (glsl/rain.glsl:32-56): amount times
+ *             max(0, simplex fBm of the cell) plus the mountain bonus, from all
+ *             Rain_data fields; its wall-clock seed is replaced by the explicit
+ *             time of hg_set_rain_time.  hg_step_grid with HG_STEP_RAIN applies
+ *             it before step s when (s + 1) % period == 0, the reference's
+ *             cadence (src/main.cpp:313-319); hg_rain applies it once. */
+enum { HG_RAIN_OFF = 0, HG_RAIN_CONSTANT = 1, HG_RAIN_UNIFORM = 2, HG_RAIN_REFERENCE = 3 };
 
 /* Simplex fBm heightmap initialisation (this project's seeded-permutation
  * simplex; see hg_init_fbm).  Heights are min-max rescaled to
@@ -169,6 +174,12 @@ int hg_get_erosion(const hg_ctx* ctx, hg_erosion_settings* s);
 
 /* Replaces Rain_settings::push_data; `mode` is one of HG_RAIN_*. */
 int hg_set_rain(hg_ctx* ctx, const hg_rain_settings* s, int mode, uint64_t seed);
+
+/* Time seed of HG_RAIN_REFERENCE (the `time` uniform the reference sets from
+ * glfwGetTime, src/erosion.cpp:76-89) and the map's max_height
+ * (Map_settings_data, glsl/bindings.glsl:70-71) that scales the mountain
+ * bonus.  max_height must be > 0. */
+int hg_set_rain_time(hg_ctx* ctx, float time, float max_height);
 
 /* Config heightmap init (in place of World::gen_heightmap,
  * src/state.cpp:116-147).  Resets the step counter to 0. */
@@ -293,7 +304,7 @@ int hg_get_tiles(const hg_ctx* ctx, int* hydro_variant, int* thermal_variant);
  * contiguous run of tiles.  2: each XCD gets a column strip of tiles (when
  * the tile columns divide by 8) and walks it row by row, so the halos a tile
  * shares with its horizontal and vertical neighbours are read through one
- * L2.  Returns HG_ERR_INVALID for other modes. */
+ * L2 (the default).  Returns HG_ERR_INVALID for other modes. */
 int hg_set_xcd_order(hg_ctx* ctx, int mode);
 /* Times `steps` fused steps per variant (state and step counter are restored
  * afterwards) and selects the fastest of each kernel. */

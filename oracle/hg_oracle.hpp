@@ -230,6 +230,89 @@ void pass_rain(State<T>& s, int mode, T amount, uint64_t seed, uint64_t step, ui
 }
 
 // ---------------------------------------------------------------------------
+// Reference rain (glsl/rain.glsl:32-56), the mode the reference runs every
+// `period` steps, with its wall-clock seed replaced by an explicit time:
+//   r = max(0, sfbm(x, y)), sfbm = gln_sfbm (glsl/simplex_noise.glsl:419-456)
+//   with seed fract(time * 1372.914227) * 1000, persistence 0.5, lacunarity
+//   2, scale = drops, 8 octaves, over gln_simplex (:377-402);
+//   incr = amount * r, plus mountain * multip * r / ((1 - thresh) * max_h)
+//   where mountain = w - max_h * thresh > 0 (w = rock + dirt + water).
+// GLSL semantics: mod(x, y) = x - y * floor(x / y), fract(x) = x - floor(x);
+// the literals are FP32 constants.  pow(result, 1.0) in gln_sfbm is taken as
+// result (GLSL leaves pow of a negative base undefined; max(0, .) follows).
+// ---------------------------------------------------------------------------
+template <class T> inline T glsl_mod(T x, T y) { return x - y * floor(x / y); }
+template <class T> inline T glsl_fract(T x) { return x - floor(x); }
+
+template <class T> T ref_simplex(T vx, T vy) {
+    const T Cx = T(0.211324865405187f), Cy = T(0.366025403784439f), Cz = T(-0.577350269189626f),
+            Cw = T(0.024390243902439f);
+    const T d = vx * Cy + vy * Cy;                                    // dot(v, C.yy)
+    T ix = floor(vx + d), iy = floor(vy + d);
+    const T di = ix * Cx + iy * Cx;                                   // dot(i, C.xx)
+    const T x0x = (vx - ix) + di, x0y = (vy - iy) + di;
+    const T i1x = val(x0x) > val(x0y) ? T(1) : T(0), i1y = T(1) - i1x;
+    const T x1x = (x0x + Cx) - i1x, x1y = (x0y + Cx) - i1y;           // x12.xy
+    const T x2x = x0x + Cz, x2y = x0y + Cz;                           // x12.zw
+    ix = glsl_mod(ix, T(289));
+    iy = glsl_mod(iy, T(289));
+    auto rnd = [](T q) { return glsl_mod((q * T(34) + T(1)) * q, T(289)); };   // gln_rand3
+    const T p[3] = {rnd((rnd(iy) + ix)), rnd((rnd(iy + i1y) + ix) + i1x), rnd((rnd(iy + T(1)) + ix) + T(1))};
+    const T gx[3] = {x0x, x1x, x2x}, gy[3] = {x0y, x1y, x2y};
+    T acc = T(0);
+    for (int c = 0; c < 3; ++c) {
+        T m = tmax(T(0.5f) - (gx[c] * gx[c] + gy[c] * gy[c]), T(0));
+        m = m * m;
+        m = m * m;
+        const T x = T(2) * glsl_fract(p[c] * Cw) - T(1);
+        const T h = fabs(x) - T(0.5f);
+        const T a0 = x - floor(x + T(0.5f));
+        m = m * (T(1.79284291400159f) - T(0.85373472095314f) * (a0 * a0 + h * h));
+        acc = acc + m * (a0 * gx[c] + h * gy[c]);                    // dot(m, g), x then y then z
+    }
+    return T(130) * acc;
+}
+
+template <class T> T ref_sfbm(T x, T y, T seed, T persistence, T lacunarity, T scale, int octaves) {
+    x = x + seed * T(100);
+    y = y + seed * T(100);
+    T result = T(0), amplitude = T(1), frequency = T(1), maximum = amplitude;
+    for (int o = 0; o < octaves; ++o) {
+        const T px = (x * frequency) * scale, py = (y * frequency) * scale;
+        result = result + ref_simplex(px, py) * amplitude;
+        frequency = frequency * lacunarity;
+        amplitude = amplitude * persistence;
+        maximum = maximum + amplitude;
+    }
+    return result / maximum;
+}
+
+struct RefRain {
+    float amount = 0.0f, mountain_thresh = 0.0f, mountain_multip = 0.0f, drops = 0.0f;
+    float time = 0.0f, max_height = 0.0f;
+};
+
+template <class T> T ref_rain_incr(int x, int y, T w, const RefRain& rr) {
+    const T seed = glsl_fract(T(rr.time) * T(1.372914227e3f)) * T(1000);
+    const T r = tmax(T(0), ref_sfbm(T(x), T(y), seed, T(0.5f), T(2), T(rr.drops), 8));
+    T incr = T(rr.amount) * r;
+    const T mountain = w - T(rr.max_height) * T(rr.mountain_thresh);
+    if (val(mountain) > 0)
+        incr = incr + ((mountain * T(rr.mountain_multip)) * r) / ((T(1) - T(rr.mountain_thresh)) * T(rr.max_height));
+    return incr;
+}
+
+template <class T> void pass_rain_reference(State<T>& s, const RefRain& rr) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < s.ny; ++y)
+        for (int x = 0; x < s.nx; ++x) {
+            const size_t i = s.at(x, y);
+            const T w = s.r[i] + s.d[i] + s.b[i];
+            s.b[i] = s.b[i] + ref_rain_incr(x, y, w, rr);
+        }
+}
+
+// ---------------------------------------------------------------------------
 // hydro_flux: glsl/hydro_flux.glsl:77-166 (helpers :33-47).
 // ---------------------------------------------------------------------------
 template <class T> void pass_flux(State<T>& s, const Settings<T>& k) {

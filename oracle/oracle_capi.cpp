@@ -190,6 +190,36 @@ double orc_droplets_f32(float* rock, float* dirt, int n, const hg_droplet_settin
     return ex;
 }
 
+// Reference rain (one pass) on the state planes; rs is the Rain_data block.
+int orc_rain_reference_f64(double* planes, int nx, int ny, const hg_rain_settings* rs, float time,
+                           float max_height) {
+    if (!planes || !rs || nx < 1 || ny < 1) return -1;
+    orc::State<double> st;
+    load(st, planes, nx, ny);
+    orc::RefRain rr{rs->amount, rs->mountain_thresh, rs->mountain_multip, rs->drops, time, max_height};
+    orc::pass_rain_reference(st, rr);
+    store(st, planes);
+    return 0;
+}
+
+int orc_rain_reference_f32(float* planes, int nx, int ny, const hg_rain_settings* rs, float time,
+                           float max_height) {
+    if (!planes || !rs || nx < 1 || ny < 1) return -1;
+    orc::State<float> st;
+    load(st, planes, nx, ny);
+    orc::RefRain rr{rs->amount, rs->mountain_thresh, rs->mountain_multip, rs->drops, time, max_height};
+    orc::pass_rain_reference(st, rr);
+    store(st, planes);
+    return 0;
+}
+
+double orc_ref_simplex_f64(double x, double y) { return orc::ref_simplex<double>(x, y); }
+float orc_ref_simplex_f32(float x, float y) { return orc::ref_simplex<float>(x, y); }
+
+double orc_ref_sfbm_f64(double x, double y, double seed, double scale, int octaves) {
+    return orc::ref_sfbm<double>(x, y, seed, 0.5, 2.0, scale, octaves);
+}
+
 // FP64 rollout loss (see rollout_loss); planes are the initial state.
 int orc_rollout_loss_f64(const double* planes, int nx, int ny, const hg_erosion_settings* s,
                          const double coef[4], int rain_mode, float rain_amount, uint64_t seed,

@@ -41,6 +41,21 @@ class ErosionSettings(ctypes.Structure):
     ]
 
 
+class RainSettings(ctypes.Structure):
+    """Rain_data (glsl/bindings.glsl:62-68), std140, 32 bytes."""
+    _fields_ = [("amount", ctypes.c_float), ("mountain_thresh", ctypes.c_float),
+                ("mountain_multip", ctypes.c_float), ("period", ctypes.c_int32), ("drops", ctypes.c_float),
+                ("_pad", ctypes.c_uint32 * 3)]
+
+
+def rain_settings(amount=0.01, mountain_thresh=0.55, mountain_multip=0.05, period=512, drops=0.02):
+    """Defaults: src/settings.hpp:15-21."""
+    r = RainSettings()
+    r.amount = amount; r.mountain_thresh = mountain_thresh; r.mountain_multip = mountain_multip
+    r.period = period; r.drops = drops
+    return r
+
+
 class FbmParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("octaves", ctypes.c_int32),
                 ("lacunarity", ctypes.c_float), ("gain", ctypes.c_float),
@@ -148,6 +163,17 @@ def lib():
         L.orc_rollout_grad_dual.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), dp,
                                             ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int,
                                             ctypes.c_uint, dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, dp, dp]
+        L.orc_ref_simplex_f64.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_ref_simplex_f64.restype = ctypes.c_double
+        L.orc_ref_simplex_f32.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.orc_ref_simplex_f32.restype = ctypes.c_float
+        L.orc_ref_sfbm_f64.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int]
+        L.orc_ref_sfbm_f64.restype = ctypes.c_double
+        for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            getattr(L, f"orc_rain_reference_{name}").argtypes = [
+                ctypes.POINTER(t), ctypes.c_int, ctypes.c_int, ctypes.POINTER(RainSettings), ctypes.c_float,
+                ctypes.c_float]
         for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
             p = ctypes.POINTER(t)
             getattr(L, f"orc_fbm_init_{name}").argtypes = [p, ctypes.c_int, ctypes.c_int,
@@ -233,6 +259,25 @@ def run_pass(state: np.ndarray, settings: ErosionSettings, pass_id: int, rain_mo
             step, pass_id)
     if rc != 0:
         raise ValueError("orc_pass failed")
+    return state
+
+
+def ref_simplex(x: float, y: float, dtype=np.float64) -> float:
+    """gln_simplex restated (glsl/simplex_noise.glsl:377-402)."""
+    return lib().orc_ref_simplex_f64(x, y) if dtype == np.float64 else lib().orc_ref_simplex_f32(x, y)
+
+
+def ref_sfbm(x: float, y: float, seed: float, scale: float, octaves: int = 8) -> float:
+    """gln_sfbm restated with persistence 0.5, lacunarity 2, redistribution 1."""
+    return lib().orc_ref_sfbm_f64(x, y, seed, scale, octaves)
+
+
+def rain_reference(state: np.ndarray, rs: RainSettings, time: float, max_height: float) -> np.ndarray:
+    """One reference rain pass (glsl/rain.glsl:32-56) in place; returns state."""
+    _, ny, nx = state.shape
+    fn = lib().orc_rain_reference_f64 if state.dtype == np.float64 else lib().orc_rain_reference_f32
+    if fn(_ptr(state), nx, ny, ctypes.byref(rs), time, max_height) != 0:
+        raise ValueError("orc_rain_reference failed")
     return state
 
 
