@@ -124,6 +124,161 @@ __global__ void k_mm_reset(unsigned* mm) {
     if (threadIdx.x == 0) { mm[0] = 0xffffffffu; mm[1] = 0u; }
 }
 
+// ---------------------------------------------------------------------------
+// Reference heightmap (glsl/heightmap.glsl:52-158) and its noise
+// (glsl/simplex_noise.glsl:579-676): the Hugo Elias integer hash, gradient
+// noise with quintic interpolation and analytic derivatives (noised), its fBm
+// (perlfbm) and the derivative-damped fBm (erosion_perlfbm).  Every operation
+// follows the shader's order in FP32 (int32 arithmetic wraps), so the host's
+// FP32 restatement agrees bit for bit except through exp().
+// ---------------------------------------------------------------------------
+struct MapParams {
+    float max_height, max_dirt, height_mult, seed, persistance, lacunarity, scale, uplift_scale,
+        domain_warp_scale, terrace_scale;
+    int octaves, domain_warp, terrace;
+    unsigned mask_round, mask_exp, mask_power, mask_slope, uplift;
+};
+
+__device__ __forceinline__ void ehash(int px, int py, float& hx, float& hy) {
+    uint32_t nx = uint32_t(px) * 3u + uint32_t(py) * 311u;       // 2D -> 1D
+    uint32_t ny = uint32_t(px) * 37u + uint32_t(py) * 113u;
+    nx = (nx << 13) ^ nx;
+    ny = (ny << 13) ^ ny;
+    nx = nx * (nx * nx * 15731u + 789221u) + 1376312589u;        // Hugo Elias
+    ny = ny * (ny * ny * 15731u + 789221u) + 1376312589u;
+    const float den = float(0x0fffffff);
+    hx = -1.0f + (2.0f * float(int(nx & 0x0fffffffu))) / den;
+    hy = -1.0f + (2.0f * float(int(ny & 0x0fffffffu))) / den;
+}
+
+// value, d/dx, d/dy
+__device__ __forceinline__ void noised(float px, float py, float& v, float& dx, float& dy) {
+    const float flx = floorf(px), fly = floorf(py);
+    const int ix = int(flx), iy = int(fly);
+    const float fx = px - flx, fy = py - fly;
+    const float ux = ((fx * fx) * fx) * (fx * (fx * 6.0f - 15.0f) + 10.0f);
+    const float uy = ((fy * fy) * fy) * (fy * (fy * 6.0f - 15.0f) + 10.0f);
+    const float dux = ((30.0f * fx) * fx) * (fx * (fx - 2.0f) + 1.0f);
+    const float duy = ((30.0f * fy) * fy) * (fy * (fy - 2.0f) + 1.0f);
+    float gax, gay, gbx, gby, gcx, gcy, gdx, gdy;
+    ehash(ix, iy, gax, gay);
+    ehash(ix + 1, iy, gbx, gby);
+    ehash(ix, iy + 1, gcx, gcy);
+    ehash(ix + 1, iy + 1, gdx, gdy);
+    const float va = gax * fx + gay * fy;
+    const float vb = gbx * (fx - 1.0f) + gby * fy;
+    const float vc = gcx * fx + gcy * (fy - 1.0f);
+    const float vd = gdx * (fx - 1.0f) + gdy * (fy - 1.0f);
+    const float k4 = ((va - vb) - vc) + vd;
+    v = ((va + ux * (vb - va)) + uy * (vc - va)) + (ux * uy) * k4;
+    dx = (((gax + ux * (gbx - gax)) + uy * (gcx - gax)) + (ux * uy) * (((gax - gbx) - gcx) + gdx)) +
+         dux * ((uy * k4 + vb) - va);
+    dy = (((gay + ux * (gby - gay)) + uy * (gcy - gay)) + (ux * uy) * (((gay - gby) - gcy) + gdy)) +
+         duy * ((ux * k4 + vc) - va);
+}
+
+__device__ float perlfbm(float x, float y, const MapParams& m) {
+    x = x + m.seed * 100.0f;
+    y = y + m.seed * 100.0f;
+    float result = 0.0f, amplitude = 1.0f, frequency = 1.0f, maximum = amplitude;
+    for (int o = 0; o < m.octaves; ++o) {
+        float v, dx, dy;
+        noised((x * frequency) * m.scale, (y * frequency) * m.scale, v, dx, dy);
+        result = result + ((v + 1.0f) / 2.0f) * amplitude;
+        frequency = frequency * m.lacunarity;
+        amplitude = amplitude * m.persistance;
+        maximum = maximum + amplitude;
+    }
+    return result / maximum;
+}
+
+__device__ float erosion_perlfbm(float x, float y, const MapParams& m) {
+    x = x + m.seed * 100.0f;
+    y = y + m.seed * 100.0f;
+    float result = 0.0f, amplitude = 1.0f, frequency = 1.5f, maximum = amplitude;
+    float sx = 0.0f, sy = 0.0f;
+    for (int o = 0; o < m.octaves; ++o) {
+        float v, dx, dy;
+        noised((x * frequency) * m.scale, (y * frequency) * m.scale, v, dx, dy);
+        sx = sx + dx;
+        sy = sy + dy;
+        const float nv = (v + 1.0f) / 2.0f;
+        result = result + (nv * amplitude) / (1.0f + (sx * sx + sy * sy));
+        frequency = frequency * m.lacunarity;
+        amplitude = amplitude * m.persistance;
+        maximum = maximum + amplitude;
+    }
+    return result / maximum;
+}
+
+// gln_sfbm with turbulence and ridge (the uplift field)
+__device__ float ridged_sfbm(float x, float y, const MapParams& m) {
+    x = x + m.seed * 100.0f;
+    y = y + m.seed * 100.0f;
+    const float scale = m.scale / m.uplift_scale;
+    float result = 0.0f, amplitude = 1.0f, frequency = 1.0f, maximum = amplitude;
+    for (int o = 0; o < m.octaves; ++o) {
+        float nv = ref_simplex((x * frequency) * scale, (y * frequency) * scale);
+        nv = 1.0f - fabsf(nv);
+        result = result + nv * amplitude;
+        frequency = frequency * m.lacunarity;
+        amplitude = amplitude * m.persistance;
+        maximum = maximum + amplitude;
+    }
+    return result / maximum;
+}
+
+__global__ __launch_bounds__(256) void k_gen_heightmap(float* rock, float* dirt, int n, MapParams m) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= n || y >= n) return;
+    const float fx = float(x), fy = float(y);
+    const float uvx = fx / float(n), uvy = fy / float(n);
+    float distx = 1.0f, disty = 1.0f;
+    if (m.domain_warp != 0) {
+        distx = perlfbm(fx + 2.3f, fy + 2.9f, m);
+        disty = perlfbm(fx - 3.1f, fy - 4.3f, m);
+        if (m.domain_warp == 2) {
+            const float wx = fx + m.domain_warp_scale * distx, wy = fy + m.domain_warp_scale * disty;
+            distx = perlfbm(wx - 5.7f, wy + 27.9f, m);
+            disty = perlfbm(wx + 11.5f, wy - 23.7f, m);
+        }
+    }
+    float val = erosion_perlfbm(fx + m.domain_warp_scale * distx, fy + m.domain_warp_scale * disty, m);
+    float hm = m.height_mult;
+    if (m.uplift != 0) {
+        const float up = ridged_sfbm(fx - 7.3f, fy + 19.9f, m);
+        if (m.mask_exp != 0) hm += 2.5f;
+        hm += 1.0f;
+        val *= up;
+    }
+    if (m.mask_round != 0) {
+        if (m.mask_exp != 0) hm += 16.0f;
+        const float ax = uvx - 0.5f, ay = uvy - 0.5f;
+        const float v = 1.0f - ((ax * ax + ay * ay) + 0.75f);
+        val = val * fmaxf(0.0f, v * 1.0f);
+    }
+    if (m.mask_exp != 0) {
+        hm += 2.0f;
+        val = (val * (expf(val) - 1.0f)) / 1.718f;
+    }
+    if (m.mask_power != 0) {
+        hm += 1.25f;
+        if (m.mask_exp != 0) hm += 2.0f;
+        const float b = val + 0.5f;
+        val = val * ((((b * b) * b - 0.125f) / 3.25f) * 0.55f + 0.45f);
+    }
+    if (m.mask_slope != 0) val = 0.25f * val + 0.75f * ((val * uvx) * uvy);
+    val = val + float(4u * m.mask_round) * val;
+    if (m.terrace > 0) {
+        const float step = 1.0f / (float(m.terrace) * hm);
+        const float lol = floorf(val / step);
+        val = ((m.terrace_scale * lol) * step) + val * (1.0f - m.terrace_scale);
+    }
+    const size_t i = size_t(y) * size_t(n) + size_t(x);
+    rock[i] = fminf(m.max_height, (val * m.max_height) * hm);
+    dirt[i] = (perlfbm(fx + 13.7f, fy + 27.1f, m) + 1.5f) * m.max_dirt;
+}
+
 }  // namespace
 
 // Philox-driven Fisher-Yates permutation, identical to the oracle's.
@@ -160,6 +315,19 @@ hipError_t launch_minmax(const float* v, size_t n, unsigned* mm_dev, hipStream_t
 hipError_t launch_rescale(float* v, size_t n, const unsigned* mm_dev, float lo, float hi,
                           hipStream_t s) {
     k_rescale<<<unsigned((n + 255) / 256), 256, 0, s>>>(v, n, mm_dev, lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_heightmap(float* rock, float* dirt, int n, const hg_map_settings& ms, hipStream_t s) {
+    MapParams m;
+    m.max_height = ms.max_height; m.max_dirt = ms.max_dirt; m.height_mult = ms.height_mult; m.seed = ms.seed;
+    m.persistance = ms.persistance; m.lacunarity = ms.lacunarity; m.scale = ms.scale;
+    m.uplift_scale = ms.uplift_scale; m.domain_warp_scale = ms.domain_warp_scale;
+    m.terrace_scale = ms.terrace_scale; m.octaves = ms.octaves; m.domain_warp = ms.domain_warp;
+    m.terrace = ms.terrace; m.mask_round = ms.mask_round; m.mask_exp = ms.mask_exp;
+    m.mask_power = ms.mask_power; m.mask_slope = ms.mask_slope; m.uplift = ms.uplift;
+    const dim3 grid((n + 63) / 64, (n + 3) / 4);
+    k_gen_heightmap<<<grid, 256, 0, s>>>(rock, dirt, n, m);
     return hipGetLastError();
 }
 

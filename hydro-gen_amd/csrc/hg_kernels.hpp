@@ -98,5 +98,6 @@ hipError_t launch_minmax(const float* v, size_t n, unsigned* mm_dev, hipStream_t
 hipError_t launch_rescale(float* v, size_t n, const unsigned* mm_dev, float lo, float hi,
                           hipStream_t s);
 hipError_t launch_fill(float* p, size_t n, float value, hipStream_t s);
+hipError_t launch_gen_heightmap(float* rock, float* dirt, int n, const hg_map_settings& m, hipStream_t s);
 
 }  // namespace hgk

@@ -368,6 +368,30 @@ int hg_set_rain_time(hg_ctx* c, float time, float max_height) {
     return HG_OK;
 }
 
+static_assert(sizeof(hg_map_settings) == 96, "Map_settings_data is 96 bytes in std140");
+
+int hg_gen_heightmap(hg_ctx* c, const hg_map_settings* m) {
+    if (!c) return HG_ERR_INVALID;
+    if (!m) return fail(c, HG_ERR_INVALID, "hg_gen_heightmap: settings is NULL");
+    const float f[] = {m->max_height, m->max_dirt, m->height_mult, m->seed, m->persistance, m->lacunarity,
+                       m->scale, m->uplift_scale, m->domain_warp_scale, m->terrace_scale};
+    for (float v : f)
+        if (!std::isfinite(v)) return fail(c, HG_ERR_INVALID, "hg_gen_heightmap: non-finite setting");
+    if (m->octaves < 0 || m->octaves > 30 || m->domain_warp < 0 || m->domain_warp > 2 ||
+        (m->uplift != 0 && m->uplift_scale == 0.0f))
+        return fail(c, HG_ERR_INVALID, "hg_gen_heightmap: octaves 0..30, domain_warp 0..2, uplift_scale != 0");
+    DeviceGuard guard(c->device);
+    const size_t pn = plane_n(c);
+    for (int pl = 0; pl < HG_NUM_PLANES; ++pl)
+        if (pl != HG_P_ROCK && pl != HG_P_DIRT) HG_HIP(c, hgk::launch_fill(c->cur[pl], pn, 0.0f, c->stream));
+    int rc = timed_launch(c, "gen_heightmap", [&] {
+        return hgk::launch_gen_heightmap(c->cur[HG_P_ROCK], c->cur[HG_P_DIRT], c->n, *m, c->stream);
+    });
+    if (rc != HG_OK) return rc;
+    c->step = 0;
+    return HG_OK;
+}
+
 int hg_init_fbm(hg_ctx* c, const hg_fbm_params* p) {
     if (!c) return HG_ERR_INVALID;
     if (!p) return fail(c, HG_ERR_INVALID, "hg_init_fbm: params is NULL");

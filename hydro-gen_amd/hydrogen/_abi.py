@@ -33,7 +33,7 @@ NUM_PASSES = 9
 # Every exported symbol of include/hg.h (checked by the ABI test).
 EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_size", "hg_set_erosion", "hg_get_erosion",
-    "hg_set_rain", "hg_set_rain_time", "hg_init_fbm", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
+    "hg_set_rain", "hg_set_rain_time", "hg_init_fbm", "hg_gen_heightmap", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
     "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
@@ -65,6 +65,33 @@ class RainSettings(ctypes.Structure):
     _fields_ = [("amount", ctypes.c_float), ("mountain_thresh", ctypes.c_float),
                 ("mountain_multip", ctypes.c_float), ("period", ctypes.c_int32),
                 ("drops", ctypes.c_float), ("_pad", ctypes.c_uint32 * 3)]
+
+
+class MapSettings(ctypes.Structure):
+    """hg_map_settings = Map_settings_data (glsl/bindings.glsl:70-97), 96 bytes."""
+    _fields_ = [("max_height", ctypes.c_float), ("max_dirt", ctypes.c_float), ("hmap_dims", ctypes.c_int32 * 2),
+                ("height_mult", ctypes.c_float), ("water_lvl", ctypes.c_float), ("seed", ctypes.c_float),
+                ("persistance", ctypes.c_float), ("lacunarity", ctypes.c_float), ("scale", ctypes.c_float),
+                ("redistribution", ctypes.c_float), ("octaves", ctypes.c_int32), ("fake_erosion", ctypes.c_uint32),
+                ("mask_round", ctypes.c_uint32), ("mask_exp", ctypes.c_uint32), ("mask_power", ctypes.c_uint32),
+                ("mask_slope", ctypes.c_uint32), ("uplift", ctypes.c_uint32), ("uplift_scale", ctypes.c_float),
+                ("domain_warp", ctypes.c_int32), ("domain_warp_scale", ctypes.c_float), ("terrace", ctypes.c_int32),
+                ("terrace_scale", ctypes.c_float), ("_pad", ctypes.c_uint32)]
+
+
+def map_settings(**kw) -> MapSettings:
+    """Map settings with the reference defaults (src/settings.hpp:29-55); the
+    reference draws its seed from rand(), here it is a parameter (1234.5)."""
+    d = dict(max_height=256.0, max_dirt=2.0, height_mult=1.0, water_lvl=96.0, seed=1234.5, persistance=0.44,
+             lacunarity=2.0, scale=0.00075, redistribution=1.0, octaves=8, fake_erosion=0, mask_round=0,
+             mask_exp=1, mask_power=1, mask_slope=0, uplift=0, uplift_scale=1.16, domain_warp=1,
+             domain_warp_scale=100.0, terrace=0, terrace_scale=0.5)
+    d.update(kw)
+    m = MapSettings()
+    m.hmap_dims[0] = m.hmap_dims[1] = 1024
+    for k, v in d.items():
+        setattr(m, k, v)
+    return m
 
 
 class FbmParams(ctypes.Structure):
@@ -166,6 +193,7 @@ def load() -> ctypes.CDLL:
     L.hg_set_rain.argtypes = [vp, ctypes.POINTER(RainSettings), ctypes.c_int, ctypes.c_uint64]
     L.hg_set_rain_time.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     L.hg_init_fbm.argtypes = [vp, ctypes.POINTER(FbmParams)]
+    L.hg_gen_heightmap.argtypes = [vp, ctypes.POINTER(MapSettings)]
     L.hg_rain.argtypes = [vp]
     L.hg_step_grid.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.hg_run_pass.argtypes = [vp, ctypes.c_int]
@@ -276,6 +304,10 @@ class Terrain:
     def set_rain_time(self, time: float, max_height: float = 256.0):
         """Time seed and map max height of RAIN_REFERENCE (hg_set_rain_time)."""
         self._check(self.lib.hg_set_rain_time(self.h, time, max_height))
+
+    def gen_heightmap(self, m: MapSettings):
+        """Reference heightmap generator (hg_gen_heightmap)."""
+        self._check(self.lib.hg_gen_heightmap(self.h, ctypes.byref(m)))
 
     def init_fbm(self, p: FbmParams):
         self._check(self.lib.hg_init_fbm(self.h, ctypes.byref(p)))

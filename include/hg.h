@@ -73,6 +73,34 @@ typedef struct hg_rain_settings {
     uint32_t _pad[3];        /* 20 */
 } hg_rain_settings;
 
+/* Map_settings_data, glsl/bindings.glsl:70-97 (std140, 96 bytes): settings of
+ * the reference heightmap generator (defaults src/settings.hpp:29-55). */
+typedef struct hg_map_settings {
+    float max_height;        /*  0 */
+    float max_dirt;          /*  4 */
+    int32_t hmap_dims[2];    /*  8  not used: the grid is the context's n x n */
+    float height_mult;       /* 16 */
+    float water_lvl;         /* 20  not used by the generator */
+    float seed;              /* 24 */
+    float persistance;       /* 28 */
+    float lacunarity;        /* 32 */
+    float scale;             /* 36 */
+    float redistribution;    /* 40  not used: the shader passes 1.0 */
+    int32_t octaves;         /* 44 */
+    uint32_t fake_erosion;   /* 48 */
+    uint32_t mask_round;     /* 52 */
+    uint32_t mask_exp;       /* 56 */
+    uint32_t mask_power;     /* 60 */
+    uint32_t mask_slope;     /* 64 */
+    uint32_t uplift;         /* 68 */
+    float uplift_scale;      /* 72 */
+    int32_t domain_warp;     /* 76 */
+    float domain_warp_scale; /* 80 */
+    int32_t terrace;         /* 84 */
+    float terrace_scale;     /* 88 */
+    uint32_t _pad;           /* 92 */
+} hg_map_settings;
+
 /* Rain modes.
  *  CONSTANT:  water += amount on every cell, every step.
  *  UNIFORM:   water += amount * u, u = (Philox4x32-10(ctr, key) >> 8) * 2^-24 with
@@ -184,6 +212,15 @@ int hg_set_rain_time(hg_ctx* ctx, float time, float max_height);
 /* Config heightmap init (in place of World::gen_heightmap,
  * src/state.cpp:116-147).  Resets the step counter to 0. */
 int hg_init_fbm(hg_ctx* ctx, const hg_fbm_params* p);
+
+/* The reference heightmap (replaces World::gen_heightmap, src/state.cpp:116-147,
+ * and glsl/heightmap.glsl:52-158): domain-warped, derivative-damped gradient
+ * noise fBm (erosion_perlfbm), optional ridged simplex uplift, round / exp /
+ * power / slope masks and terraces into rock, perlfbm into dirt; water, flux,
+ * velocity and sediment are zeroed.  Two choices where GLSL is undefined or
+ * inexact: pow(x, 2) of a negative x in the round mask is x * x, and
+ * pow(x, 3) of the power mask is x * x * x.  Resets the step counter to 0. */
+int hg_gen_heightmap(hg_ctx* ctx, const hg_map_settings* m);
 
 /* ---- stepping ----------------------------------------------------------- */
 

@@ -44,6 +44,7 @@ using std::sqrt;
 using std::atan;
 using std::floor;
 using std::fabs;
+using std::exp;
 
 // FP32 literal 999999999999.0 used by the reference for out-of-grid heights
 // (glsl/hydro_flux.glsl:45-52, glsl/thermal_erosion.glsl:20-26).
@@ -310,6 +311,160 @@ template <class T> void pass_rain_reference(State<T>& s, const RefRain& rr) {
             const T w = s.r[i] + s.d[i] + s.b[i];
             s.b[i] = s.b[i] + ref_rain_incr(x, y, w, rr);
         }
+}
+
+// ---------------------------------------------------------------------------
+// Reference heightmap generator (glsl/heightmap.glsl:52-158) and its noise
+// (glsl/simplex_noise.glsl:579-676).  int32 arithmetic of the hash wraps; the
+// literals are FP32 constants; GLSL's pow(x, 2) of a negative x (round mask)
+// is x*x and pow(x, 3) (power mask) is x*x*x; pow(result, 1) of the fBms is
+// result.
+// ---------------------------------------------------------------------------
+// Hugo Elias hash of an integer lattice point to a gradient in [-1, 1]^2.
+template <class T> void elias_hash(int px, int py, T& gx, T& gy) {
+    const uint32_t ux = uint32_t(px), uy = uint32_t(py);
+    uint32_t n[2] = {ux * 3u + uy * 311u, ux * 37u + uy * 113u};
+    T g[2];
+    for (int c = 0; c < 2; ++c) {
+        uint32_t v = (n[c] << 13) ^ n[c];
+        v = v * (v * v * 15731u + 789221u) + 1376312589u;
+        const int32_t lo = int32_t(v & 0x0fffffffu);
+        // float(int) rounds to FP32 first, like the shader's vec2(ivec2)
+        g[c] = T(-1.0f) + (T(2.0f) * T(float(lo))) / T(float(0x0fffffff));
+    }
+    gx = g[0];
+    gy = g[1];
+}
+
+// Gradient noise with quintic interpolation: value and analytic gradient.
+template <class T> void noised(T px, T py, T& v, T& dvx, T& dvy) {
+    const T flx = floor(px), fly = floor(py);
+    const int ix = int(val(flx)), iy = int(val(fly));
+    const T fx = px - flx, fy = py - fly;
+    auto quintic = [](T f) { return ((f * f) * f) * (f * (f * T(6) - T(15)) + T(10)); };
+    auto dquintic = [](T f) { return ((T(30) * f) * f) * (f * (f - T(2)) + T(1)); };
+    const T ux = quintic(fx), uy = quintic(fy), dux = dquintic(fx), duy = dquintic(fy);
+    T ga[2], gb[2], gc[2], gd[2];
+    elias_hash(ix, iy, ga[0], ga[1]);
+    elias_hash(ix + 1, iy, gb[0], gb[1]);
+    elias_hash(ix, iy + 1, gc[0], gc[1]);
+    elias_hash(ix + 1, iy + 1, gd[0], gd[1]);
+    const T va = ga[0] * fx + ga[1] * fy;
+    const T vb = gb[0] * (fx - T(1)) + gb[1] * fy;
+    const T vc = gc[0] * fx + gc[1] * (fy - T(1));
+    const T vd = gd[0] * (fx - T(1)) + gd[1] * (fy - T(1));
+    const T k4 = ((va - vb) - vc) + vd;
+    v = ((va + ux * (vb - va)) + uy * (vc - va)) + (ux * uy) * k4;
+    T d[2];
+    const T uo[2] = {uy, ux}, du[2] = {dux, duy}, vbc[2] = {vb, vc};
+    for (int c = 0; c < 2; ++c)
+        d[c] = (((ga[c] + ux * (gb[c] - ga[c])) + uy * (gc[c] - ga[c])) +
+                (ux * uy) * (((ga[c] - gb[c]) - gc[c]) + gd[c])) +
+               du[c] * ((uo[c] * k4 + vbc[c]) - va);
+    dvx = d[0];
+    dvy = d[1];
+}
+
+struct MapCfg {
+    float max_height, max_dirt, height_mult, seed, persistance, lacunarity, scale, uplift_scale,
+        domain_warp_scale, terrace_scale;
+    int octaves, domain_warp, terrace;
+    unsigned mask_round, mask_exp, mask_power, mask_slope, uplift;
+};
+
+// perlfbm (damped = false) and erosion_perlfbm (damped = true)
+template <class T> T grad_fbm(T x, T y, const MapCfg& m, bool damped) {
+    x = x + T(m.seed) * T(100);
+    y = y + T(m.seed) * T(100);
+    T result = T(0), amplitude = T(1), frequency = damped ? T(1.5f) : T(1), maximum = amplitude;
+    T sx = T(0), sy = T(0);
+    for (int o = 0; o < m.octaves; ++o) {
+        T v, dx, dy;
+        noised((x * frequency) * T(m.scale), (y * frequency) * T(m.scale), v, dx, dy);
+        const T nv = (v + T(1)) / T(2);
+        if (damped) {
+            sx = sx + dx;
+            sy = sy + dy;
+            result = result + (nv * amplitude) / (T(1) + (sx * sx + sy * sy));
+        } else {
+            result = result + nv * amplitude;
+        }
+        frequency = frequency * T(m.lacunarity);
+        amplitude = amplitude * T(m.persistance);
+        maximum = maximum + amplitude;
+    }
+    return result / maximum;
+}
+
+template <class T> T ridged_sfbm(T x, T y, const MapCfg& m) {
+    x = x + T(m.seed) * T(100);
+    y = y + T(m.seed) * T(100);
+    const T scale = T(m.scale) / T(m.uplift_scale);
+    T result = T(0), amplitude = T(1), frequency = T(1), maximum = amplitude;
+    for (int o = 0; o < m.octaves; ++o) {
+        const T nv = T(1) - fabs(ref_simplex((x * frequency) * scale, (y * frequency) * scale));
+        result = result + nv * amplitude;
+        frequency = frequency * T(m.lacunarity);
+        amplitude = amplitude * T(m.persistance);
+        maximum = maximum + amplitude;
+    }
+    return result / maximum;
+}
+
+template <class T> void gen_heightmap_cell(int x, int y, int n, const MapCfg& m, T& rock, T& dirt) {
+    const T fx = T(float(x)), fy = T(float(y));
+    const T uvx = fx / T(float(n)), uvy = fy / T(float(n));
+    T distx = T(1), disty = T(1);
+    if (m.domain_warp != 0) {
+        distx = grad_fbm(fx + T(2.3f), fy + T(2.9f), m, false);
+        disty = grad_fbm(fx - T(3.1f), fy - T(4.3f), m, false);
+        if (m.domain_warp == 2) {
+            const T wx = fx + T(m.domain_warp_scale) * distx, wy = fy + T(m.domain_warp_scale) * disty;
+            distx = grad_fbm(wx - T(5.7f), wy + T(27.9f), m, false);
+            disty = grad_fbm(wx + T(11.5f), wy - T(23.7f), m, false);
+        }
+    }
+    T v = grad_fbm(fx + T(m.domain_warp_scale) * distx, fy + T(m.domain_warp_scale) * disty, m, true);
+    T hm = T(m.height_mult);
+    if (m.uplift != 0) {
+        const T up = ridged_sfbm(fx - T(7.3f), fy + T(19.9f), m);
+        if (m.mask_exp != 0) hm = hm + T(2.5f);
+        hm = hm + T(1);
+        v = v * up;
+    }
+    if (m.mask_round != 0) {
+        if (m.mask_exp != 0) hm = hm + T(16);
+        const T ax = uvx - T(0.5f), ay = uvy - T(0.5f);
+        const T w = T(1) - ((ax * ax + ay * ay) + T(0.75f));
+        v = v * tmax(T(0), w * T(1));
+    }
+    if (m.mask_exp != 0) {
+        hm = hm + T(2);
+        v = (v * (exp(v) - T(1))) / T(1.718f);
+    }
+    if (m.mask_power != 0) {
+        hm = hm + T(1.25f);
+        if (m.mask_exp != 0) hm = hm + T(2);
+        const T b = v + T(0.5f);
+        v = v * ((((b * b) * b - T(0.125f)) / T(3.25f)) * T(0.55f) + T(0.45f));
+    }
+    if (m.mask_slope != 0) v = T(0.25f) * v + T(0.75f) * ((v * uvx) * uvy);
+    v = v + T(float(4u * m.mask_round)) * v;
+    if (m.terrace > 0) {
+        const T step = T(1) / (T(float(m.terrace)) * hm);
+        const T lol = floor(v / step);
+        v = ((T(m.terrace_scale) * lol) * step) + v * (T(1) - T(m.terrace_scale));
+    }
+    rock = tmin(T(m.max_height), (v * T(m.max_height)) * hm);
+    dirt = (grad_fbm(fx + T(13.7f), fy + T(27.1f), m, false) + T(1.5f)) * T(m.max_dirt);
+}
+
+template <class T> void gen_heightmap(std::vector<T>& rock, std::vector<T>& dirt, int n, const MapCfg& m) {
+    rock.assign(size_t(n) * n, T(0));
+    dirt.assign(size_t(n) * n, T(0));
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) gen_heightmap_cell(x, y, n, m, rock[size_t(y) * n + x], dirt[size_t(y) * n + x]);
 }
 
 // ---------------------------------------------------------------------------

@@ -220,6 +220,40 @@ double orc_ref_sfbm_f64(double x, double y, double seed, double scale, int octav
     return orc::ref_sfbm<double>(x, y, seed, 0.5, 2.0, scale, octaves);
 }
 
+static orc::MapCfg map_cfg(const hg_map_settings* s) {
+    orc::MapCfg m;
+    m.max_height = s->max_height; m.max_dirt = s->max_dirt; m.height_mult = s->height_mult; m.seed = s->seed;
+    m.persistance = s->persistance; m.lacunarity = s->lacunarity; m.scale = s->scale;
+    m.uplift_scale = s->uplift_scale; m.domain_warp_scale = s->domain_warp_scale;
+    m.terrace_scale = s->terrace_scale; m.octaves = s->octaves; m.domain_warp = s->domain_warp;
+    m.terrace = s->terrace; m.mask_round = s->mask_round; m.mask_exp = s->mask_exp;
+    m.mask_power = s->mask_power; m.mask_slope = s->mask_slope; m.uplift = s->uplift;
+    return m;
+}
+
+// Reference heightmap (rock and dirt planes of n x n).
+int orc_gen_heightmap_f64(double* rock, double* dirt, int n, const hg_map_settings* s) {
+    if (!rock || !dirt || !s || n < 1) return -1;
+    std::vector<double> r, d;
+    orc::gen_heightmap<double>(r, d, n, map_cfg(s));
+    std::memcpy(rock, r.data(), r.size() * sizeof(double));
+    std::memcpy(dirt, d.data(), d.size() * sizeof(double));
+    return 0;
+}
+
+int orc_gen_heightmap_f32(float* rock, float* dirt, int n, const hg_map_settings* s) {
+    if (!rock || !dirt || !s || n < 1) return -1;
+    std::vector<float> r, d;
+    orc::gen_heightmap<float>(r, d, n, map_cfg(s));
+    std::memcpy(rock, r.data(), r.size() * sizeof(float));
+    std::memcpy(dirt, d.data(), d.size() * sizeof(float));
+    return 0;
+}
+
+void orc_elias_hash(int px, int py, double out[2]) { orc::elias_hash<double>(px, py, out[0], out[1]); }
+
+void orc_noised_f64(double x, double y, double out[3]) { orc::noised<double>(x, y, out[0], out[1], out[2]); }
+
 // FP64 rollout loss (see rollout_loss); planes are the initial state.
 int orc_rollout_loss_f64(const double* planes, int nx, int ny, const hg_erosion_settings* s,
                          const double coef[4], int rain_mode, float rain_amount, uint64_t seed,

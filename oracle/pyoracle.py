@@ -56,6 +56,32 @@ def rain_settings(amount=0.01, mountain_thresh=0.55, mountain_multip=0.05, perio
     return r
 
 
+class MapSettings(ctypes.Structure):
+    """Map_settings_data (glsl/bindings.glsl:70-97), std140, 96 bytes."""
+    _fields_ = [("max_height", ctypes.c_float), ("max_dirt", ctypes.c_float), ("hmap_dims", ctypes.c_int32 * 2),
+                ("height_mult", ctypes.c_float), ("water_lvl", ctypes.c_float), ("seed", ctypes.c_float),
+                ("persistance", ctypes.c_float), ("lacunarity", ctypes.c_float), ("scale", ctypes.c_float),
+                ("redistribution", ctypes.c_float), ("octaves", ctypes.c_int32), ("fake_erosion", ctypes.c_uint32),
+                ("mask_round", ctypes.c_uint32), ("mask_exp", ctypes.c_uint32), ("mask_power", ctypes.c_uint32),
+                ("mask_slope", ctypes.c_uint32), ("uplift", ctypes.c_uint32), ("uplift_scale", ctypes.c_float),
+                ("domain_warp", ctypes.c_int32), ("domain_warp_scale", ctypes.c_float), ("terrace", ctypes.c_int32),
+                ("terrace_scale", ctypes.c_float), ("_pad", ctypes.c_uint32)]
+
+
+def map_settings(**kw) -> MapSettings:
+    """The reference defaults (src/settings.hpp:29-55) with a fixed seed."""
+    d = dict(max_height=256.0, max_dirt=2.0, height_mult=1.0, water_lvl=96.0, seed=1234.5, persistance=0.44,
+             lacunarity=2.0, scale=0.00075, redistribution=1.0, octaves=8, fake_erosion=0, mask_round=0,
+             mask_exp=1, mask_power=1, mask_slope=0, uplift=0, uplift_scale=1.16, domain_warp=1,
+             domain_warp_scale=100.0, terrace=0, terrace_scale=0.5)
+    d.update(kw)
+    m = MapSettings()
+    m.hmap_dims[0] = m.hmap_dims[1] = 1024
+    for k, v in d.items():
+        setattr(m, k, v)
+    return m
+
+
 class FbmParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("octaves", ctypes.c_int32),
                 ("lacunarity", ctypes.c_float), ("gain", ctypes.c_float),
@@ -163,6 +189,11 @@ def lib():
         L.orc_rollout_grad_dual.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), dp,
                                             ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int,
                                             ctypes.c_uint, dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, dp, dp]
+        for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            getattr(L, f"orc_gen_heightmap_{name}").argtypes = [ctypes.POINTER(t), ctypes.POINTER(t), ctypes.c_int,
+                                                                 ctypes.POINTER(MapSettings)]
+        L.orc_elias_hash.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.orc_noised_f64.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.orc_ref_simplex_f64.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_ref_simplex_f64.restype = ctypes.c_double
         L.orc_ref_simplex_f32.argtypes = [ctypes.c_float, ctypes.c_float]
@@ -279,6 +310,29 @@ def rain_reference(state: np.ndarray, rs: RainSettings, time: float, max_height:
     if fn(_ptr(state), nx, ny, ctypes.byref(rs), time, max_height) != 0:
         raise ValueError("orc_rain_reference failed")
     return state
+
+
+def gen_heightmap(n: int, m: MapSettings, dtype=np.float64):
+    """Reference heightmap generator: (rock, dirt) planes of n x n."""
+    rock = np.zeros((n, n), dtype=dtype)
+    dirt = np.zeros((n, n), dtype=dtype)
+    fn = lib().orc_gen_heightmap_f64 if dtype == np.float64 else lib().orc_gen_heightmap_f32
+    if fn(_ptr(rock), _ptr(dirt), n, ctypes.byref(m)) != 0:
+        raise ValueError("orc_gen_heightmap failed")
+    return rock, dirt
+
+
+def elias_hash(px: int, py: int):
+    out = (ctypes.c_double * 2)()
+    lib().orc_elias_hash(px, py, out)
+    return out[0], out[1]
+
+
+def noised(x: float, y: float):
+    """(value, d/dx, d/dy) of the reference gradient noise."""
+    out = (ctypes.c_double * 3)()
+    lib().orc_noised_f64(x, y, out)
+    return out[0], out[1], out[2]
 
 
 def rollout_loss(state: np.ndarray, settings: ErosionSettings, coef, nsteps: int, target: np.ndarray,

@@ -49,6 +49,17 @@ def test_rain_settings_std140_layout():
             R.drops.offset) == (0, 4, 8, 12, 16)
 
 
+def test_map_settings_std140_layout():
+    offs = dict(max_height=0, max_dirt=4, hmap_dims=8, height_mult=16, water_lvl=20, seed=24, persistance=28,
+                lacunarity=32, scale=36, redistribution=40, octaves=44, fake_erosion=48, mask_round=52,
+                mask_exp=56, mask_power=60, mask_slope=64, uplift=68, uplift_scale=72, domain_warp=76,
+                domain_warp_scale=80, terrace=84, terrace_scale=88)
+    for S in (hg.MapSettings, O.MapSettings):
+        assert ctypes.sizeof(S) == 96
+        for name, off in offs.items():
+            assert getattr(S, name).offset == off, name
+
+
 def test_create_without_device_fails_cleanly():
     # In a container without a GPU hg_create must fail with a message, not crash.
     import torch
