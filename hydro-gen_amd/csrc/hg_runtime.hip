@@ -587,6 +587,59 @@ int hg_read(hg_ctx* c, int channel, float* rgba, size_t n_floats) {
     return HG_OK;
 }
 
+// ---- heightmap files: NumPy .npy v1.0, little-endian float32, shape (n, n, 4)
+// holding the HEIGHT channel (rock, dirt, water, total) in row-major order.
+namespace {
+std::string npy_header(size_t n) {
+    std::string dict = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(n) + ", " +
+                       std::to_string(n) + ", 4), }";
+    const size_t base = 10 + dict.size() + 1;              // magic+version+len, dict, '\n'
+    dict.append((64 - base % 64) % 64, ' ');
+    dict.push_back('\n');
+    std::string h("\x93NUMPY\x01\x00", 8);
+    h.push_back(char(dict.size() & 0xff));
+    h.push_back(char(dict.size() >> 8));
+    return h + dict;
+}
+}  // namespace
+
+int hg_save_height(hg_ctx* c, const char* path) {
+    if (!c) return HG_ERR_INVALID;
+    if (!path) return fail(c, HG_ERR_INVALID, "hg_save_height: path is NULL");
+    const size_t pn = plane_n(c);
+    std::vector<float> rgba(4 * pn);
+    int rc = hg_read(c, HG_CH_HEIGHT, rgba.data(), rgba.size());
+    if (rc != HG_OK) return rc;
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(c, HG_ERR_INVALID, std::string("hg_save_height: cannot open ") + path);
+    const std::string h = npy_header(size_t(c->n));
+    const bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size() &&
+                    std::fwrite(rgba.data(), sizeof(float), rgba.size(), f) == rgba.size();
+    const bool closed = std::fclose(f) == 0;
+    if (!ok || !closed) return fail(c, HG_ERR_INVALID, std::string("hg_save_height: write failed: ") + path);
+    return HG_OK;
+}
+
+int hg_load_height(hg_ctx* c, const char* path) {
+    if (!c) return HG_ERR_INVALID;
+    if (!path) return fail(c, HG_ERR_INVALID, "hg_load_height: path is NULL");
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(c, HG_ERR_INVALID, std::string("hg_load_height: cannot open ") + path);
+    const std::string want = npy_header(size_t(c->n));
+    std::string got(want.size(), '\0');
+    const size_t pn = plane_n(c);
+    std::vector<float> rgba(4 * pn);
+    const bool ok = std::fread(&got[0], 1, got.size(), f) == got.size() && got == want &&
+                    std::fread(rgba.data(), sizeof(float), rgba.size(), f) == rgba.size() &&
+                    std::fgetc(f) == EOF;
+    std::fclose(f);
+    if (!ok)
+        return fail(c, HG_ERR_INVALID,
+                    std::string("hg_load_height: not a little-endian float32 .npy of shape (n, n, 4) for this n: ") +
+                        path);
+    return hg_write(c, HG_CH_HEIGHT, rgba.data(), rgba.size());
+}
+
 int hg_write(hg_ctx* c, int channel, const float* rgba, size_t n_floats) {
     if (!c) return HG_ERR_INVALID;
     const size_t pn = plane_n(c);

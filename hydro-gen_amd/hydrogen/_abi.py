@@ -34,7 +34,7 @@ NUM_PASSES = 9
 EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_size", "hg_set_erosion", "hg_get_erosion",
     "hg_set_rain", "hg_set_rain_time", "hg_init_fbm", "hg_gen_heightmap", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
-    "hg_set_step", "hg_read", "hg_write", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
+    "hg_set_step", "hg_read", "hg_write", "hg_save_height", "hg_load_height", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
@@ -193,6 +193,8 @@ def load() -> ctypes.CDLL:
     L.hg_set_rain.argtypes = [vp, ctypes.POINTER(RainSettings), ctypes.c_int, ctypes.c_uint64]
     L.hg_set_rain_time.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     L.hg_init_fbm.argtypes = [vp, ctypes.POINTER(FbmParams)]
+    L.hg_save_height.argtypes = [vp, ctypes.c_char_p]
+    L.hg_load_height.argtypes = [vp, ctypes.c_char_p]
     L.hg_gen_heightmap.argtypes = [vp, ctypes.POINTER(MapSettings)]
     L.hg_rain.argtypes = [vp]
     L.hg_step_grid.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
@@ -304,6 +306,14 @@ class Terrain:
     def set_rain_time(self, time: float, max_height: float = 256.0):
         """Time seed and map max height of RAIN_REFERENCE (hg_set_rain_time)."""
         self._check(self.lib.hg_set_rain_time(self.h, time, max_height))
+
+    def save_height(self, path: str):
+        """HEIGHT channel to a .npy file of shape (n, n, 4) (hg_save_height)."""
+        self._check(self.lib.hg_save_height(self.h, os.fsencode(path)))
+
+    def load_height(self, path: str):
+        """HEIGHT channel from a .npy file written by save_height (hg_load_height)."""
+        self._check(self.lib.hg_load_height(self.h, os.fsencode(path)))
 
     def gen_heightmap(self, m: MapSettings):
         """Reference heightmap generator (hg_gen_heightmap)."""

@@ -248,6 +248,13 @@ int hg_set_step(hg_ctx* ctx, uint64_t step);
 int hg_read(hg_ctx* ctx, int channel, float* rgba, size_t n_floats);
 int hg_write(hg_ctx* ctx, int channel, const float* rgba, size_t n_floats);
 
+/* Heightmap files (no counterpart in the reference, which has none): the
+ * HEIGHT channel as a NumPy .npy v1.0 file, little-endian float32, shape
+ * (n, n, 4) = (rock, dirt, water, total) row-major.  Load requires the same n
+ * and ignores the stored total. */
+int hg_save_height(hg_ctx* ctx, const char* path);
+int hg_load_height(hg_ctx* ctx, const char* path);
+
 /* Plane I/O (HG_P_*), n_floats must be n*n. */
 int hg_read_plane(hg_ctx* ctx, int plane, float* dst, size_t n_floats);
 int hg_write_plane(hg_ctx* ctx, int plane, const float* src, size_t n_floats);

@@ -55,3 +55,24 @@ def test_heightmap_settings_rejected():
                     dict(scale=float("nan"))):
             with pytest.raises(hg.HGError):
                 t.gen_heightmap(hg.map_settings(**bad))
+
+
+def test_height_file_round_trip(tmp_path):
+    n = 64
+    path = str(tmp_path / "h.npy")
+    with hg.Terrain(n, device=0) as t:
+        t.gen_heightmap(hg.map_settings(seed=3.0))
+        t.dispatch_grid(5)
+        want = t.read(hg.CH_HEIGHT)
+        t.save_height(path)
+    a = np.load(path)
+    assert a.shape == (n, n, 4) and a.dtype == np.float32
+    assert np.array_equal(a, want)
+    with hg.Terrain(n, device=0) as t2:
+        t2.load_height(path)
+        assert np.array_equal(t2.read(hg.CH_HEIGHT), want)
+    with hg.Terrain(32, device=0) as t3:
+        with pytest.raises(hg.HGError):
+            t3.load_height(path)                      # wrong size
+        with pytest.raises(hg.HGError):
+            t3.load_height(str(tmp_path / "missing.npy"))
