@@ -22,7 +22,7 @@ def _terrain(n, rs, time, max_h):
     return t
 
 
-@pytest.mark.parametrize("n,time", [(96, 37.25), (512, 1234.5678)])
+@pytest.mark.parametrize("n,time", [(96, 0.0), (96, 37.25), (512, 1234.5678)])
 def test_reference_rain_pass_matches_oracle(n, time):
     st = spun_up_state(n, 4, seed=3)
     max_h = 64.0
@@ -37,7 +37,7 @@ def test_reference_rain_pass_matches_oracle(n, time):
     ref32 = O.rain_reference(st.copy(), ors, time, max_h)[O.WATER]
     assert np.array_equal(got, ref32), f"{np.count_nonzero(got != ref32)} cells differ from the FP32 oracle"
     assert (got > st[hg.P_WATER]).mean() > 0.2                # it rained on a good share of the map
-    if time < 100:                                            # small noise coordinates: FP64 agrees too
+    if time == 0.0:         # seed fract(0) * 1000 = 0: small noise coordinates, FP64 agrees too
         ref64 = O.rain_reference(st.astype(np.float64), ors, time, max_h)[O.WATER]
         assert np.abs(got - ref64).max() <= 1e-6 * max(1.0, np.abs(ref64).max())
 
