@@ -81,17 +81,19 @@ def batched_workload(seed: int = 2026, count: int = 64):
     return sides, coeffs
 
 
-def bench_batched(steps: int, base_steps: int, tiles=None) -> dict:
+def bench_batched(steps: int, base_steps: int) -> dict:
     sides, coeffs = batched_workload()
     cells = float(sum(n * n for n in sides))
     b = hg.Batch(sides)
-    if tiles:
-        b.set_tiles(*tiles)
     b.set_rain(hg.RAIN_UNIFORM, amount=0.02, seed=7)
     for i, n in enumerate(sides):
         b.set_erosion(hg.erosion_settings(**coeffs[i]), terrain=i)
         b.init_fbm(i, hg.fbm_params(seed=1000 + i, base_freq=1.0 / 256, hi=float(n) / 8))
     out = {"terrains": len(sides), "cells": int(cells), "sides_min": min(sides), "sides_max": max(sides)}
+    # tile shapes for the batched kernels, timed on the batch itself (state restored)
+    hv, tv = b.autotune(4)
+    out["tile_variants"] = {"hydro": hv, "thermal": tv}
+    log(f"batched tiles: hydro {hv}, thermal {tv}")
     # baselines: per-terrain launches, single-terrain fused kernels / one kernel per pass
     for key, flags, k in (("per_terrain_fused", hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN, base_steps),
                           ("per_terrain_per_pass",
@@ -345,7 +347,7 @@ def main() -> int:
     if cpu:
         log(f"CPU baseline: {cpu['value']:.3e} cell-steps/s")
     t.close()
-    batched = bench_batched(args.batched_steps, 10, tiles) if args.batched_steps > 0 else None
+    batched = bench_batched(args.batched_steps, 10) if args.batched_steps > 0 else None
     if batched:
         log("batched terrains done")
     particles = (bench_particles(args.particle_iters, args.particle_oracle_iters)

@@ -42,6 +42,7 @@ struct hg_batch {
     int rain_mode = HG_RAIN_OFF;
     uint64_t seed = 0;
     int hydro_variant = 0, thermal_variant = 1;
+    size_t total_floats = 0;            // size of `block`
     bool persistent = false;            // false: one workgroup per work-list entry
     bool xcd_order = false;             // interleave the work-list for the 8 XCDs
     uint32_t* dev_tiles_h = nullptr;
@@ -318,6 +319,7 @@ int hg_batch_create(int device, int count, const uint32_t* sizes, hg_batch** out
     };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup("hipStreamCreate failed", HG_ERR_HIP);
+    b->total_floats = total;
     if (hipMalloc(&b->block, total * sizeof(float)) != hipSuccess)
         return cleanup("hipMalloc of the batch state failed", HG_ERR_OOM);
     if (hipMalloc(&b->dev_terr, sizeof(BatchTerrain) * count) != hipSuccess ||
@@ -418,6 +420,74 @@ int hg_batch_set_tiles(hg_batch* b, int hydro_variant, int thermal_variant) {
     if (thermal_variant >= 0) b->thermal_variant = thermal_variant;
     b->tiles_dirty = true;
     drop_graph(b);
+    return HG_OK;
+}
+
+int hg_batch_autotune(hg_batch* b, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best) {
+    if (!b) return HG_ERR_INVALID;
+    if (flags & ~(HG_STEP_RAIN | HG_STEP_THERMAL | HG_STEP_SMOOTH))
+        return bfail(b, HG_ERR_INVALID, "hg_batch_autotune: only rain/thermal/smooth flags");
+    if (steps == 0) steps = 4;
+    Guard g(b->device);
+    int rc = upload_terr(b);
+    if (rc != HG_OK) return rc;
+    // the trials advance the state: back up both buffers of every plane, the
+    // buffer selection and the device step counter
+    void* backup = nullptr;
+    BH(b, hipMalloc(&backup, b->total_floats * sizeof(float) + 256));
+    struct Free {
+        void* p;
+        ~Free() { (void)hipFree(p); }
+    } free_backup{backup};
+    char* bk = static_cast<char*>(backup);
+    BH(b, hipMemcpyAsync(bk, b->block, b->total_floats * sizeof(float), hipMemcpyDeviceToDevice, b->stream));
+    BH(b, hipMemcpyAsync(bk + b->total_floats * sizeof(float), b->dev_step, 256, hipMemcpyDeviceToDevice,
+                         b->stream));
+    const uint32_t cur0 = b->cur;
+    const uint64_t step0 = b->step;
+    const int h0 = b->hydro_variant, t0 = b->thermal_variant;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BH(b, hipEventCreate(&e0));
+    BH(b, hipEventCreate(&e1));
+    int pick[2] = {h0, t0};
+    for (int kind = 0; kind < 2 && rc == HG_OK; ++kind) {
+        const int nv = kind == 0 ? hgk::hydro_variant_count() : hgk::thermal_variant_count();
+        float best = 1e30f;
+        for (int v = 0; v < nv && rc == HG_OK; ++v) {
+            if (kind == 0) b->hydro_variant = v; else b->thermal_variant = v;
+            b->tiles_dirty = true;
+            rc = build_tiles(b);
+            if (rc == HG_OK) rc = enqueue_steps(b, 1, flags);            // warm-up
+            if (rc == HG_OK && hipEventRecord(e0, b->stream) != hipSuccess) rc = bfail(b, HG_ERR_HIP, "event");
+            if (rc == HG_OK) rc = enqueue_steps(b, steps, flags);
+            if (rc == HG_OK && (hipEventRecord(e1, b->stream) != hipSuccess ||
+                                hipEventSynchronize(e1) != hipSuccess))
+                rc = bfail(b, HG_ERR_HIP, "event");
+            float ms = 0.0f;
+            if (rc == HG_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+                best = ms;
+                pick[kind] = v;
+            }
+        }
+        if (kind == 0) b->hydro_variant = pick[0]; else b->thermal_variant = pick[1];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipMemcpyAsync(b->block, bk, b->total_floats * sizeof(float), hipMemcpyDeviceToDevice, b->stream);
+    (void)hipMemcpyAsync(b->dev_step, bk + b->total_floats * sizeof(float), 256, hipMemcpyDeviceToDevice,
+                         b->stream);
+    (void)hipStreamSynchronize(b->stream);
+    b->cur = cur0;
+    b->step = step0;
+    b->tiles_dirty = true;
+    drop_graph(b);
+    if (rc != HG_OK) {
+        b->hydro_variant = h0;
+        b->thermal_variant = t0;
+        return rc;
+    }
+    if (hydro_best) *hydro_best = pick[0];
+    if (thermal_best) *thermal_best = pick[1];
     return HG_OK;
 }
 

@@ -39,7 +39,7 @@ EXPORTS = [
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
     "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
-    "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles",
+    "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles", "hg_batch_autotune",
     "hg_batch_set_persistent", "hg_batch_set_xcd_order",
     "hg_batch_step", "hg_batch_capture", "hg_batch_run", "hg_batch_sync", "hg_batch_get_step",
     "hg_batch_read_plane", "hg_batch_write_plane",
@@ -234,6 +234,7 @@ def load() -> ctypes.CDLL:
     L.hg_batch_set_rain.argtypes = [vp, ctypes.POINTER(RainSettings), ctypes.c_int, ctypes.c_uint64]
     L.hg_batch_init_fbm.argtypes = [vp, ctypes.c_int, ctypes.POINTER(FbmParams)]
     L.hg_batch_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hg_batch_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ip, ip]
     L.hg_batch_step.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.hg_batch_set_persistent.argtypes = [vp, ctypes.c_int]
     L.hg_batch_set_xcd_order.argtypes = [vp, ctypes.c_int]

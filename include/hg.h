@@ -372,6 +372,10 @@ int hg_batch_set_erosion(hg_batch* b, int terrain, const hg_erosion_settings* s)
 int hg_batch_set_rain(hg_batch* b, const hg_rain_settings* s, int mode, uint64_t seed);
 int hg_batch_init_fbm(hg_batch* b, int terrain, const hg_fbm_params* p);
 int hg_batch_set_tiles(hg_batch* b, int hydro_variant, int thermal_variant);
+/* Times `steps` batched steps per tile variant (state, buffers and step
+ * counter are restored afterwards) and selects the fastest of each kernel;
+ * drops a captured graph. */
+int hg_batch_autotune(hg_batch* b, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best);
 /* 0 (default): each batched launch has one workgroup per tile of the work-list;
  * 1: a persistent grid of resident workgroups strides over the work-list. */
 int hg_batch_set_persistent(hg_batch* b, int persistent);

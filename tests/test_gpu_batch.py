@@ -73,6 +73,22 @@ def test_batched_tile_variants_agree(states):
                 assert all(np.array_equal(st[p], ref[i][p]) for p in range(hg.NUM_STATE_PLANES)), (hv, tv, i)
 
 
+def test_batch_autotune_restores_state(states):
+    ref = run_mode(states, 3, hg.STEP_DEFAULT)
+    nh, nt = len(hg.tile_variants(hg.KERNEL_HYDRO)), len(hg.tile_variants(hg.KERNEL_THERMAL))
+    with make_batch(states) as b:
+        b.capture(2)                    # the tuner drops a captured graph
+        hv, tv = b.autotune(2)
+        assert 0 <= hv < nh and 0 <= tv < nt
+        assert b.step_count == 0
+        with pytest.raises(hg.HGError):
+            b.run()
+        b.step(3)
+        for i in range(len(SIZES)):
+            st = b.read_state(i)
+            assert all(np.array_equal(st[p], ref[i][p]) for p in range(hg.NUM_STATE_PLANES)), (hv, tv, i)
+
+
 @pytest.mark.parametrize("persistent", [False, True])
 def test_graph_replay_matches_stepping(states, persistent):
     eager = run_mode(states, 8, hg.STEP_DEFAULT)
