@@ -355,10 +355,30 @@ __device__ __forceinline__ float thermal_g_t(const float D[8], float h_layer, in
     return S * rcp(bk);
 }
 
+// thermal_g_t without the tape: the largest cross and diagonal differences by
+// max3 instead of a tracked compare-and-select.  Both forms give the same g:
+// max values are equal (no NaN arises from finite D), and when bk > 0 they
+// are > 0, so the sign of a zero maximum cannot reach the result.
 __device__ __forceinline__ float thermal_g(const float D[8], float h_layer, int layer,
                                            const KParams& p) {
-    ThermalTape tp;
-    return thermal_g_t(D, h_layer, layer, p, tp);
+    const float ta = p.tan_alpha[layer], td = p.thr_diag[layer];
+    float bk = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bk += D[k] > ta ? D[k] : 0.0f;
+#pragma unroll
+    for (int k = 4; k < 8; ++k) bk += D[k] > td ? D[k] : 0.0f;
+    const float mc = fmaxf(fmaxf(D[0], D[1]), fmaxf(D[2], D[3]));
+    const float md = fmaxf(fmaxf(D[4], D[5]), fmaxf(D[6], D[7]));
+    const float H = fminf(h_layer, fmaxf(fmaxf(mc, md), 0.0f));
+    const float rmax = fmaxf(mc, md * 0.70710678118654752f);
+    float sharp = fmaxf(1.0f + atanf(rmax) - p.Kalpha[layer], 1.0f);
+    sharp = sharp * (sharp * sharp);
+    const float S = (p.dt * p.Kspeed[layer] * 0.5f) * sharp * H;
+    // Evaluated whatever bk is, with no branch: m = [bk > 0] gives
+    // (S*1)*rcp(bk+0) = S*rcp(bk) when bk > 0, and (S*0)*rcp(1) = 0 otherwise
+    // (S is finite for finite D).
+    const float m = bk > 0.0f ? 1.0f : 0.0f;
+    return (S * m) * rcp(bk + (1.0f - m));
 }
 
 // Height change of cell p in direction k (thermal_transport.glsl:146-172):

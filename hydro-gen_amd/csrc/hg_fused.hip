@@ -337,22 +337,60 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, 
 // ---------------------------------------------------------------------------
 constexpr int TH = 5;   // dependency radius of thermal L0 + L1 + smoothing
 
+// Cells of the region of radius RAD around the tile.  When the tile width is a
+// multiple of 64, the region is split into its core columns (the tile's own,
+// one wave per 64-wide row chunk: lanes read and write 64 consecutive LDS
+// words, free of bank conflicts, and global rows start 256-B aligned) and the
+// two side strips of RAD columns, spread over the threads.  Otherwise the
+// region is walked linearly.
 template <int TX, int TY, int NT, int RAD>
 struct Region {
-    static constexpr int W = TX + 2 * RAD, N = W * (TY + 2 * RAD);
-    static constexpr int ITER = (N + NT - 1) / NT;
+    static constexpr int W = TX + 2 * RAD, H = TY + 2 * RAD, N = W * H;
+    static constexpr bool ROWS = TX % 64 == 0 && NT % 64 == 0;
+    static constexpr int NW = NT / 64, CPR = TX / 64;       // waves; 64-wide chunks per row
+    static constexpr int NCORE = H * CPR, NSIDE = 2 * RAD * H;
+    static constexpr int ITC = ROWS ? (NCORE + NW - 1) / NW : 0;
+    static constexpr int ITS = ROWS ? (NSIDE + NT - 1) / NT : 0;
+    static constexpr int ITER = ROWS ? ITC + ITS : (N + NT - 1) / NT;
 };
 
-// Calls fn(it, lx, ly) for the cells of the region of radius RAD (LDS coords).
+// Calls fn(it, lx, ly) for the cells of the region of radius RAD (LDS coords);
+// `it` < Region::ITER numbers a thread's calls.
 template <int TX, int TY, int NT, int RAD, class F>
 __device__ __forceinline__ void region_for(int tid, F&& fn) {
     using R = Region<TX, TY, NT, RAD>;
+    if constexpr (R::ROWS) {
+        const int lane = tid & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 #pragma unroll
-    for (int it = 0; it < R::ITER; ++it) {
-        const int i = tid + it * NT;
-        if (R::N % NT == 0 || i < R::N) {
-            const int ry = i / R::W, rx = i - ry * R::W;
-            fn(it, rx + (TH - RAD), ry + (TH - RAD));
+        for (int it = 0; it < R::ITC; ++it) {
+            // The scalar guard stays even where every wave passes it: the
+            // branch bounds how far the scheduler hoists LDS reads across
+            // cells, which otherwise spills registers.
+            const int q = wave + it * R::NW;              // wave-uniform
+            if (q < R::NCORE) {
+                const int row = q / R::CPR, c = q - row * R::CPR;
+                fn(it, TH + c * 64 + lane, TH - RAD + row);
+            }
+        }
+        if constexpr (RAD > 0) {
+#pragma unroll
+            for (int it = 0; it < R::ITS; ++it) {
+                const int j = tid + it * NT;
+                if (R::NSIDE % NT == 0 || j < R::NSIDE) {
+                    const int row = j / (2 * RAD), c = j - row * (2 * RAD);
+                    fn(R::ITC + it, c < RAD ? TH - RAD + c : TH + TX - RAD + c, TH - RAD + row);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < R::ITER; ++it) {
+            const int i = tid + it * NT;
+            if (R::N % NT == 0 || i < R::N) {
+                const int ry = i / R::W, rx = i - ry * R::W;
+                fn(it, rx + (TH - RAD), ry + (TH - RAD));
+            }
         }
     }
 }
@@ -385,34 +423,36 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
         return gx >= 0 && gy >= 0 && gx < nx && gy < ny;
     };
 
-    // load rock, dirt; out of grid reads HG_OOB_HEIGHT (thermal_erosion.glsl:20-26)
-    for (int i = tid; i < N0; i += NT) {
-        const int ly = i / W0, lx = i - ly * W0;
-        const int gx = x0 + lx, gy = y0 + ly;
-        float r = HG_OOB_HEIGHT, d = HG_OOB_HEIGHT;
-        if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
-            const size_t g = at(gx, gy, nx);
-            r = io.r[g];
-            d = io.d[g];
-        }
-        s_r[i] = r;
-        s_d[i] = d;
+    // load rock, dirt; out of grid reads HG_OOB_HEIGHT (thermal_erosion.glsl:20-26).
+    // Loads use clamped addresses and are all issued before the first LDS write.
+    {
+        constexpr int IL = Region<TX, TY, NT, TH>::ITER;
+        float lr[IL], ld[IL];
+        region_for<TX, TY, NT, TH>(tid, [&](int it, int lx, int ly) {
+            const int gx = x0 + lx, gy = y0 + ly;
+            const size_t g = at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx);
+            const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+            lr[it] = in ? io.r[g] : HG_OOB_HEIGHT;
+            ld[it] = in ? io.d[g] : HG_OOB_HEIGHT;
+        });
+        region_for<TX, TY, NT, TH>(tid, [&](int it, int lx, int ly) {
+            s_r[ly * W0 + lx] = lr[it];
+            s_d[ly * W0 + lx] = ld[it];
+        });
     }
     __syncthreads();
 
     if (do_thermal) {
         // ---- A: layer-0 outflow scale
+        // (evaluated out of the grid too, on the out-of-grid height, and dropped)
         region_for<TX, TY, NT, 4>(tid, [&](int, int lx, int ly) {
             const int li = ly * W0 + lx;
-            float g = 0.0f;
-            if (in_grid(lx, ly)) {
-                const float r = s_r[li];
-                float D[8];
+            const float r = s_r[li];
+            float D[8];
 #pragma unroll
-                for (int kk = 0; kk < 8; ++kk) D[kk] = r - s_r[li + OFF[kk]];
-                g = thermal_g(D, r, 0, k);
-            }
-            s_g[li] = g;
+            for (int kk = 0; kk < 8; ++kk) D[kk] = r - s_r[li + OFF[kk]];
+            const float g = thermal_g(D, r, 0, k);
+            s_g[li] = in_grid(lx, ly) ? g : 0.0f;
         });
         __syncthreads();
         // ---- B: layer-0 transport, rock in place
@@ -439,15 +479,12 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
         // ---- C: layer-1 outflow scale (rock after layer 0)
         region_for<TX, TY, NT, 2>(tid, [&](int, int lx, int ly) {
             const int li = ly * W0 + lx;
-            float g = 0.0f;
-            if (in_grid(lx, ly)) {
-                const float r = s_r[li], d = s_d[li];
-                float D[8];
+            const float r = s_r[li], d = s_d[li];
+            float D[8];
 #pragma unroll
-                for (int kk = 0; kk < 8; ++kk) D[kk] = (r - s_r[li + OFF[kk]]) + (d - s_d[li + OFF[kk]]);
-                g = thermal_g(D, d, 1, k);
-            }
-            s_g[li] = g;
+            for (int kk = 0; kk < 8; ++kk) D[kk] = (r - s_r[li + OFF[kk]]) + (d - s_d[li + OFF[kk]]);
+            const float g = thermal_g(D, d, 1, k);
+            s_g[li] = in_grid(lx, ly) ? g : 0.0f;
         });
         __syncthreads();
         // ---- D: layer-1 transport, dirt in place
@@ -476,19 +513,21 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
     // ---- E: smoothing on the tile, write back
     region_for<TX, TY, NT, 0>(tid, [&](int, int lx, int ly) {
         const int gx = x0 + lx, gy = y0 + ly;
-        if (gx >= nx || gy >= ny) return;
         const int li = ly * W0 + lx;
-        float r = s_r[li], d = s_d[li];
-        const size_t g = at(gx, gy, nx);
-        if (EXTRA && io.tr5) {
-            io.tr5[g] = r;
-            io.td6[g] = d;
-        }
+        const float r0 = s_r[li], d0 = s_d[li];
+        float r = r0, d = d0;
         if (do_smooth && gx > 0 && gy > 0 && gx < nx - 1 && gy < ny - 1)
             smooth_cell(r, d, s_r[li - 1], s_d[li - 1], s_r[li + 1], s_d[li + 1], s_r[li + W0],
                         s_d[li + W0], s_r[li - W0], s_d[li - W0], k.smooth_m);
-        io.ro[g] = r;
-        io.dout[g] = d;
+        if (gx < nx && gy < ny) {
+            const size_t g = at(gx, gy, nx);
+            if (EXTRA && io.tr5) {
+                io.tr5[g] = r0;
+                io.td6[g] = d0;
+            }
+            io.ro[g] = r;
+            io.dout[g] = d;
+        }
     });
 }
 

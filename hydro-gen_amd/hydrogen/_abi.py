@@ -505,6 +505,13 @@ class Batch:
     def set_tiles(self, hydro: int = -1, thermal: int = -1):
         self._check(self.lib.hg_batch_set_tiles(self.h, hydro, thermal))
 
+    def autotune(self, steps: int = 4, flags: int = STEP_DEFAULT):
+        """Select the fastest tile variants for this batch (hg_batch_autotune);
+        the state and step counter are restored and a captured graph is dropped."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.hg_batch_autotune(self.h, steps, flags, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def step(self, nsteps: int = 1, flags: int = STEP_DEFAULT):
         self._check(self.lib.hg_batch_step(self.h, nsteps, flags))
 
