@@ -15,7 +15,8 @@
 //  * thermal outflow S*D_k/bk is formed as (S/bk)*D_k, so a cell's outflows
 //    are described by one scalar g = S/bk and its height differences;
 //  * mix(a,b,t) = a*(1-t)+b*t is evaluated as a + t*(b-a);
-//  * divisions by a per-cell value use the hardware reciprocal (1 ulp), and
+//  * divisions by a per-cell value use the hardware reciprocal (1 ulp), square
+//    roots the hardware square root (1 ulp), and
 //    the thermal height change is accumulated per direction,
 //    h += sum_k (in_k - out_k), instead of -sum(out) + sum(in).
 #pragma once
@@ -131,6 +132,9 @@ __host__ __device__ inline float u24(uint32_t w) { return float(w >> 8) * (1.0f 
 
 // Hardware reciprocal (v_rcp_f32, 1 ulp); rcp(0) = +inf.
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// Square root of the erosion pass: the hardware v_sqrt_f32 (1 ulp), as GLSL's
+// sqrt is not correctly rounded either.
+__device__ __forceinline__ float hsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // ---------------------------------------------------------------------------
 // hydro_flux (glsl/hydro_flux.glsl:77-166) for one cell.
@@ -220,7 +224,7 @@ struct ErosionTape {
 __device__ __forceinline__ void erosion_cell_t(float& r, float& d, float& s0, float& s1, float vx,
                                                float vy, float vz, float hl, float hr, float ht,
                                                float hb, const KParams& k, ErosionTape& tp) {
-    float ero = sqrtf(vx * vx + vy * vy);                                              // :45-52
+    float ero = hsqrt(vx * vx + vy * vy);                                              // :45-52
     tp.ero0 = ero;
     tp.slow = vz < 1e-3f;
     tp.sm = 0.0f;
@@ -237,7 +241,7 @@ __device__ __forceinline__ void erosion_cell_t(float& r, float& d, float& s0, fl
     }
     const float dx = hr - hl, dz = ht - hb;                                            // :23-35
     const float g2 = dx * dx + dz * dz;
-    const float sin_a = sqrtf(g2 * rcp(g2 + 4.0f));                                    // :57
+    const float sin_a = hsqrt(g2 * rcp(g2 + 4.0f));                                    // :57
     tp.dx = dx; tp.dz = dz; tp.g2 = g2; tp.sin_a = sin_a;
     const float cap0 = k.Kc * fmaxf(0.02f, sin_a) * ero;
     tp.cap0 = cap0;

@@ -72,8 +72,69 @@ __device__ void eroded_sediment_global(const HydroIO& io, int qx, int qy, int nx
 }
 
 // ---------------------------------------------------------------------------
+// Cells of the region of radius RAD around a TX x TY tile.  When the tile
+// width is a multiple of 64, the region is split into its core columns (the
+// tile's own: one wave per 64-wide row chunk, so lanes touch 64 consecutive
+// LDS words, free of bank conflicts, and global rows start 256-B aligned) and
+// the two side strips of RAD columns, spread over the threads.  Otherwise the
+// region is walked linearly.
+// ---------------------------------------------------------------------------
+template <int TX, int TY, int NT, int RAD>
+struct Region {
+    static constexpr int W = TX + 2 * RAD, H = TY + 2 * RAD, N = W * H;
+    static constexpr bool ROWS = TX % 64 == 0 && NT % 64 == 0;
+    static constexpr int NW = NT / 64, CPR = TX / 64;       // waves; 64-wide chunks per row
+    static constexpr int NCORE = H * CPR, NSIDE = 2 * RAD * H;
+    static constexpr int ITC = ROWS ? (NCORE + NW - 1) / NW : 0;
+    static constexpr int ITS = ROWS ? (NSIDE + NT - 1) / NT : 0;
+    static constexpr int ITER = ROWS ? ITC + ITS : (N + NT - 1) / NT;
+};
+
+// Calls fn(it, rx, ry) for the cells of the region, in region coordinates
+// rx in [0, W), ry in [0, H); `it` < Region::ITER numbers a thread's calls.
+template <int TX, int TY, int NT, int RAD, class F>
+__device__ __forceinline__ void tile_region_for(int tid, F&& fn) {
+    using R = Region<TX, TY, NT, RAD>;
+    if constexpr (R::ROWS) {
+        const int lane = tid & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+        for (int it = 0; it < R::ITC; ++it) {
+            // The scalar guard stays even where every wave passes it: the
+            // branch bounds how far the scheduler hoists LDS reads across
+            // cells, which otherwise spills registers.
+            const int q = wave + it * R::NW;              // wave-uniform
+            if (q < R::NCORE) {
+                const int row = q / R::CPR, c = q - row * R::CPR;
+                fn(it, RAD + c * 64 + lane, row);
+            }
+        }
+        if constexpr (RAD > 0) {
+#pragma unroll
+            for (int it = 0; it < R::ITS; ++it) {
+                const int j = tid + it * NT;
+                if (R::NSIDE % NT == 0 || j < R::NSIDE) {
+                    const int row = j / (2 * RAD), c = j - row * (2 * RAD);
+                    fn(R::ITC + it, c < RAD ? c : TX + c, row);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < R::ITER; ++it) {
+            const int i = tid + it * NT;
+            if (R::N % NT == 0 || i < R::N) {
+                const int ry = i / R::W, rx = i - ry * R::W;
+                fn(it, rx, ry);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // hydro kernel
 // ---------------------------------------------------------------------------
+
 template <int TX, int TY, int HA, int NT>
 struct HydroTile {
     // input region: tile +- (HA+1) cells, loaded in aligned quads, so the LDS
@@ -85,7 +146,6 @@ struct HydroTile {
     static constexpr int OY = HA + 1;                      // LDS y of tile row 0
     static constexpr int WE = TX + 2 * HA, HE = TY + 2 * HA;  // eroded-sediment region
     static constexpr int NE = WE * HE;
-    static constexpr int ITER_E = (NE + NT - 1) / NT;
     static constexpr int NTILE = TX * TY;
     static constexpr int ITER_T = (NTILE + NT - 1) / NT;
     static constexpr int NLDS = WI * HI;
@@ -178,15 +238,26 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
     __syncthreads();
 
     // ---- stage 1: flux + erosion on the tile +- HA region -----------------
-    float es0[T::ITER_E], es1[T::ITER_E], evx[T::ITER_E], evy[T::ITER_E];
+    // (region coordinates ex in [0, TX + 2HA), ey in [0, TY + 2HA))
+    constexpr int IE = Region<TX, TY, NT, HA>::ITER;
+    auto walk = [&](auto&& fn) { tile_region_for<TX, TY, NT, HA>(tid, fn); };
+    float es0[IE], es1[IE], evx[IE], evy[IE];
 #pragma unroll
-    for (int it = 0; it < T::ITER_E; ++it) {
-        es0[it] = 0.0f; es1[it] = 0.0f; evx[it] = 0.0f; evy[it] = 0.0f;
-        const int e = tid + it * NT;
-        if (e >= T::NE) continue;
-        const int ey = e / T::WE, ex = e - ey * T::WE;
+    for (int it = 0; it < IE; ++it) { es0[it] = 0.0f; es1[it] = 0.0f; evx[it] = 0.0f; evy[it] = 0.0f; }
+    // rock, dirt and sediment of the region, at clamped addresses, all
+    // issued before the first cell is evaluated
+    float gr[IE], gd[IE], g0[IE], g1[IE];
+    walk([&](int it, int ex, int ey) {
+        const int gx = min(max(x0 - HA + ex, 0), nx - 1), gy = min(max(y0 - HA + ey, 0), ny - 1);
+        const size_t g = at(gx, gy, nx);
+        gr[it] = io.r[g];
+        gd[it] = io.d[g];
+        g0[it] = io.s0[g];
+        g1[it] = io.s1[g];
+    });
+    walk([&](int it, int ex, int ey) {
         const int gx = x0 - HA + ex, gy = y0 - HA + ey;
-        if (gx < 0 || gy < 0 || gx >= nx || gy >= ny) continue;
+        if (gx < 0 || gy < 0 || gx >= nx || gy >= ny) return;
         const int li = (ey + 1) * T::WI + (ex + T::OX - HA);   // LDS index of this cell
         float f[4] = {s_fl[li], s_fr[li], s_ft[li], s_fb[li]};
         const float b = s_b[li];
@@ -199,8 +270,8 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
         float vx, vy, vz;
         const float d2 = flux_cell(f, b, w, wn, fin, wall, k, vx, vy, vz);
         const size_t g = at(gx, gy, nx);
-        float r = io.r[g], d = io.d[g];
-        float s0 = io.s0[g], s1 = io.s1[g];
+        float r = gr[it], d = gd[it];
+        float s0 = g0[it], s1 = g1[it];
         erosion_cell(r, d, s0, s1, vx, vy, vz, s_rd[li - 1], s_rd[li + 1], s_rd[li + T::WI],
                      s_rd[li - T::WI], k);
         es0[it] = s0; es1[it] = s1; evx[it] = vx; evy[it] = vy;
@@ -225,7 +296,7 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
                 io.ts1[g] = s1;
             }
         }
-    }
+    });
     __syncthreads();   // all reads of the staged inputs are done
 
     // eroded sediment of the region and velocity of the tile, aliased over LDS
@@ -233,19 +304,16 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
     float* s_s1 = lds + T::NE;
     float* s_vx = lds + 2 * T::NE;
     float* s_vy = lds + 2 * T::NE + T::NTILE;
-#pragma unroll
-    for (int it = 0; it < T::ITER_E; ++it) {
-        const int e = tid + it * NT;
-        if (e >= T::NE) continue;
+    walk([&](int it, int ex, int ey) {
+        const int e = ey * T::WE + ex;
         s_s0[e] = es0[it];
         s_s1[e] = es1[it];
-        const int ey = e / T::WE, ex = e - ey * T::WE;
         if (ex >= HA && ex < HA + TX && ey >= HA && ey < HA + TY) {
             const int t = (ey - HA) * TX + (ex - HA);
             s_vx[t] = evx[it];
             s_vy[t] = evy[it];
         }
-    }
+    });
     __syncthreads();
 
     // ---- stage 2: back-trace + bilinear gather on the tile -----------------
@@ -337,62 +405,11 @@ __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, 
 // ---------------------------------------------------------------------------
 constexpr int TH = 5;   // dependency radius of thermal L0 + L1 + smoothing
 
-// Cells of the region of radius RAD around the tile.  When the tile width is a
-// multiple of 64, the region is split into its core columns (the tile's own,
-// one wave per 64-wide row chunk: lanes read and write 64 consecutive LDS
-// words, free of bank conflicts, and global rows start 256-B aligned) and the
-// two side strips of RAD columns, spread over the threads.  Otherwise the
-// region is walked linearly.
-template <int TX, int TY, int NT, int RAD>
-struct Region {
-    static constexpr int W = TX + 2 * RAD, H = TY + 2 * RAD, N = W * H;
-    static constexpr bool ROWS = TX % 64 == 0 && NT % 64 == 0;
-    static constexpr int NW = NT / 64, CPR = TX / 64;       // waves; 64-wide chunks per row
-    static constexpr int NCORE = H * CPR, NSIDE = 2 * RAD * H;
-    static constexpr int ITC = ROWS ? (NCORE + NW - 1) / NW : 0;
-    static constexpr int ITS = ROWS ? (NSIDE + NT - 1) / NT : 0;
-    static constexpr int ITER = ROWS ? ITC + ITS : (N + NT - 1) / NT;
-};
-
-// Calls fn(it, lx, ly) for the cells of the region of radius RAD (LDS coords);
-// `it` < Region::ITER numbers a thread's calls.
+// Calls fn(it, lx, ly) for the cells of the region of radius RAD around the
+// thermal tile, in LDS coordinates.
 template <int TX, int TY, int NT, int RAD, class F>
 __device__ __forceinline__ void region_for(int tid, F&& fn) {
-    using R = Region<TX, TY, NT, RAD>;
-    if constexpr (R::ROWS) {
-        const int lane = tid & 63;
-        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-#pragma unroll
-        for (int it = 0; it < R::ITC; ++it) {
-            // The scalar guard stays even where every wave passes it: the
-            // branch bounds how far the scheduler hoists LDS reads across
-            // cells, which otherwise spills registers.
-            const int q = wave + it * R::NW;              // wave-uniform
-            if (q < R::NCORE) {
-                const int row = q / R::CPR, c = q - row * R::CPR;
-                fn(it, TH + c * 64 + lane, TH - RAD + row);
-            }
-        }
-        if constexpr (RAD > 0) {
-#pragma unroll
-            for (int it = 0; it < R::ITS; ++it) {
-                const int j = tid + it * NT;
-                if (R::NSIDE % NT == 0 || j < R::NSIDE) {
-                    const int row = j / (2 * RAD), c = j - row * (2 * RAD);
-                    fn(R::ITC + it, c < RAD ? TH - RAD + c : TH + TX - RAD + c, TH - RAD + row);
-                }
-            }
-        }
-    } else {
-#pragma unroll
-        for (int it = 0; it < R::ITER; ++it) {
-            const int i = tid + it * NT;
-            if (R::N % NT == 0 || i < R::N) {
-                const int ry = i / R::W, rx = i - ry * R::W;
-                fn(it, rx + (TH - RAD), ry + (TH - RAD));
-            }
-        }
-    }
+    tile_region_for<TX, TY, NT, RAD>(tid, [&](int it, int rx, int ry) { fn(it, rx + (TH - RAD), ry + (TH - RAD)); });
 }
 
 template <int TX, int TY>
