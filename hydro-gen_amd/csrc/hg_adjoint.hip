@@ -684,7 +684,7 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     const uint32_t nck = (nsteps + k_ckpt - 1) / k_ckpt;
     AdjWork w;
     w.pn = pn;
-    w.pitch = (pn * sizeof(float) + 255) / 256 * 256;
+    w.pitch = hgk::plane_stride_bytes(pn);
     const size_t n_pool = 64;
     const size_t nplanes = size_t(nck) * HG_NUM_STATE_PLANES + size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES +
                            size_t(k_ckpt) * NTAPE + n_pool + 2;

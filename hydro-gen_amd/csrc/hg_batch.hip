@@ -83,7 +83,7 @@ struct Guard {
     }
 };
 
-size_t pitch_floats(int n) { return (size_t(n) * n + 63) / 64 * 64; }
+size_t pitch_floats(int n) { return hgk::plane_stride_bytes(size_t(n) * n) / sizeof(float); }
 
 KParams kparams(const hg_erosion_settings& s, const hg_batch* b) {
     KParams k{};

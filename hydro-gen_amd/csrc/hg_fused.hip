@@ -643,7 +643,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     X(3,  64, 64, 512)         \
     X(4, 128, 32, 512)         \
     X(5,  32, 32, 256)         \
-    X(6, 128, 16, 512)
+    X(6, 128, 16, 512)         \
+    X(7,  64, 64, 1024)        \
+    X(8, 128, 32, 1024)
 
 }  // namespace
 

@@ -9,6 +9,14 @@
 
 namespace hgk {
 
+// Byte distance between consecutive FP32 planes of one allocation: the plane
+// rounded up to 256 B plus a 16,640-B stagger.  Planes of power-of-two size
+// placed back to back all start on the same HBM channel and bank, and a kernel
+// streaming 9 planes in and 9 out then reaches 5.0 TB/s at 4096^2; with the
+// stagger it reaches 5.5 TB/s.
+constexpr size_t kPlaneStagger = 16640;
+inline size_t plane_stride_bytes(size_t cells) { return (cells * sizeof(float) + 255) / 256 * 256 + kPlaneStagger; }
+
 // Device plane pointers, indexed by HG_P_*.
 struct Planes {
     float* p[HG_NUM_PLANES];

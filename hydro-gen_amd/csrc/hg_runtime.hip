@@ -244,7 +244,7 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
     c->device = device;
     c->n = int(n);
     const size_t pn = size_t(n) * n;
-    const size_t pitch = (pn * sizeof(float) + 255) / 256 * 256;
+    const size_t pitch = hgk::plane_stride_bytes(pn);
     const size_t bytes = pitch * 2 * HG_NUM_PLANES + 256;   // + init reduction, droplet counters
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
