@@ -74,25 +74,64 @@ __device__ void reduce_coef(double* gcoef, double a, double b, double c, double 
 // phase 1 computes every ring and tile cell's contributions into LDS, phase 2
 // sums, for each tile cell, its own part and what its neighbours send it, and
 // writes each output once.  No atomics and no zeroed buffers.
+//
+// Loads come first and together: the inputs a ring cell reads at its
+// neighbours are staged in LDS over the tile + 2-cell region (stage_walk),
+// and a `pre` pass loads each ring cell's own inputs into registers for all
+// of a thread's cells before any is evaluated.  Loads use clamped addresses;
+// values off the grid are replaced by the pass's fill after the load.
 // ---------------------------------------------------------------------------
 constexpr int GTX = 64, GTY = 8, GNT = 256;
 constexpr int GRW = GTX + 2, GRH = GTY + 2, GNR = GRW * GRH;
+constexpr int GIT = (GNR + GNT - 1) / GNT;          // ring/tile cells per thread
+constexpr int SRW = GTX + 4, SRH = GTY + 4, SRN = SRW * SRH;
+constexpr int SIT = (SRN + GNT - 1) / GNT;          // staged cells per thread
 
-// Phase 1 must not store to global memory: its loads, unrolled over the
-// ring/tile cells of a thread, are then free to be issued together.
-template <class Cell, class Gather>
-__device__ __forceinline__ void tile_gather(int nx, int ny, Cell&& cell, Gather&& gather) {
-    const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
-    constexpr int ITER = (GNR + GNT - 1) / GNT;
+// Staged-region index of ring cell e.
+__device__ __forceinline__ int sidx(int e) {
+    const int ey = e / GRW, ex = e - ey * GRW;
+    return (ey + 1) * SRW + ex + 1;
+}
+
+// fn(it, s, i, in) for the cells of the tile + 2-cell region: s its index in
+// the staged arrays, i the (clamped) global index, in whether it is on the grid.
+// Threads past the end redo the last cell (same values, same LDS slot).
+template <class F>
+__device__ __forceinline__ void stage_walk(int nx, int ny, F&& fn) {
+    const int x0 = blockIdx.x * GTX - 2, y0 = blockIdx.y * GTY - 2;
 #pragma unroll
-    for (int it = 0; it < ITER; ++it) {
+    for (int it = 0; it < SIT; ++it) {
+        const int s = min(int(threadIdx.x) + it * GNT, SRN - 1);
+        const int sy = s / SRW, sx = s - sy * SRW;
+        const int gx = x0 + sx, gy = y0 + sy;
+        const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+        fn(it, s, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx), in);
+    }
+}
+
+// pre(it, e, i): loads of ring cell e (i its clamped global index) for all of
+// a thread's cells; cell(it, e, px, py, in, tile): phase 1; gather(e, px, py):
+// phase 2 for the tile cells.
+template <class Pre, class Cell, class Gather>
+__device__ __forceinline__ void tile_gather(int nx, int ny, Pre&& pre, Cell&& cell, Gather&& gather) {
+    const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
+#pragma unroll
+    for (int it = 0; it < GIT; ++it) {
+        const int e = min(int(threadIdx.x) + it * GNT, GNR - 1);
+        const int ey = e / GRW, ex = e - ey * GRW;
+        const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
+        pre(it, e, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx));
+    }
+    __syncthreads();   // staged LDS regions are complete
+#pragma unroll
+    for (int it = 0; it < GIT; ++it) {
         const int e = threadIdx.x + it * GNT;
         if (GNR % GNT != 0 && e >= GNR) break;
         const int ey = e / GRW, ex = e - ey * GRW;
         const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
         const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
         const bool tile = ex >= 1 && ex <= GTX && ey >= 1 && ey <= GTY;
-        cell(e, gx, gy, in, tile);
+        cell(it, e, gx, gy, in, tile);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < GTX * GTY; t += GNT) {
@@ -110,36 +149,51 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
     // per ring/tile cell: outflow adj (4), inflow adj sent to neighbours (4),
     // total-height adj (own, 4 sent), water adj
     __shared__ float s_fo[4][GNR], s_fin[4][GNR], s_wn[4][GNR], s_wp[GNR], s_d1[GNR];
+    __shared__ float t_w[SRN], t_fl[SRN], t_fr[SRN], t_ft[SRN], t_fb[SRN];
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
     const float* B = x.p[HG_P_WATER];
-    auto total = [&](int xx, int yy) -> float {
-        const float t = ldc(R, xx, yy, nx, ny, 0.0f) + ldc(D, xx, yy, nx, ny, 0.0f) + ldc(B, xx, yy, nx, ny, 0.0f);
-        return (xx < 0 || yy < 0 || xx >= nx || yy >= ny) ? HG_OOB_HEIGHT : t;
-    };
+    {
+        // total height (out of grid: HG_OOB_HEIGHT) and old flux (0) of the region
+        float w[SIT], fl[SIT], fr[SIT], ft[SIT], fb[SIT];
+        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
+            const float t = (R[i] + D[i]) + B[i];
+            const float a = x.p[HG_P_FLUX_L][i], b = x.p[HG_P_FLUX_R][i], c = x.p[HG_P_FLUX_T][i],
+                        d = x.p[HG_P_FLUX_B][i];
+            w[it] = in ? t : HG_OOB_HEIGHT;
+            fl[it] = in ? a : 0.0f; fr[it] = in ? b : 0.0f; ft[it] = in ? c : 0.0f; fb[it] = in ? d : 0.0f;
+        });
+        stage_walk(nx, ny, [&](int it, int s, size_t, bool) {
+            t_w[s] = w[it]; t_fl[s] = fl[it]; t_fr[s] = fr[it]; t_ft[s] = ft[it]; t_fb[s] = fb[it];
+        });
+    }
+    float pb[GIT], plf[GIT][4], pld2[GIT], plv[GIT][3];
     tile_gather(nx, ny,
-        [&](int e, int px, int py, bool in, bool) {
+        [&](int it, int, size_t i) {
+            pb[it] = B[i];
+            plf[it][0] = lo.p[HG_P_FLUX_L][i]; plf[it][1] = lo.p[HG_P_FLUX_R][i];
+            plf[it][2] = lo.p[HG_P_FLUX_T][i]; plf[it][3] = lo.p[HG_P_FLUX_B][i];
+            pld2[it] = lo.p[HG_P_WATER][i];
+            plv[it][0] = lo.p[HG_P_VEL_U][i]; plv[it][1] = lo.p[HG_P_VEL_V][i]; plv[it][2] = lo.p[HG_P_VEL_DEPTH][i];
+        },
+        [&](int it, int e, int px, int py, bool in, bool) {
             float lfo[4] = {0, 0, 0, 0}, lfin[4] = {0, 0, 0, 0}, lwn[4] = {0, 0, 0, 0};
             float ld1 = 0.0f, lwp = 0.0f;
             if (in) {
-                const size_t i = at(px, py, nx);
-                float f[4] = {x.p[HG_P_FLUX_L][i], x.p[HG_P_FLUX_R][i], x.p[HG_P_FLUX_T][i], x.p[HG_P_FLUX_B][i]};
-                const float w = R[i] + D[i] + B[i];
-                const float wn[4] = {total(px - 1, py), total(px + 1, py), total(px, py + 1), total(px, py - 1)};
-                const float fin[4] = {ldc(x.p[HG_P_FLUX_R], px - 1, py, nx, ny, 0.0f),
-                                      ldc(x.p[HG_P_FLUX_L], px + 1, py, nx, ny, 0.0f),
-                                      ldc(x.p[HG_P_FLUX_B], px, py + 1, nx, ny, 0.0f),
-                                      ldc(x.p[HG_P_FLUX_T], px, py - 1, nx, ny, 0.0f)};
+                const int sc = sidx(e);
+                float f[4] = {t_fl[sc], t_fr[sc], t_ft[sc], t_fb[sc]};
+                const float w = t_w[sc];
+                const float wn[4] = {t_w[sc - 1], t_w[sc + 1], t_w[sc + SRW], t_w[sc - SRW]};
+                const float fin[4] = {t_fr[sc - 1], t_fl[sc + 1], t_fb[sc + SRW], t_ft[sc - SRW]};
                 const unsigned wall = (px == 0 ? 1u : 0u) | (px == nx - 1 ? 2u : 0u) | (py == 0 ? 4u : 0u) |
                                       (py == ny - 1 ? 8u : 0u);
                 FluxTape tp;
                 float vx, vy, vz;
-                (void)flux_cell_t(f, B[i], w, wn, fin, wall, k, vx, vy, vz, tp);
-                const float lf[4] = {lo.p[HG_P_FLUX_L][i], lo.p[HG_P_FLUX_R][i], lo.p[HG_P_FLUX_T][i],
-                                     lo.p[HG_P_FLUX_B][i]};
-                float ld2 = lo.p[HG_P_WATER][i];
-                const float lvx = lo.p[HG_P_VEL_U][i], lvy = lo.p[HG_P_VEL_V][i];
-                float lvz = lo.p[HG_P_VEL_DEPTH][i];
+                (void)flux_cell_t(f, pb[it], w, wn, fin, wall, k, vx, vy, vz, tp);
+                const float* lf = plf[it];
+                float ld2 = pld2[it];
+                const float lvx = plv[it][0], lvy = plv[it][1];
+                float lvz = plv[it][2];
                 if (vz > 0.0f) {                              // v = n * inv, inv = 1 / vz
                     const float lnx = lvx * tp.inv, lny = lvy * tp.inv;
                     lvz -= (lvx * vx + lvy * vy) * tp.inv;
@@ -210,26 +264,41 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
                                                      double* gcoef) {
     __shared__ float s_dx[GNR], s_dz[GNR], s_r[GNR], s_d[GNR], s_s0[GNR], s_s1[GNR], s_vx[GNR], s_vy[GNR],
         s_vz[GNR];
+    __shared__ float t_r[SRN], t_d[SRN];
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
-    auto rd = [&](int xx, int yy) -> float {
-        return ldc(R, xx, yy, nx, ny, 0.0f) + ldc(D, xx, yy, nx, ny, 0.0f);
-    };
+    {
+        // rock and dirt of the region (0 out of the grid)
+        float a[SIT], b[SIT];
+        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
+            const float r = R[i], d = D[i];
+            a[it] = in ? r : 0.0f;
+            b[it] = in ? d : 0.0f;
+        });
+        stage_walk(nx, ny, [&](int it, int s, size_t, bool) { t_r[s] = a[it]; t_d[s] = b[it]; });
+    }
+    auto rd = [&](int sc) -> float { return t_r[sc] + t_d[sc]; };
+    float p0[GIT], p1[GIT], pvx[GIT], pvy[GIT], pvz[GIT], plr[GIT], pld[GIT], pl0[GIT], pl1[GIT];
     double gKc = 0, gKs = 0, gKd = 0;
     tile_gather(nx, ny,
-        [&](int e, int px, int py, bool in, bool tile) {
+        [&](int it, int, size_t i) {
+            p0[it] = x.p[HG_P_SED_ROCK][i]; p1[it] = x.p[HG_P_SED_DIRT][i];
+            pvx[it] = x.p[HG_P_VEL_U][i]; pvy[it] = x.p[HG_P_VEL_V][i]; pvz[it] = x.p[HG_P_VEL_DEPTH][i];
+            plr[it] = lo.p[HG_P_ROCK][i]; pld[it] = lo.p[HG_P_DIRT][i];
+            pl0[it] = lo.p[HG_P_SED_ROCK][i]; pl1[it] = lo.p[HG_P_SED_DIRT][i];
+        },
+        [&](int it, int e, int px, int py, bool in, bool tile) {
             float lr_in = 0.0f, ld_in = 0.0f, ldx = 0.0f, ldz = 0.0f;
             float ls0_in = 0.0f, ls1_in = 0.0f, lvx = 0.0f, lvy = 0.0f, lvz = 0.0f;
             if (in) {
-                const size_t i = at(px, py, nx);
-                const float r_in = R[i], d_in = D[i], s0_in = x.p[HG_P_SED_ROCK][i], s1_in = x.p[HG_P_SED_DIRT][i];
-                const float vx = x.p[HG_P_VEL_U][i], vy = x.p[HG_P_VEL_V][i], vz = x.p[HG_P_VEL_DEPTH][i];
+                const int sc = sidx(e);
+                const float r_in = t_r[sc], d_in = t_d[sc], s0_in = p0[it], s1_in = p1[it];
+                const float vx = pvx[it], vy = pvy[it], vz = pvz[it];
                 float r = r_in, d = d_in, s0 = s0_in, s1 = s1_in;
                 ErosionTape tp;
-                erosion_cell_t(r, d, s0, s1, vx, vy, vz, rd(px - 1, py), rd(px + 1, py), rd(px, py + 1),
-                               rd(px, py - 1), k, tp);
-                const float lr = lo.p[HG_P_ROCK][i], ld = lo.p[HG_P_DIRT][i];
-                const float ls0 = lo.p[HG_P_SED_ROCK][i], ls1 = lo.p[HG_P_SED_DIRT][i];
+                erosion_cell_t(r, d, s0, s1, vx, vy, vz, rd(sc - 1), rd(sc + 1), rd(sc + SRW), rd(sc - SRW), k, tp);
+                const float lr = plr[it], ld = pld[it];
+                const float ls0 = pl0[it], ls1 = pl1[it];
                 double cKc = 0, cKs = 0, cKd = 0;
                 // conversion s1 += s0 kc, s0 -= s0 kc
                 const float kc = k.Kconv * k.dt;
@@ -347,15 +416,17 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
     const float* VX = x.p[HG_P_VEL_U];
     const float* VY = x.p[HG_P_VEL_V];
     double gKe = 0;
+    float pvx[GIT], pvy[GIT], pl0[GIT], pl1[GIT];
     tile_gather(nx, ny,
-        [&](int e, int px, int py, bool in, bool) {
-            float vx = 0.0f, vy = 0.0f, l0 = 0.0f, l1 = 0.0f;
-            if (in) {
-                const size_t i = at(px, py, nx);
-                vx = VX[i]; vy = VY[i];
-                l0 = lo.p[HG_P_SED_ROCK][i]; l1 = lo.p[HG_P_SED_DIRT][i];
-            }
-            s_vx[e] = vx; s_vy[e] = vy; s_l0[e] = l0; s_l1[e] = l1;
+        [&](int it, int, size_t i) {
+            pvx[it] = VX[i]; pvy[it] = VY[i];
+            pl0[it] = lo.p[HG_P_SED_ROCK][i]; pl1[it] = lo.p[HG_P_SED_DIRT][i];
+        },
+        [&](int it, int e, int, int, bool in, bool) {
+            s_vx[e] = in ? pvx[it] : 0.0f;
+            s_vy[e] = in ? pvy[it] : 0.0f;
+            s_l0[e] = in ? pl0[it] : 0.0f;
+            s_l1[e] = in ? pl1[it] : 0.0f;
         },
         [&](int e, int px, int py) {
             const size_t i = at(px, py, nx);
@@ -433,8 +504,6 @@ __global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, 
     aadd(li.p[HG_P_SED_DIRT] + j11, l1 * t.w11);
 }
 
-__device__ const int TDX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
-__device__ const int TDY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
 
 // ---- thermal layer `layer`: thermal_erosion then thermal_transport --------
 // Forward: T_k = outflow(h_in); h_out(p) = h_in(p) + sum_k (T_opp(k)(p+k) - T_k(p)).
@@ -446,32 +515,47 @@ __device__ const int TDY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
 __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes li, int nx, int ny, int layer,
                                                      KParams k) {
     __shared__ float s_lD[8][GNR], s_self[GNR], s_hl[GNR];
+    __shared__ float t_r[SRN], t_d[SRN], t_lh[SRN];
+    constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+    constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
     const float* R = x.p[HG_P_ROCK];
     const float* Dt = x.p[HG_P_DIRT];
     const float* LH = lo.p[layer == 0 ? HG_P_ROCK : HG_P_DIRT];
+    {
+        // rock, dirt (out of grid: HG_OOB_HEIGHT) and the layer's adjoint (0)
+        float a[SIT], b[SIT], c[SIT];
+        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
+            const float r = R[i], d = Dt[i], l = LH[i];
+            a[it] = in ? r : HG_OOB_HEIGHT;
+            b[it] = in ? d : HG_OOB_HEIGHT;
+            c[it] = in ? l : 0.0f;
+        });
+        stage_walk(nx, ny, [&](int it, int s, size_t, bool) { t_r[s] = a[it]; t_d[s] = b[it]; t_lh[s] = c[it]; });
+    }
     tile_gather(nx, ny,
-        [&](int e, int px, int py, bool in, bool) {
+        [&](int, int, size_t) {},
+        [&](int, int e, int px, int py, bool in, bool) {
             float lD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             float lself = 0.0f, lh_layer = 0.0f;
             if (in) {
-                const size_t i = at(px, py, nx);
-                const float r = R[i], d = Dt[i];
+                const int sc = sidx(e);
+                const float r = t_r[sc], d = t_d[sc];
                 float Dk[8], lhn[8];
                 bool ok[8];
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
-                    const int xx = px + TDX[kk], yy = py + TDY[kk];
+                    const int xx = px + DX[kk], yy = py + DY[kk];
+                    const int o = sc + DX[kk] + DY[kk] * SRW;
                     ok[kk] = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
-                    const float rn = ldc(R, xx, yy, nx, ny, HG_OOB_HEIGHT);
-                    float v = r - rn;
-                    if (layer == 1) v = v + (d - ldc(Dt, xx, yy, nx, ny, HG_OOB_HEIGHT));
+                    float v = r - t_r[o];
+                    if (layer == 1) v = v + (d - t_d[o]);
                     Dk[kk] = v;
-                    lhn[kk] = ldc(LH, xx, yy, nx, ny, 0.0f);
+                    lhn[kk] = t_lh[o];
                 }
                 ThermalTape tp;
                 const float g = thermal_g_t(Dk, layer == 0 ? r : d, layer, k, tp);
                 if (tp.bk > 0.0f) {
-                    const float lh = LH[i];
+                    const float lh = t_lh[sc];
                     float lg = 0.0f;
 #pragma unroll
                     for (int kk = 0; kk < 8; ++kk) {
@@ -511,12 +595,13 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
             const size_t i = at(px, py, nx);
             float recv = 0.0f;                                // from q = p - dir_k: -dL/dD_k(q)
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) recv -= s_lD[kk][e - (TDX[kk] + TDY[kk] * GRW)];
+            for (int kk = 0; kk < 8; ++kk) recv -= s_lD[kk][e - (DX[kk] + DY[kk] * GRW)];
             const float shared = s_self[e] + recv;
+            const float lh = t_lh[sidx(e)];
             if (layer == 0) {
-                li.p[HG_P_ROCK][i] = LH[i] + shared + s_hl[e];
+                li.p[HG_P_ROCK][i] = lh + shared + s_hl[e];
             } else {
-                li.p[HG_P_DIRT][i] = LH[i] + shared + s_hl[e];
+                li.p[HG_P_DIRT][i] = lh + shared + s_hl[e];
                 li.p[HG_P_ROCK][i] += shared;
             }
         });
@@ -526,44 +611,56 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
 // y = x + m (mean5 - x) at an extremum.  Gather: p keeps its own share and
 // receives 0.2 m dL/dy(q) from each interior neighbour q that is an extremum.
 // Writes li's rock and dirt (fresh).
-__global__ __launch_bounds__(BX * BY) void k_adj_smooth(Planes x, Planes lo, Planes li, int nx, int ny,
-                                                        KParams k) {
-    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
-    if (px >= nx || py >= ny) return;
-    const size_t i = at(px, py, nx);
-    const float* R = x.p[HG_P_ROCK];
-    const float* D = x.p[HG_P_DIRT];
+__global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
+    // rock, dirt and their adjoints on the tile + 2-cell region; cells off the
+    // grid hold the clamped cell's values, which only feed extremum flags of
+    // non-interior cells, and those are not used
+    __shared__ float t_r[SRN], t_d[SRN], t_lr[SRN], t_ld[SRN];
+    {
+        float a[SIT], b[SIT], c[SIT], d[SIT];
+        stage_walk(nx, ny, [&](int it, int, size_t i, bool) {
+            a[it] = x.p[HG_P_ROCK][i]; b[it] = x.p[HG_P_DIRT][i];
+            c[it] = lo.p[HG_P_ROCK][i]; d[it] = lo.p[HG_P_DIRT][i];
+        });
+        stage_walk(nx, ny, [&](int it, int s, size_t, bool) {
+            t_r[s] = a[it]; t_d[s] = b[it]; t_lr[s] = c[it]; t_ld[s] = d[it];
+        });
+    }
+    __syncthreads();
     const float m = k.smooth_m;
     auto interior = [&](int xx, int yy) { return xx > 0 && yy > 0 && xx < nx - 1 && yy < ny - 1; };
-    auto flags = [&](size_t j, bool& er, bool& eg) {
-        smooth_flags(R[j], D[j], R[j - 1], D[j - 1], R[j + 1], D[j + 1], R[j + nx], D[j + nx], R[j - nx],
-                     D[j - nx], er, eg);
+    auto flags = [&](int sc, bool& er, bool& eg) {
+        smooth_flags(t_r[sc], t_d[sc], t_r[sc - 1], t_d[sc - 1], t_r[sc + 1], t_d[sc + 1], t_r[sc + SRW],
+                     t_d[sc + SRW], t_r[sc - SRW], t_d[sc - SRW], er, eg);
     };
-    const float lr = lo.p[HG_P_ROCK][i], ld = lo.p[HG_P_DIRT][i];
-    float ar = lr, ad = ld;
-    if (interior(px, py)) {
-        bool er, eg;
-        flags(i, er, eg);
-        if (er) ar = lr * (1.0f - 0.8f * m);
-        if (eg) ad = ld * (1.0f - 0.8f * m);
-    }
-    const int NXo[4] = {-1, 1, 0, 0}, NYo[4] = {0, 0, 1, -1};
+    const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
+    for (int t = threadIdx.x; t < GTX * GTY; t += GNT) {
+        const int ty = t / GTX, tx = t - ty * GTX;
+        const int px = x0 + tx, py = y0 + ty;
+        if (px >= nx || py >= ny) continue;
+        const int sp = (ty + 2) * SRW + tx + 2;
+        const float lr = t_lr[sp], ld = t_ld[sp];
+        float ar = lr, ad = ld;
+        if (interior(px, py)) {
+            bool er, eg;
+            flags(sp, er, eg);
+            if (er) ar = lr * (1.0f - 0.8f * m);
+            if (eg) ad = ld * (1.0f - 0.8f * m);
+        }
+        constexpr int NXo[4] = {-1, 1, 0, 0}, NYo[4] = {0, 0, 1, -1};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        // flags of the neighbour, evaluated at a clamped interior cell so the
-        // loads are unconditional; used only when the neighbour is interior
-        const int qx = px + NXo[j], qy = py + NYo[j];
-        const int cx = min(max(qx, 1), nx - 2), cy = min(max(qy, 1), ny - 2);
-        const size_t q = at(cx, cy, nx);
-        bool er, eg;
-        flags(q, er, eg);
-        const bool use = interior(qx, qy);
-        const float lrq = lo.p[HG_P_ROCK][q], ldq = lo.p[HG_P_DIRT][q];
-        if (use && er) ar += lrq * 0.2f * m;
-        if (use && eg) ad += ldq * 0.2f * m;
+        for (int j = 0; j < 4; ++j) {
+            const int sq = sp + NXo[j] + NYo[j] * SRW;
+            bool er, eg;
+            flags(sq, er, eg);
+            const bool use = interior(px + NXo[j], py + NYo[j]);
+            if (use && er) ar += t_lr[sq] * 0.2f * m;
+            if (use && eg) ad += t_ld[sq] * 0.2f * m;
+        }
+        const size_t i = at(px, py, nx);
+        li.p[HG_P_ROCK][i] = ar;
+        li.p[HG_P_DIRT][i] = ad;
     }
-    li.p[HG_P_ROCK][i] = ar;
-    li.p[HG_P_DIRT][i] = ad;
 }
 
 // Zeroes `count` planes of n floats (n a multiple of 4: n = side^2, side % 8 == 0).
@@ -824,7 +921,7 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                                                                      pass == HG_PASS_THERMAL_TRANSPORT_0 ? 0 : 1, c->kp);
                         break;
                     case HG_PASS_SMOOTH:
-                        k_adj_smooth<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp);
+                        k_adj_smooth<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
                     default: break;
                 }
