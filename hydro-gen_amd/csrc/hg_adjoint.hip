@@ -52,8 +52,11 @@ __device__ __forceinline__ void aadd(float* p, float v) {
     if (v != 0.0f) unsafeAtomicAdd(p, v);
 }
 
-// Block reduction of up to 4 doubles, then one atomic add per value.
-__device__ void reduce_coef(double* gcoef, double a, double b, double c, double d) {
+// Block reduction of up to 4 doubles, added by thread 0 to the block's own 4
+// slots of `gpart` (one slot set per workgroup of the launch, so no atomics:
+// same-address atomics from every workgroup serialise in one L2 channel).
+// The host sums the slots in workgroup order at the end of the rollout.
+__device__ void reduce_coef(double* gpart, double a, double b, double c, double d) {
     __shared__ double sh[4][BX * BY];
     const int t = threadIdx.y * BX + threadIdx.x;
     sh[0][t] = a; sh[1][t] = b; sh[2][t] = c; sh[3][t] = d;
@@ -63,9 +66,11 @@ __device__ void reduce_coef(double* gcoef, double a, double b, double c, double 
             for (int j = 0; j < 4; ++j) sh[j][t] += sh[j][t + s];
         __syncthreads();
     }
-    if (t == 0)
+    if (t == 0) {
+        double* q = gpart + 4 * (size_t(blockIdx.y) * gridDim.x + blockIdx.x);
         for (int j = 0; j < 4; ++j)
-            if (sh[j][0] != 0.0) unsafeAtomicAdd(gcoef + j, sh[j][0]);
+            if (sh[j][0] != 0.0) q[j] += sh[j][0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -261,7 +266,7 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
 // ---- hydro_erosion -------------------------------------------------------
 // Writes li's rock, dirt and sediment (fresh) and adds to li's velocity.
 __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
-                                                     double* gcoef) {
+                                                     double* gpart) {
     __shared__ float s_dx[GNR], s_dz[GNR], s_r[GNR], s_d[GNR], s_s0[GNR], s_s1[GNR], s_vx[GNR], s_vy[GNR],
         s_vz[GNR];
     __shared__ float t_r[SRN], t_d[SRN];
@@ -385,7 +390,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
             li.p[HG_P_VEL_V][i] += s_vy[e];
             li.p[HG_P_VEL_DEPTH][i] += s_vz[e];
         });
-    reduce_coef(gcoef, gKc, gKs, gKd, 0.0);
+    reduce_coef(gpart, gKc, gKs, gKd, 0.0);
 }
 
 // ---- sediment_transport --------------------------------------------------
@@ -411,7 +416,7 @@ __device__ __forceinline__ Taps taps_of(int qx, int qy, float vx, float vy, int 
 }
 
 __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
-                                                       double* gcoef) {
+                                                       double* gpart) {
     __shared__ float s_vx[GNR], s_vy[GNR], s_l0[GNR], s_l1[GNR];
     const float* VX = x.p[HG_P_VEL_U];
     const float* VY = x.p[HG_P_VEL_V];
@@ -481,7 +486,7 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
             li.p[HG_P_WATER][i] = lb * k.evap;                   // b' = b (1 - Ke dt)
             gKe += double(lb) * double(x.p[HG_P_WATER][i]) * double(-k.dt);
         });
-    reduce_coef(gcoef, 0.0, 0.0, 0.0, gKe);
+    reduce_coef(gpart, 0.0, 0.0, 0.0, gKe);
 }
 
 __global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, Planes li, int nx, int ny,
@@ -706,7 +711,7 @@ struct AdjWork {
     std::vector<float*> tape;      // k_ckpt x NTAPE planes: each step's intermediates
     std::vector<float*> pool;      // adjoint planes (free list)
     float* zero = nullptr;
-    double* gcoef = nullptr;
+    double* gpart = nullptr;       // 4 coefficient partial sums per gather workgroup
 };
 
 // Intermediates of one step recorded by the fused kernels.
@@ -798,9 +803,11 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     for (size_t i = 0; i < size_t(k_ckpt) * NTAPE; ++i) w.tape.push_back(plane());
     for (size_t i = 0; i < n_pool; ++i) w.pool.push_back(plane());
     w.zero = plane();
-    w.gcoef = reinterpret_cast<double*>(base + used * w.pitch);
+    // one slot set per gather workgroup: 32 B per 64 x 8 cells, well within a plane
+    const size_t nblk = size_t((c->n + hgk::GTX - 1) / hgk::GTX) * size_t((c->n + hgk::GTY - 1) / hgk::GTY);
+    w.gpart = reinterpret_cast<double*>(base + used * w.pitch);
     HG_HIP(c, hipMemsetAsync(w.zero, 0, pn * sizeof(float), c->stream));
-    HG_HIP(c, hipMemsetAsync(w.gcoef, 0, 4 * sizeof(double), c->stream));
+    HG_HIP(c, hipMemsetAsync(w.gpart, 0, nblk * 4 * sizeof(double), c->stream));
     const uint64_t step0 = c->step;
 
     // ---- forward with checkpoints every k_ckpt steps
@@ -908,10 +915,10 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         k_adj_flux<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
                     case HG_PASS_EROSION:
-                        k_adj_erosion<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
+                        k_adj_erosion<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gpart);
                         break;
                     case HG_PASS_TRANSPORT:
-                        k_adj_transport<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gcoef);
+                        k_adj_transport<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gpart);
                         HG_HIP(c, hipGetLastError());
                         k_adj_transport_far<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
@@ -935,7 +942,13 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     }
     c->step = step_end;
     HG_HIP(c, hipMemcpyAsync(dL_dH0, lam[HG_P_ROCK], pn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HG_HIP(c, hipMemcpyAsync(dL_dcoef, w.gcoef, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> part(nblk * 4);
+    HG_HIP(c, hipMemcpyAsync(part.data(), w.gpart, nblk * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HG_HIP(c, hipStreamSynchronize(c->stream));
+    for (int j = 0; j < 4; ++j) {
+        double sum = 0.0;
+        for (size_t b = 0; b < nblk; ++b) sum += part[b * 4 + j];
+        dL_dcoef[j] = sum;
+    }
     return HG_OK;
 }
