@@ -56,12 +56,13 @@ __device__ __forceinline__ void aadd(float* p, float v) {
 // slots of `gpart` (one slot set per workgroup of the launch, so no atomics:
 // same-address atomics from every workgroup serialise in one L2 channel).
 // The host sums the slots in workgroup order at the end of the rollout.
+template <int NT>
 __device__ void reduce_coef(double* gpart, double a, double b, double c, double d) {
-    __shared__ double sh[4][BX * BY];
-    const int t = threadIdx.y * BX + threadIdx.x;
+    __shared__ double sh[4][NT];
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
     sh[0][t] = a; sh[1][t] = b; sh[2][t] = c; sh[3][t] = d;
     __syncthreads();
-    for (int s = BX * BY / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if (t < s)
             for (int j = 0; j < 4; ++j) sh[j][t] += sh[j][t + s];
         __syncthreads();
@@ -86,7 +87,7 @@ __device__ void reduce_coef(double* gpart, double a, double b, double c, double 
 // of a thread's cells before any is evaluated.  Loads use clamped addresses;
 // values off the grid are replaced by the pass's fill after the load.
 // ---------------------------------------------------------------------------
-constexpr int GTX = 64, GTY = 8, GNT = 256;
+constexpr int GTX = 64, GTY = 8, GNT = 256;   // 64 x 16 / 512 threads measured slower at 1024^2
 constexpr int GRW = GTX + 2, GRH = GTY + 2, GNR = GRW * GRH;
 constexpr int GIT = (GNR + GNT - 1) / GNT;          // ring/tile cells per thread
 constexpr int SRW = GTX + 4, SRH = GTY + 4, SRN = SRW * SRH;
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
             li.p[HG_P_VEL_V][i] += s_vy[e];
             li.p[HG_P_VEL_DEPTH][i] += s_vz[e];
         });
-    reduce_coef(gpart, gKc, gKs, gKd, 0.0);
+    reduce_coef<GNT>(gpart, gKc, gKs, gKd, 0.0);
 }
 
 // ---- sediment_transport --------------------------------------------------
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
             li.p[HG_P_WATER][i] = lb * k.evap;                   // b' = b (1 - Ke dt)
             gKe += double(lb) * double(x.p[HG_P_WATER][i]) * double(-k.dt);
         });
-    reduce_coef(gpart, 0.0, 0.0, 0.0, gKe);
+    reduce_coef<GNT>(gpart, 0.0, 0.0, 0.0, gKe);
 }
 
 __global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, Planes li, int nx, int ny,
@@ -707,8 +708,12 @@ struct AdjWork {
     size_t pn = 0, pitch = 0;
     void* block = nullptr;
     std::vector<float*> ckpt;      // (#checkpoints) x 9 planes
-    std::vector<float*> seg;       // (k_ckpt + 1) x 9 planes: each step's start state in a segment
-    std::vector<float*> tape;      // k_ckpt x NTAPE planes: each step's intermediates
+    // Two sets, so that one segment is recomputed while the previous one is
+    // reversed: (k_ckpt + 1) x 9 planes, each step's start state in a segment
+    // (slot 0 is the checkpoint itself), and k_ckpt x NTAPE planes, each
+    // step's intermediates.
+    std::vector<float*> seg[2];
+    std::vector<float*> tape[2];
     std::vector<float*> pool;      // adjoint planes (free list)
     float* zero = nullptr;
     double* gpart = nullptr;       // 4 coefficient partial sums per gather workgroup
@@ -727,7 +732,7 @@ float* take(std::vector<float*>& pool) {
 // intermediates in `tp`: hydro writes rock/dirt after erosion to tp[T_R3/T_D3]
 // (the thermal kernel then produces out's rock/dirt from them).
 int step_into(hg_ctx* c, float* const* in, float* const* out, float* const* tp, uint32_t step, bool rain,
-              bool thermal, bool smooth) {
+              bool thermal, bool smooth, hipStream_t st) {
     const bool th = thermal || smooth;
     hgk::HydroIO io;
     io.r = in[HG_P_ROCK]; io.d = in[HG_P_DIRT]; io.b = in[HG_P_WATER];
@@ -740,15 +745,14 @@ int step_into(hg_ctx* c, float* const* in, float* const* out, float* const* tp, 
     io.s0o = out[HG_P_SED_ROCK]; io.s1o = out[HG_P_SED_DIRT];
     io.vx = tp[T_VX]; io.vy = tp[T_VY]; io.vz = tp[T_VZ];
     io.tb1 = tp[T_B1]; io.tb2 = tp[T_B2]; io.ts0 = tp[T_S0]; io.ts1 = tp[T_S1];
-    HG_HIP(c, hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->xcd_order, c->kp,
-                                      c->stream));
+    HG_HIP(c, hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->xcd_order, c->kp, st));
     if (!th) return HG_OK;
     hgk::ThermIO tio;
     tio.r = tp[T_R3]; tio.d = tp[T_D3];
     tio.ro = out[HG_P_ROCK]; tio.dout = out[HG_P_DIRT];
     tio.tr5 = tp[T_R5]; tio.td6 = tp[T_D6];
     HG_HIP(c, hgk::launch_fused_thermal(c->thermal_variant, tio, c->n, c->n, thermal, smooth, c->xcd_order, c->kp,
-                                        c->stream));
+                                        st));
     return HG_OK;
 }
 
@@ -788,8 +792,8 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     w.pn = pn;
     w.pitch = hgk::plane_stride_bytes(pn);
     const size_t n_pool = 64;
-    const size_t nplanes = size_t(nck) * HG_NUM_STATE_PLANES + size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES +
-                           size_t(k_ckpt) * NTAPE + n_pool + 2;
+    const size_t nplanes = size_t(nck) * HG_NUM_STATE_PLANES +
+                           2 * (size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES + size_t(k_ckpt) * NTAPE) + n_pool + 2;
     HG_HIP(c, hipMalloc(&w.block, nplanes * w.pitch + 256));
     struct Free {
         void* p;
@@ -799,8 +803,10 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     size_t used = 0;
     auto plane = [&]() { float* p = reinterpret_cast<float*>(base + used * w.pitch); ++used; return p; };
     for (size_t i = 0; i < size_t(nck) * HG_NUM_STATE_PLANES; ++i) w.ckpt.push_back(plane());
-    for (size_t i = 0; i < size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES; ++i) w.seg.push_back(plane());
-    for (size_t i = 0; i < size_t(k_ckpt) * NTAPE; ++i) w.tape.push_back(plane());
+    for (int b = 0; b < 2; ++b) {
+        for (size_t i = 0; i < size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES; ++i) w.seg[b].push_back(plane());
+        for (size_t i = 0; i < size_t(k_ckpt) * NTAPE; ++i) w.tape[b].push_back(plane());
+    }
     for (size_t i = 0; i < n_pool; ++i) w.pool.push_back(plane());
     w.zero = plane();
     // one slot set per gather workgroup: 32 B per 64 x 8 cells, well within a plane
@@ -847,23 +853,65 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
 
     const dim3 blk(BX, BY), grd((n + BX - 1) / BX, (n + BY - 1) / BY);
     const dim3 gtile((n + GTX - 1) / GTX, (n + GTY - 1) / GTY);
-    for (int seg = int(nck) - 1; seg >= 0; --seg) {
+    // The taped recompute of a segment runs on a second stream (default
+    // priority) while the main stream reverses the segment after it: at 1024^2
+    // neither kernel sequence fills the chip alone.  Segment i (counting from
+    // the last) uses plane set i % 2; its recompute waits for the reversal that
+    // last used that set, and its reversal waits for its recompute.
+    hipStream_t rs = nullptr;
+    hipEvent_t ev_fwd = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
+    struct Streams {
+        hipStream_t& rs;
+        hipEvent_t &f, *r, *v;
+        ~Streams() {
+            if (rs) { (void)hipStreamSynchronize(rs); (void)hipStreamDestroy(rs); }
+            for (hipEvent_t e : {f, r[0], r[1], v[0], v[1]})
+                if (e) (void)hipEventDestroy(e);
+        }
+    } streams{rs, ev_fwd, ev_rec, ev_rev};
+    HG_HIP(c, hipStreamCreateWithFlags(&rs, hipStreamNonBlocking));
+    HG_HIP(c, hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming));
+    for (int b = 0; b < 2; ++b) {
+        HG_HIP(c, hipEventCreateWithFlags(&ev_rec[b], hipEventDisableTiming));
+        HG_HIP(c, hipEventCreateWithFlags(&ev_rev[b], hipEventDisableTiming));
+    }
+    HG_HIP(c, hipEventRecord(ev_fwd, c->stream));     // checkpoints and the zeroed adjoint are written
+    HG_HIP(c, hipStreamWaitEvent(rs, ev_fwd, 0));
+    // recompute segment `seg` from its checkpoint into plane set b, recording
+    // every step's start state and intermediates
+    auto recompute = [&](int seg, int b, bool wait_reversal) -> int {
         const uint32_t s_begin = uint32_t(seg) * k_ckpt;
         const uint32_t s_end = std::min(nsteps, s_begin + k_ckpt);
-        // recompute the segment from its checkpoint, recording every step's
-        // start state (slot 0 is the checkpoint itself) and intermediates
-        std::vector<float*>& slots = w.seg;
+        if (wait_reversal) HG_HIP(c, hipStreamWaitEvent(rs, ev_rev[b], 0));
+        std::vector<float*>& slots = w.seg[b];
         for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) slots[size_t(p)] = w.ckpt[size_t(seg) * HG_NUM_STATE_PLANES + p];
         for (uint32_t s = s_begin; s < s_end; ++s) {
             const size_t j = s - s_begin;
             const int rc = step_into(c, &slots[j * HG_NUM_STATE_PLANES], &slots[(j + 1) * HG_NUM_STATE_PLANES],
-                                     &w.tape[j * NTAPE], uint32_t(step0 + s), rain, thermal, smooth);
+                                     &w.tape[b][j * NTAPE], uint32_t(step0 + s), rain, thermal, smooth, rs);
             if (rc != HG_OK) return rc;
         }
+        HG_HIP(c, hipEventRecord(ev_rec[b], rs));
+        return HG_OK;
+    };
+    {
+        const int rc = recompute(int(nck) - 1, 0, false);
+        if (rc != HG_OK) return rc;
+    }
+    for (int seg = int(nck) - 1; seg >= 0; --seg) {
+        const int i = int(nck) - 1 - seg, b = i % 2;
+        const uint32_t s_begin = uint32_t(seg) * k_ckpt;
+        const uint32_t s_end = std::min(nsteps, s_begin + k_ckpt);
+        if (seg > 0) {
+            const int rc = recompute(seg - 1, 1 - b, i >= 1);
+            if (rc != HG_OK) return rc;
+        }
+        HG_HIP(c, hipStreamWaitEvent(c->stream, ev_rec[b], 0));
+        std::vector<float*>& slots = w.seg[b];
         // reverse the steps of the segment
         for (int s = int(s_end) - 1; s >= int(s_begin); --s) {
             float* const* in = &slots[size_t(s - int(s_begin)) * HG_NUM_STATE_PLANES];
-            float* const* tp = &w.tape[size_t(s - int(s_begin)) * NTAPE];
+            float* const* tp = &w.tape[b][size_t(s - int(s_begin)) * NTAPE];
             // pass inputs, from the step's start state and its tape
             auto x_of = [&](int pass) {
                 Planes X;
@@ -939,6 +987,7 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                 }
             }
         }
+        HG_HIP(c, hipEventRecord(ev_rev[b], c->stream));   // plane set b is free again
     }
     c->step = step_end;
     HG_HIP(c, hipMemcpyAsync(dL_dH0, lam[HG_P_ROCK], pn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
