@@ -131,6 +131,29 @@ def bench_batched(steps: int, base_steps: int) -> dict:
     return out
 
 
+def bench_reference_particles(iterations: int, n: int = 2048, count: int = 262144) -> dict:
+    """The reference's own particle mode (hg_step_particle_reference) with its
+    defaults (src/state.cpp:60-78, particle_count 262144 from the reference's
+    config default): n^2 fBm [0, 256], every dispatch = movement, erosion,
+    thermal erosion of both layers and smoothing with its particle branch."""
+    t = hg.Terrain(n)
+    t.init_fbm(hg.fbm_params(seed=3, base_freq=1.0 / 512, hi=256.0))
+    t.set_erosion(hg.reference_particle_settings(count))
+    t.ref_particles_init(count)
+    t.dispatch_particle_reference(5, True, 0.0, 1.0 / 60)
+    t.sync()
+    t0 = time.perf_counter()
+    t.dispatch_particle_reference(iterations, True, 5.0 / 60, 1.0 / 60)
+    t.sync()
+    dt = time.perf_counter() - t0
+    h = t.read_plane(hg.P_ROCK)
+    out = {"grid": n, "particles": count, "dispatches": iterations, "ms_per_dispatch": dt / iterations * 1e3,
+           "particle_steps_per_s": count * iterations / dt, "cas_giveups": t.ref_particle_stats(),
+           "state_finite": bool(np.isfinite(h).all())}
+    t.close()
+    return out
+
+
 def bench_particles(iterations: int, oracle_iters: int) -> dict:
     """Particle config: 2048^2 fBm (frequency 1/512) in [0, 256], 2^20 droplets
     per iteration, 64 lifetime steps, inertia 0.05, capacity 4, deposition and
@@ -352,6 +375,9 @@ def main() -> int:
         log("batched terrains done")
     particles = (bench_particles(args.particle_iters, args.particle_oracle_iters)
                  if args.particle_iters > 0 else None)
+    if particles is not None:
+        particles["reference_mode"] = bench_reference_particles(args.particle_iters)
+        log("reference particle mode done")
     if particles:
         log("droplets done")
     adjoint = bench_adjoint(steps=args.adjoint_steps) if args.adjoint_steps > 0 else None

@@ -48,6 +48,9 @@ struct hg_ctx {
     int xcd_order = 2;               // fused kernels' tile order (hg_set_xcd_order)
     float rain_time = 0.0f;          // HG_RAIN_REFERENCE time seed
     float map_max_height = 256.0f;   // HG_RAIN_REFERENCE mountain scale (src/settings.hpp:30)
+    hg_ref_particle* refp = nullptr; // the reference particle mode's particles (hg_ref_particles_init)
+    uint32_t refp_count = 0;
+    unsigned long long* dev_giveups = nullptr;  // in the block's tail
     std::string arch;
     std::string err;
     // timing

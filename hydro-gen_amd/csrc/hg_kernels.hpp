@@ -98,6 +98,20 @@ Brush make_brush(int r);
 hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
                            uint32_t iteration, double* exported, unsigned long long* steps, hipStream_t s);
 
+// the reference's particle mode (hg_refparticle.hip)
+struct RefParticleParams {
+    float Kc, Kconv, Ks[2], Kd[2], Ke, G, dt, init_volume, friction, inertia, min_volume, min_velocity;
+    uint32_t ttl, particle_count;
+};
+hipError_t launch_ref_particle_move(hg_ref_particle* parts, uint32_t count, const float* rock, const float* dirt,
+                                    const float* water, const float* mx, const float* my, int n,
+                                    const RefParticleParams& k, float time, int should_rain, hipStream_t s);
+hipError_t launch_ref_particle_erode(hg_ref_particle* parts, uint32_t count, float* rock, float* dirt, float* water,
+                                     float* mz, float* mw, int n, const RefParticleParams& k,
+                                     unsigned long long* giveups, hipStream_t s);
+hipError_t launch_ref_particle_smooth_extra(float* water, float* mx, float* my, float* mz, float* mw, int n,
+                                            uint32_t particle_count, hipStream_t s);
+
 // heightmap init (hg_init.hip)
 void fbm_permutation(uint64_t seed, uint8_t perm[256]);
 hipError_t launch_fbm(float* out, int nx, int ny, const uint8_t* perm_host, int octaves,

@@ -73,12 +73,14 @@ __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* 
                             h11 * u * v;
             dx = dx * p.inertia - gx * (1.0f - p.inertia);
             dy = dy * p.inertia - gy * (1.0f - p.inertia);
-            // IEEE round-to-nearest sqrt and divide (as on the host): a droplet's
-            // path is sensitive to the last bit of its direction
-            const float len = __fsqrt_rn(dx * dx + dy * dy);
+            // correctly rounded sqrt and divide (as on the host): a droplet's
+            // path is sensitive to the last bit of its direction.  Plain `/`
+            // and sqrtf lower to the IEEE sequences; HIP's __fsqrt_rn lowers to
+            // the native 3-ulp sqrt.
+            const float len = sqrtf(dx * dx + dy * dy);
             if (len > 0.0f) {
-                dx = __fdiv_rn(dx, len);
-                dy = __fdiv_rn(dy, len);
+                dx = dx / len;
+                dy = dy / len;
             }
             const float ox = px, oy = py;
             if (dx == 0.0f && dy == 0.0f) {       // not moving: drop the load here
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* 
                         if (x >= 0 && y >= 0 && x < n && y < n) wsum += brush.w[t];
                     }
                 }
-                const float scale = interior ? amount : __fdiv_rn(amount, wsum);
+                const float scale = interior ? amount : amount / wsum;
                 for (int t = 0; t < brush.count; ++t) {
                     const int x = cx + brush.dx[t], y = cy + brush.dy[t];
                     if (!interior && !(x >= 0 && y >= 0 && x < n && y < n)) continue;
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* 
                     sed += td + tr;
                 }
             }
-            speed = __fsqrt_rn(fmaxf(0.0f, speed * speed - dh * p.gravity));
+            speed = sqrtf(fmaxf(0.0f, speed * speed - dh * p.gravity));
             water = water * (1.0f - p.evaporation);
         }
         if (alive) deposit(D, n, px, py, sed);

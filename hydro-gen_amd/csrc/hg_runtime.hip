@@ -265,6 +265,7 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
     c->minmax = reinterpret_cast<unsigned*>(base + pitch * 2 * HG_NUM_PLANES);
     c->dev_exported = reinterpret_cast<double*>(base + pitch * 2 * HG_NUM_PLANES + 64);
     c->dev_dsteps = reinterpret_cast<unsigned long long*>(base + pitch * 2 * HG_NUM_PLANES + 128);
+    c->dev_giveups = reinterpret_cast<unsigned long long*>(base + pitch * 2 * HG_NUM_PLANES + 192);
     e = hipMemsetAsync(c->block, 0, bytes, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -305,6 +306,7 @@ int hg_destroy(hg_ctx* c) {
     for (auto& p : c->pending) { (void)hipEventDestroy(p.start); (void)hipEventDestroy(p.end); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipFree(c->block);
+    if (c->refp) (void)hipFree(c->refp);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HG_OK;
@@ -514,6 +516,112 @@ int hg_step_particle(hg_ctx* c, uint32_t iterations, uint32_t flags) {
         }
         c->particle_iter += 1;
     }
+    return HG_OK;
+}
+
+// ---- the reference's particle mode ---------------------------------------
+
+int hg_ref_particles_init(hg_ctx* c, uint32_t count) {
+    if (!c) return HG_ERR_INVALID;
+    if (count == 0 || count > (1u << 26)) return fail(c, HG_ERR_INVALID, "hg_ref_particles_init: count must be in [1, 2^26]");
+    DeviceGuard guard(c->device);
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->refp) {
+        (void)hipFree(c->refp);
+        c->refp = nullptr;
+        c->refp_count = 0;
+    }
+    HG_HIP(c, hipMalloc(&c->refp, size_t(count) * sizeof(hg_ref_particle)));
+    c->refp_count = count;
+    HG_HIP(c, hipMemsetAsync(c->refp, 0, size_t(count) * sizeof(hg_ref_particle), c->stream));
+    const size_t pn = plane_n(c);
+    for (int p = HG_P_FLUX_L; p <= HG_P_FLUX_B; ++p)
+        HG_HIP(c, hipMemsetAsync(c->cur[p], 0, pn * sizeof(float), c->stream));
+    HG_HIP(c, hipMemsetAsync(c->dev_giveups, 0, sizeof(unsigned long long), c->stream));
+    return HG_OK;
+}
+
+int hg_ref_particles_read(hg_ctx* c, hg_ref_particle* out, uint32_t count) {
+    if (!c || !out) return HG_ERR_INVALID;
+    if (count != c->refp_count) return fail(c, HG_ERR_INVALID, "hg_ref_particles_read: count differs from hg_ref_particles_init");
+    DeviceGuard guard(c->device);
+    HG_HIP(c, hipMemcpyAsync(out, c->refp, size_t(count) * sizeof(hg_ref_particle), hipMemcpyDeviceToHost, c->stream));
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    return HG_OK;
+}
+
+int hg_ref_particles_write(hg_ctx* c, const hg_ref_particle* in, uint32_t count) {
+    if (!c || !in) return HG_ERR_INVALID;
+    if (count != c->refp_count) return fail(c, HG_ERR_INVALID, "hg_ref_particles_write: count differs from hg_ref_particles_init");
+    DeviceGuard guard(c->device);
+    HG_HIP(c, hipMemcpyAsync(c->refp, in, size_t(count) * sizeof(hg_ref_particle), hipMemcpyHostToDevice, c->stream));
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    return HG_OK;
+}
+
+int hg_step_particle_reference(hg_ctx* c, uint32_t iterations, int should_rain, float time0, float time_step,
+                               uint32_t flags) {
+    if (!c) return HG_ERR_INVALID;
+    if (!c->refp) return fail(c, HG_ERR_STATE, "hg_step_particle_reference: call hg_ref_particles_init first");
+    if (flags & ~(HG_STEP_THERMAL | HG_STEP_SMOOTH))
+        return fail(c, HG_ERR_INVALID, "hg_step_particle_reference: only thermal/smooth flags");
+    if (!std::isfinite(time0) || !std::isfinite(time_step))
+        return fail(c, HG_ERR_INVALID, "hg_step_particle_reference: time must be finite");
+    if (c->n < 8) return fail(c, HG_ERR_INVALID, "hg_step_particle_reference: grid too small");
+    DeviceGuard guard(c->device);
+    const hg_erosion_settings& e = c->es;
+    hgk::RefParticleParams k{};
+    k.Kc = e.Kc; k.Kconv = e.Kconv; k.Ke = e.Ke; k.G = e.G; k.dt = e.d_t;
+    for (int l = 0; l < 2; ++l) { k.Ks[l] = e.Ks[l]; k.Kd[l] = e.Kd[l]; }
+    k.init_volume = e.init_volume; k.friction = e.friction; k.inertia = e.inertia;
+    k.min_volume = e.min_volume; k.min_velocity = e.min_velocity; k.ttl = e.ttl;
+    k.particle_count = e.particle_count;
+    const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
+    const int n = c->n;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        const float time = time0 + float(it) * time_step;
+        int rc = timed_launch(c, "ref_particle_move", [&] {
+            return hgk::launch_ref_particle_move(c->refp, c->refp_count, c->cur[HG_P_ROCK], c->cur[HG_P_DIRT],
+                                                 c->cur[HG_P_WATER], c->cur[HG_P_FLUX_L], c->cur[HG_P_FLUX_R], n, k,
+                                                 time, should_rain, c->stream);
+        });
+        if (rc != HG_OK) return rc;
+        rc = timed_launch(c, "ref_particle_erode", [&] {
+            return hgk::launch_ref_particle_erode(c->refp, c->refp_count, c->cur[HG_P_ROCK], c->cur[HG_P_DIRT],
+                                                  c->cur[HG_P_WATER], c->cur[HG_P_FLUX_T], c->cur[HG_P_FLUX_B], n, k,
+                                                  c->dev_giveups, c->stream);
+        });
+        if (rc != HG_OK) return rc;
+        if (thermal || smooth) {
+            hgk::ThermIO tio;
+            tio.r = c->cur[HG_P_ROCK]; tio.d = c->cur[HG_P_DIRT];
+            tio.ro = c->alt[HG_P_ROCK]; tio.dout = c->alt[HG_P_DIRT];
+            rc = timed_launch(c, "fused_thermal", [&] {
+                return hgk::launch_fused_thermal(c->thermal_variant, tio, n, n, thermal, smooth, c->xcd_order, c->kp,
+                                                 c->stream);
+            });
+            if (rc != HG_OK) return rc;
+            swap_planes(c, {HG_P_ROCK, HG_P_DIRT});
+        }
+        if (smooth) {
+            rc = timed_launch(c, "ref_particle_smooth_extra", [&] {
+                return hgk::launch_ref_particle_smooth_extra(c->cur[HG_P_WATER], c->cur[HG_P_FLUX_L],
+                                                             c->cur[HG_P_FLUX_R], c->cur[HG_P_FLUX_T],
+                                                             c->cur[HG_P_FLUX_B], n, k.particle_count, c->stream);
+            });
+            if (rc != HG_OK) return rc;
+        }
+    }
+    return HG_OK;
+}
+
+int hg_ref_particle_stats(hg_ctx* c, uint64_t* cas_giveups) {
+    if (!c || !cas_giveups) return HG_ERR_INVALID;
+    DeviceGuard guard(c->device);
+    unsigned long long g = 0;
+    HG_HIP(c, hipMemcpyAsync(&g, c->dev_giveups, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+    HG_HIP(c, hipStreamSynchronize(c->stream));
+    *cas_giveups = uint64_t(g);
     return HG_OK;
 }
 

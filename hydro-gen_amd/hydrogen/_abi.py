@@ -37,6 +37,8 @@ EXPORTS = [
     "hg_set_step", "hg_read", "hg_write", "hg_save_height", "hg_load_height", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
+    "hg_ref_particles_init", "hg_ref_particles_read", "hg_ref_particles_write", "hg_step_particle_reference",
+    "hg_ref_particle_stats",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
     "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
     "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles", "hg_batch_autotune",
@@ -141,6 +143,36 @@ def fbm_params(seed=1, octaves=6, lacunarity=2.0, gain=0.5, base_freq=1.0 / 256,
     return p
 
 
+class RefParticle(ctypes.Structure):
+    """hg_ref_particle: Particle of glsl/bindings.glsl:101-111, std430, 48 bytes."""
+    _fields_ = [("sc", ctypes.c_float), ("iters", ctypes.c_int32), ("position", ctypes.c_float * 2),
+                ("velocity", ctypes.c_float * 2), ("volume", ctypes.c_float), ("_pad0", ctypes.c_float),
+                ("sediment", ctypes.c_float * 2), ("to_kill", ctypes.c_uint32), ("_pad1", ctypes.c_uint32)]
+
+
+def reference_particle_settings(particle_count: int) -> "ErosionSettings":
+    """The reference's particle-mode Erosion_data defaults (src/state.cpp:60-78)."""
+    s = ErosionSettings()
+    s.particle_count = particle_count
+    s.Kc = 0.2
+    s.Kalpha[0] = 1.3; s.Kalpha[1] = 0.6
+    s.Kconv = 0.001
+    s.Ks[0] = 0.03; s.Ks[1] = 0.09
+    s.Kd[0] = 0.01; s.Kd[1] = 0.03
+    s.Ke = 0.03
+    s.Kspeed[0] = 0.002; s.Kspeed[1] = 0.008
+    s.G = 9.81
+    s.d_t = 0.25
+    s.density = 1.0
+    s.init_volume = 1.0
+    s.friction = 0.2
+    s.inertia = 1.0
+    s.min_volume = 0.0
+    s.min_velocity = 0.001
+    s.ttl = 15000
+    return s
+
+
 class DropletSettings(ctypes.Structure):
     """hg_droplet_settings (include/hg.h)."""
     _fields_ = [("count", ctypes.c_uint32), ("max_lifetime", ctypes.c_uint32),
@@ -217,6 +249,12 @@ def load() -> ctypes.CDLL:
                                      ctypes.POINTER(ctypes.c_double)]
     L.hg_step_particle.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.hg_particle_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
+    L.hg_ref_particles_init.argtypes = [vp, ctypes.c_uint32]
+    L.hg_ref_particles_read.argtypes = [vp, ctypes.POINTER(RefParticle), ctypes.c_uint32]
+    L.hg_ref_particles_write.argtypes = [vp, ctypes.POINTER(RefParticle), ctypes.c_uint32]
+    L.hg_step_particle_reference.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                             ctypes.c_uint32]
+    L.hg_ref_particle_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     ip = ctypes.POINTER(ctypes.c_int)
     L.hg_tile_variant_count.argtypes = [ctypes.c_int]
     L.hg_tile_variant_info.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
@@ -343,6 +381,31 @@ class Terrain:
     def dispatch_particle(self, iterations: int = 1, flags: int = 0):
         """Droplet iterations (Erosion::dispatch_particle counterpart)."""
         self._check(self.lib.hg_step_particle(self.h, iterations, flags))
+
+    # -- the reference's particle mode --
+    def ref_particles_init(self, count: int):
+        """`count` zeroed particles and a zeroed momentum map (hg_ref_particles_init)."""
+        self._check(self.lib.hg_ref_particles_init(self.h, count))
+        self._refp_count = count
+
+    def read_ref_particles(self):
+        arr = (RefParticle * self._refp_count)()
+        self._check(self.lib.hg_ref_particles_read(self.h, arr, self._refp_count))
+        return arr
+
+    def write_ref_particles(self, arr):
+        self._check(self.lib.hg_ref_particles_write(self.h, arr, len(arr)))
+
+    def dispatch_particle_reference(self, iterations: int = 1, should_rain: bool = True, time0: float = 0.0,
+                                    time_step: float = 0.0, flags: int = STEP_THERMAL | STEP_SMOOTH):
+        """Erosion::dispatch_particle of the reference (hg_step_particle_reference)."""
+        self._check(self.lib.hg_step_particle_reference(self.h, iterations, int(should_rain), time0, time_step,
+                                                         flags))
+
+    def ref_particle_stats(self) -> int:
+        g = ctypes.c_uint64()
+        self._check(self.lib.hg_ref_particle_stats(self.h, ctypes.byref(g)))
+        return g.value
 
     def particle_stats(self):
         ex, st = ctypes.c_double(), ctypes.c_uint64()

@@ -314,6 +314,46 @@ int hg_step_particle(hg_ctx* ctx, uint32_t iterations, uint32_t flags);
  * number of droplet steps simulated. */
 int hg_particle_stats(hg_ctx* ctx, double* exported_mass, uint64_t* droplet_steps);
 
+/* ---- the reference's particle mode ---------------------------------------- */
+
+/* Particle of glsl/bindings.glsl:101-111 in its std430 layout (48 bytes), the
+ * element of the reference's particle SSBO. */
+typedef struct hg_ref_particle {
+    float sc;                /*  0  sediment capacity */
+    int32_t iters;           /*  4  steps lived; 0 = not spawned */
+    float position[2];       /*  8 */
+    float velocity[2];       /* 16 */
+    float volume;            /* 24 */
+    float _pad0;             /* 28 */
+    float sediment[2];       /* 32  rock, dirt */
+    uint32_t to_kill;        /* 40  GLSL bool */
+    uint32_t _pad1;          /* 44 */
+} hg_ref_particle;
+
+/* Creates (or re-creates) `count` zeroed particles and zeroes the momentum
+ * map, as World::gen_textures does for particle mode (src/state.cpp:3-44).
+ * The momentum map (the reference's velocity texture in particle mode,
+ * rgba) is held in the four FLUX planes: L = x, R = y, T = z, B = w. */
+int hg_ref_particles_init(hg_ctx* ctx, uint32_t count);
+int hg_ref_particles_read(hg_ctx* ctx, hg_ref_particle* out, uint32_t count);
+int hg_ref_particles_write(hg_ctx* ctx, const hg_ref_particle* in, uint32_t count);
+/* `iterations` Erosion::dispatch_particle calls (src/erosion.cpp:132-156):
+ * particle movement (glsl/particle.glsl:64-136), particle erosion at the 4
+ * corners of each particle's quad (glsl/particle_erosion.glsl:22-128),
+ * thermal erosion of both layers (flags HG_STEP_THERMAL) and smoothing with
+ * its particle branch (HG_STEP_SMOOTH; glsl/smoothing.glsl:22-103).  Iteration
+ * j uses time = time0 + j * time_step for the shader's `time` uniform;
+ * should_rain as the dispatch's argument.  The erosion settings'
+ * particle_count, Kc, Ks, Kd, Kconv, Ke, G, d_t, init_volume, friction,
+ * inertia, min_volume, min_velocity and ttl are used as in the shaders.
+ * Corner updates of different particles at one pixel are atomic per layer
+ * (no per-pixel spinlock). */
+int hg_step_particle_reference(hg_ctx* ctx, uint32_t iterations, int should_rain, float time0,
+                               float time_step, uint32_t flags);
+/* Atomic height updates that gave up retrying and fell back to an unclamped
+ * add since hg_ref_particles_init (0 unless the contention is extreme). */
+int hg_ref_particle_stats(hg_ctx* ctx, uint64_t* cas_giveups);
+
 /* ---- adjoint -------------------------------------------------------------- */
 
 /* Gradient of a GRID rollout (no counterpart in the reference).  Runs nsteps

@@ -943,4 +943,231 @@ double droplets(std::vector<T>& rock, std::vector<T>& dirt, int n, const Droplet
     return exported;
 }
 
+// ---------------------------------------------------------------------------
+// The reference's own particle mode, one dispatch_particle
+// (src/erosion.cpp:132-156): movement (glsl/particle.glsl:64-136), erosion
+// (glsl/particle_erosion.glsl:22-128), thermal erosion of both layers, and
+// smoothing with its particle branch (glsl/smoothing.glsl:22-103).  Serial:
+// the movement of every particle, then the erosion of every particle in id
+// order (the shader's per-pixel spinlock serialises corner updates in some
+// order; id order is one of them).  WORLD_SCALE = 1, SED_LAYERS = 2.
+//
+// Spelled-out choices where GLSL leaves freedom: mix(a, b, t) = a + t (b - a)
+// (as in the grid passes); normalize(v) = v / length(v), length = IEEE sqrt;
+// texelFetch outside the texture reads 0; the `time` uniform is an explicit
+// argument.  The momentum map (rgba) is four planes mx, my, mz, mw.
+// ---------------------------------------------------------------------------
+template <class T> struct RefParticle {
+    T sc = T(0);
+    int32_t iters = 0;
+    T px = T(0), py = T(0), vx = T(0), vy = T(0), volume = T(0);
+    T sed[2] = {T(0), T(0)};
+    bool to_kill = false;
+};
+
+template <class T> struct RefParticleParams {
+    uint32_t particle_count = 0, ttl = 0;
+    T Kc, Kconv, Ks[2], Kd[2], Ke, G, dt, init_volume, friction, inertia, min_volume, min_velocity;
+};
+
+template <class T> inline T glsl_fract_t(T x) { return x - T(std::floor(val(x))); }
+
+// rand(vec2 p), glsl/particle.glsl:41-44
+template <class T> T ref_particle_rand(T x, T y) {
+    using std::sin; using std::fabs;
+    return glsl_fract_t(T(1e4) * sin(T(17.0) * x + y * T(0.1)) * (T(0.1) + fabs(sin(y * T(13.0) + x))));
+}
+
+// img_bilinear (glsl/img_interpolation.glsl:3-22) of one plane.
+template <class T> T ref_bilinear(const std::vector<T>& img, int n, T x, T y) {
+    const int ix = int(val(x)), iy = int(val(y));
+    const T fx = glsl_fract_t(x), fy = glsl_fract_t(y);
+    auto f = [&](int a, int b) -> T {
+        return (a >= 0 && b >= 0 && a < n && b < n) ? img[size_t(b) * n + a] : T(0);
+    };
+    const T v1 = f(ix, iy) + fx * (f(ix + 1, iy) - f(ix, iy));
+    const T v2 = f(ix, iy + 1) + fx * (f(ix + 1, iy + 1) - f(ix, iy + 1));
+    return v1 + fy * (v2 - v1);
+}
+
+// Movement of one particle (glsl/particle.glsl:64-136).  Returns false when
+// the shader returns without writing the particle back.
+template <class T>
+bool ref_particle_move(RefParticle<T>& pw, uint32_t id, const State<T>& s, const std::vector<T> mom[4],
+                       const RefParticleParams<T>& k, T time, bool should_rain) {
+    using std::sqrt;
+    const int n = s.nx;
+    RefParticle<T> p = pw;
+    if (p.iters == 0 && !should_rain) return false;
+    for (int i = 0; i < 2; ++i)
+        if (val(p.sed[i]) < 0 || p.iters == 0) p.sed[i] = T(0);
+    if (p.iters == 0 || p.to_kill) {
+        const T ax = glsl_fract_t(time * T(1.37)) * T(1000.0), ay = glsl_fract_t(time * T(7.21)) * T(1000.0);
+        p.px = ref_particle_rand(ax, T(double(id))) * T(double(n) - 4.0) / T(1.0) + T(2.0);
+        p.py = ref_particle_rand(ay, T(double(id)) + T(3.14)) * T(double(n) - 4.0) / T(1.0) + T(2.0);
+        p.to_kill = false;
+        p.vx = T(0); p.vy = T(0);
+        p.volume = k.init_volume;
+        if (should_rain) {
+            p.iters = 1;
+        } else {
+            return false;     // p.iters = 0; return (the particle is not written back)
+        }
+    }
+    // get_terr_normal (:51-63): rock + dirt sampled one cell away
+    auto rg = [&](T x, T y, T& r, T& g) { r = ref_bilinear(s.r, n, x, y); g = ref_bilinear(s.d, n, x, y); };
+    T rr, rgn, lr, lg, br, bg, tr, tg;
+    rg(p.px + T(1.0), p.py, rr, rgn);
+    rg(p.px - T(1.0), p.py, lr, lg);
+    rg(p.px, p.py - T(1.0), br, bg);
+    rg(p.px, p.py + T(1.0), tr, tg);
+    const T dx = rr + rgn - lr - lg;
+    const T dz = tr + tg - br - bg;
+    // normalize(cross((2, dx, 0), (0, dz, 2))) = normalize((2 dx, -4, 2 dz))
+    const T cx = dx * T(2.0) - T(0.0) * dz, cy = T(0.0) * T(0.0) - T(2.0) * T(2.0), cz = T(2.0) * dz - dx * T(0.0);
+    const T cl = sqrt(cx * cx + cy * cy + cz * cz);
+    const T nx = cx / cl, ny = cy / cl, nz = cz / cl;
+    const T mx = ref_bilinear(mom[0], n, p.px, p.py), my = ref_bilinear(mom[1], n, p.px, p.py);
+    const T water = ref_bilinear(s.b, n, p.px, p.py);
+    p.vx = p.vx - (k.dt * nx) / p.volume * k.G;                                   // :101
+    p.vy = p.vy - (k.dt * nz) / p.volume * k.G;
+    const T mlen = sqrt(mx * mx + my * my);
+    T vlen = sqrt(p.vx * p.vx + p.vy * p.vy);
+    if (val(mlen) > 0 && val(vlen) > 0) {                                         // :103-105
+        const T d = (mx / mlen) * (p.vx / vlen) + (my / mlen) * (p.vy / vlen);
+        const T t = k.inertia * d / (p.volume + T(1e5) * water);
+        p.vx = p.vx + t * mx;
+        p.vy = p.vy + t * my;
+    }
+    vlen = sqrt(p.vx * p.vx + p.vy * p.vy);
+    if (val(vlen) > 1.0) { p.vx = p.vx / vlen; p.vy = p.vy / vlen; }             // :108-110
+    const T ox = p.px, oy = p.py;
+    p.px = p.px + k.dt * p.vx;                                                    // :113
+    p.py = p.py + k.dt * p.vy;
+    if (val(p.px) <= 1 || val(p.py) <= 1 || val(p.px * T(1.0)) >= double(n - 2) ||
+        val(p.py * T(1.0)) >= double(n - 2)) {                                     // :114-124
+        p.px = ox; p.py = oy;
+        p.vx = T(0); p.vy = T(0);
+        p.to_kill = true;
+    }
+    const T fr = T(1.0) - k.dt * k.friction * ny;                                 // :125
+    p.vx = p.vx * fr;
+    p.vy = p.vy * fr;
+    p.volume = p.volume - k.dt * k.Ke;                                            // :126
+    const T sin_a = sqrt(T(1.0) - ny * ny);                                       // :129
+    vlen = sqrt(p.vx * p.vx + p.vy * p.vy);
+    p.sc = tmax(T(0.0), k.Kc * p.volume * vlen * tmax(T(0.02), sin_a));         // :131
+    p.iters += 1;
+    if (val(p.volume) <= val(k.min_volume) || val(vlen) < val(k.min_velocity) ||
+        uint32_t(p.iters) >= k.ttl)                                               // :134-139
+        p.to_kill = true;
+    pw = p;
+    return true;
+}
+
+// Erosion of one particle at the 4 corners of its quad, in the shader's
+// corner order (glsl/particle_erosion.glsl:102-128, erode_layers :22-75).
+template <class T>
+void ref_particle_erode(RefParticle<T>& part, State<T>& s, std::vector<T> mom[4], const RefParticleParams<T>& k) {
+    if (part.iters == 0) return;
+    const int n = s.nx;
+    const int bx = int(val(part.px * T(1.0))), by = int(val(part.py * T(1.0)));
+    const int qx[4] = {bx, bx + 1, bx + 1, bx}, qy[4] = {by, by, by + 1, by + 1};
+    const T fx = glsl_fract_t(part.px * T(1.0)), fy = glsl_fract_t(part.py * T(1.0));
+    const T ox[4] = {T(1.0) - fx, fx, fx, T(1.0) - fx}, oy[4] = {T(1.0) - fy, T(1.0) - fy, fy, fy};
+    const T old_sed[2] = {part.sed[0], part.sed[1]};
+    for (int c = 0; c < 4; ++c) {
+        if (qx[c] < 0 || qy[c] < 0 || qx[c] >= n || qy[c] >= n) continue;   // outside the image: no store
+        const size_t i = size_t(qy[c]) * n + qx[c];
+        T terr[2] = {s.r[i], s.d[i]};
+        const T multipl = ox[c] * oy[c];
+        T cap = T(0.0);
+        for (int l = 1; l >= 0; --l) {
+            if (part.to_kill) {
+                const T sed = old_sed[l] * multipl;
+                terr[l] = terr[l] + sed;
+                part.sed[l] = part.sed[l] - sed;
+                continue;
+            }
+            const T Kls = k.dt * k.Ks[l], Kld = k.dt * k.Kd[l];
+            const T cc = tmax(T(0.0), part.sc - cap);
+            T s1 = old_sed[l];
+            const T old_terr = terr[l];
+            if (val(cc) > val(s1)) {
+                const T eroded = multipl * Kls * (cc - s1);
+                s1 = s1 + eroded;
+                terr[l] = terr[l] - eroded;
+                if (val(terr[l]) < 0) {
+                    s1 = s1 + terr[l];
+                    terr[l] = T(0);
+                    cap = cap + old_terr;
+                } else {
+                    part.sed[l] = s1;
+                    break;
+                }
+            } else {
+                const T deposit = multipl * Kld * (s1 - cc);
+                s1 = s1 - deposit;
+                terr[l] = terr[l] + deposit;
+            }
+            part.sed[l] = s1;
+        }
+        const T conv = part.sed[0] * k.Kconv * k.dt;
+        part.sed[1] = part.sed[1] + conv;
+        part.sed[0] = part.sed[0] - conv;
+        s.r[i] = terr[0];
+        s.d[i] = terr[1];
+        s.b[i] = s.b[i] + T(1e-5) * part.volume * multipl;
+        mom[2][i] = mom[2][i] + part.volume * part.vx * multipl;
+        mom[3][i] = mom[3][i] + part.volume * part.vy * multipl;
+    }
+}
+
+// The particle branch of the smoothing pass (glsl/smoothing.glsl:76-95): the
+// momentum map and the water display, per interior cell.  Border cells return
+// before it (:29-34): their water is kept, and their momentum is never written
+// (erosion only touches corners of quads inside [1, n-2], so it stays 0).
+template <class T> void ref_particle_smooth_extra(State<T>& s, std::vector<T> mom[4], uint32_t particle_count) {
+    using std::sqrt;
+    const int n = s.nx;
+    const T cnt = T(double(float(particle_count)));
+    const T fm = tclamp(T(1.0) - (T(1e-12) * cnt), T(0.0), T(1.0));
+    const T fb = tclamp(T(1.0) - (T(8e-8) * cnt), T(0.0), T(1.0));
+    for (int y = 1; y < n - 1; ++y)
+        for (int x = 1; x < n - 1; ++x) {
+            const size_t i = size_t(y) * n + x;
+            T mx = mom[0][i] * fm, my = mom[1][i] * fm;
+            mx = mx + (T(1e-12) * cnt) * mom[2][i];
+            my = my + (T(1e-12) * cnt) * mom[3][i];
+            mom[2][i] = T(0); mom[3][i] = T(0);
+            T b = s.b[i] * fb;
+            if (val(b) < 1e-6) b = T(0);
+            if (val(sqrt(mx * mx + my * my)) < 1e-12) { mx = T(0); my = T(0); }
+            mom[0][i] = mx; mom[1][i] = my;
+            s.b[i] = b;
+        }
+}
+
+// One dispatch_particle (src/erosion.cpp:132-156): movement of every
+// particle, erosion of every particle (id order), thermal erosion of both
+// layers and smoothing (grid branch for rock/dirt, particle branch for water
+// and momentum).  `thermal` / `smooth` switch those passes off for tests.
+template <class T>
+void ref_particle_dispatch(State<T>& s, std::vector<T> mom[4], std::vector<RefParticle<T>>& parts,
+                           const RefParticleParams<T>& pk, const Settings<T>& k, T time, bool should_rain,
+                           bool thermal, bool smooth) {
+    for (size_t id = 0; id < parts.size(); ++id) ref_particle_move(parts[id], uint32_t(id), s, mom, pk, time, should_rain);
+    for (auto& p : parts) ref_particle_erode(p, s, mom, pk);
+    if (thermal) {
+        pass_thermal_flux(s, k, 0);
+        pass_thermal_transport(s, 0);
+        pass_thermal_flux(s, k, 1);
+        pass_thermal_transport(s, 1);
+    }
+    if (smooth) {
+        pass_smooth(s, k);
+        ref_particle_smooth_extra(s, mom, pk.particle_count);
+    }
+}
+
 }  // namespace orc

@@ -94,6 +94,53 @@ T rollout_loss(orc::State<T>& st, const hg_erosion_settings* s, const T coef[4],
 }
 
 constexpr int DUAL_N = 36;   // 4 coefficients + up to 32 cells
+
+template <class T> orc::RefParticleParams<T> ref_particle_params(const hg_erosion_settings* s) {
+    orc::RefParticleParams<T> k;
+    k.particle_count = s->particle_count;
+    k.ttl = s->ttl;
+    k.Kc = T(s->Kc); k.Kconv = T(s->Kconv); k.Ke = T(s->Ke); k.G = T(s->G); k.dt = T(s->d_t);
+    for (int l = 0; l < 2; ++l) { k.Ks[l] = T(s->Ks[l]); k.Kd[l] = T(s->Kd[l]); }
+    k.init_volume = T(s->init_volume); k.friction = T(s->friction); k.inertia = T(s->inertia);
+    k.min_volume = T(s->min_volume); k.min_velocity = T(s->min_velocity);
+    return k;
+}
+
+// `iterations` reference particle dispatches; planes as for orc_run (the
+// FLUX planes hold the momentum map x, y, z, w), particles in the std430
+// layout (converted to T for the run and rounded back to FP32 at the end).
+template <class T>
+int ref_particle_run(T* planes, int n, hg_ref_particle* parts, uint32_t count, const hg_erosion_settings* s,
+                     uint32_t iterations, int should_rain, float time0, float time_step, unsigned flags) {
+    if (!planes || !s || !parts || n < 8) return -1;
+    orc::State<T> st;
+    load(st, planes, n, n);
+    const orc::Settings<T> k = to_settings<T>(s);
+    const orc::RefParticleParams<T> pk = ref_particle_params<T>(s);
+    std::vector<orc::RefParticle<T>> ps(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const hg_ref_particle& q = parts[i];
+        auto& p = ps[i];
+        p.sc = T(q.sc); p.iters = q.iters; p.px = T(q.position[0]); p.py = T(q.position[1]);
+        p.vx = T(q.velocity[0]); p.vy = T(q.velocity[1]); p.volume = T(q.volume);
+        p.sed[0] = T(q.sediment[0]); p.sed[1] = T(q.sediment[1]); p.to_kill = q.to_kill != 0;
+    }
+    std::vector<T> mom[4] = {st.fl, st.fr, st.ft, st.fb};
+    for (uint32_t it = 0; it < iterations; ++it) {
+        const float time = time0 + float(it) * time_step;
+        orc::ref_particle_dispatch(st, mom, ps, pk, k, T(time), should_rain != 0, (flags & 2u) != 0, (flags & 4u) != 0);
+    }
+    st.fl = mom[0]; st.fr = mom[1]; st.ft = mom[2]; st.fb = mom[3];
+    for (uint32_t i = 0; i < count; ++i) {
+        const auto& p = ps[i];
+        hg_ref_particle& q = parts[i];
+        q.sc = float(p.sc); q.iters = p.iters; q.position[0] = float(p.px); q.position[1] = float(p.py);
+        q.velocity[0] = float(p.vx); q.velocity[1] = float(p.vy); q.volume = float(p.volume);
+        q.sediment[0] = float(p.sed[0]); q.sediment[1] = float(p.sed[1]); q.to_kill = p.to_kill ? 1u : 0u;
+    }
+    store(st, planes);
+    return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -292,4 +339,18 @@ int orc_rollout_grad_dual(const double* planes, int nx, int ny, const hg_erosion
     return 0;
 }
 
+
+// The reference's particle mode (orc::ref_particle_dispatch).
+int orc_ref_particle_run_f64(double* planes, int n, hg_ref_particle* parts, uint32_t count,
+                             const hg_erosion_settings* s, uint32_t iterations, int should_rain, float time0,
+                             float time_step, unsigned flags) {
+    return ref_particle_run<double>(planes, n, parts, count, s, iterations, should_rain, time0, time_step, flags);
+}
+int orc_ref_particle_run_f32(float* planes, int n, hg_ref_particle* parts, uint32_t count,
+                             const hg_erosion_settings* s, uint32_t iterations, int should_rain, float time0,
+                             float time_step, unsigned flags) {
+    return ref_particle_run<float>(planes, n, parts, count, s, iterations, should_rain, time0, time_step, flags);
+}
+float orc_ref_particle_rand_f32(float x, float y) { return orc::ref_particle_rand<float>(x, y); }
+double orc_ref_particle_rand_f64(double x, double y) { return orc::ref_particle_rand<double>(x, y); }
 }  // extern "C"

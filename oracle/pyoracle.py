@@ -119,6 +119,53 @@ def droplets(rock: np.ndarray, dirt: np.ndarray, s: DropletSettings, iteration: 
     return fn(_ptr(rock), _ptr(dirt), n, ctypes.byref(s), iteration)
 
 
+class RefParticle(ctypes.Structure):
+    """Particle (glsl/bindings.glsl:101-111), std430, 48 bytes."""
+    _fields_ = [("sc", ctypes.c_float), ("iters", ctypes.c_int32), ("position", ctypes.c_float * 2),
+                ("velocity", ctypes.c_float * 2), ("volume", ctypes.c_float), ("_pad0", ctypes.c_float),
+                ("sediment", ctypes.c_float * 2), ("to_kill", ctypes.c_uint32), ("_pad1", ctypes.c_uint32)]
+
+
+def reference_particle_defaults(particle_count: int) -> ErosionSettings:
+    """src/state.cpp:60-78 (particle mode)."""
+    s = ErosionSettings()
+    s.particle_count = particle_count
+    s.Kc = 0.2
+    s.Kalpha[0] = 1.3; s.Kalpha[1] = 0.6
+    s.Kconv = 0.001
+    s.Ks[0] = 0.03; s.Ks[1] = 0.09
+    s.Kd[0] = 0.01; s.Kd[1] = 0.03
+    s.Ke = 0.03
+    s.Kspeed[0] = 0.002; s.Kspeed[1] = 0.008
+    s.G = 9.81
+    s.d_t = 0.25
+    s.density = 1.0
+    s.init_volume = 1.0
+    s.friction = 0.2
+    s.inertia = 1.0
+    s.min_volume = 0.0
+    s.min_velocity = 0.001
+    s.ttl = 15000
+    return s
+
+
+def ref_particle_run(planes: np.ndarray, parts, s: ErosionSettings, iterations: int, should_rain: bool,
+                     time0: float, time_step: float, flags: int) -> None:
+    """`iterations` reference particle dispatches in place: planes (NUM_PLANES,
+    n, n) float64 or float32 (FLUX planes = momentum map x, y, z, w), parts a
+    ctypes array of RefParticle (rounded to FP32 after the run)."""
+    n = planes.shape[-1]
+    fn = lib().orc_ref_particle_run_f64 if planes.dtype == np.float64 else lib().orc_ref_particle_run_f32
+    rc = fn(_ptr(planes), n, parts, len(parts), ctypes.byref(s), iterations, int(should_rain), time0, time_step,
+            flags)
+    assert rc == 0
+
+
+def ref_particle_rand(x: float, y: float, dtype=np.float32) -> float:
+    """rand(vec2 p) of glsl/particle.glsl:41-44."""
+    return (lib().orc_ref_particle_rand_f32 if dtype == np.float32 else lib().orc_ref_particle_rand_f64)(x, y)
+
+
 def config_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0, slippage=0.1,
                     G=9.81, dt=0.005, Kconv=0.001, E=1.0) -> ErosionSettings:
     """The configs' scalar coefficients broadcast to both layers; the unnamed
@@ -183,6 +230,15 @@ def lib():
                                        ctypes.c_int, ctypes.POINTER(DropletSettings), ctypes.c_uint32]
         L.orc_droplets_f32.restype = ctypes.c_double
         dp = ctypes.POINTER(ctypes.c_double)
+        for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            getattr(L, f"orc_ref_particle_run_{name}").argtypes = [
+                ctypes.POINTER(t), ctypes.c_int, ctypes.POINTER(RefParticle), ctypes.c_uint32,
+                ctypes.POINTER(ErosionSettings), ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                ctypes.c_uint]
+        L.orc_ref_particle_rand_f32.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.orc_ref_particle_rand_f32.restype = ctypes.c_float
+        L.orc_ref_particle_rand_f64.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_ref_particle_rand_f64.restype = ctypes.c_double
         L.orc_rollout_loss_f64.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), dp,
                                            ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int,
                                            ctypes.c_uint, dp, dp]

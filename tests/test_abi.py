@@ -42,6 +42,16 @@ def test_erosion_settings_std140_layout():
     assert ctypes.sizeof(O.ErosionSettings) == 96
 
 
+def test_ref_particle_std430_layout():
+    """Particle (glsl/bindings.glsl:101-111) in std430: vec2 members 8-aligned."""
+    P = hg.RefParticle
+    assert ctypes.sizeof(P) == 48 and ctypes.sizeof(O.RefParticle) == 48
+    offs = {"sc": 0, "iters": 4, "position": 8, "velocity": 16, "volume": 24, "sediment": 32, "to_kill": 40}
+    for name, off in offs.items():
+        assert getattr(P, name).offset == off, name
+        assert getattr(O.RefParticle, name).offset == off, name
+
+
 def test_rain_settings_std140_layout():
     R = hg.RainSettings
     assert ctypes.sizeof(R) == 32
