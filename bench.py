@@ -223,6 +223,11 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     pert = {k: v * float(rng.uniform(0.8, 1.2)) for k, v in coef.items()}
     t = hg.Terrain(n)
     t.set_rain(hg.RAIN_CONSTANT, amount=0.01)
+    # the forward and the adjoint's recompute use the tile variants tuned for
+    # this grid (the state is re-initialised below)
+    t.set_erosion(hg.erosion_settings(**coef))
+    t.init_fbm(hg.fbm_params(seed=11, base_freq=1.0 / 256, hi=64.0))
+    tiles = t.autotune(10)
     # target terrain
     t.set_erosion(hg.erosion_settings(**pert))
     t.init_fbm(hg.fbm_params(seed=12, base_freq=1.0 / 256, hi=64.0))
@@ -244,7 +249,11 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     h = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
     loss = float(((h - target) ** 2).mean())
     g = (2.0 * (h - target) / h.size).astype(np.float32)
-    # forward + adjoint (the adjoint call runs its own checkpointed forward)
+    # forward + adjoint (the adjoint call runs its own checkpointed forward);
+    # one untimed call first, as the forward above is timed warm
+    t.write_state(init)
+    t.step = 0
+    t.adjoint_rollout(steps, k_ckpt, g)
     t.write_state(init)
     t.step = 0
     t.sync()
@@ -252,7 +261,7 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     gh, gc = t.adjoint_rollout(steps, k_ckpt, g)
     t_adj = time.perf_counter() - t0
     t.close()
-    return {"grid": n, "steps": steps, "k_ckpt": k_ckpt, "loss": loss,
+    return {"grid": n, "steps": steps, "k_ckpt": k_ckpt, "tile_variants": list(tiles), "loss": loss,
             "t_forward_ms": t_fwd * 1e3, "t_forward_adjoint_ms": t_adj * 1e3,
             "cost_ratio": t_adj / t_fwd, "grad_coef": [float(x) for x in gc],
             "grad_h0_absmax": float(np.abs(gh).max()), "grad_finite": bool(np.isfinite(gh).all())}

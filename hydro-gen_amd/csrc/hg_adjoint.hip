@@ -7,7 +7,8 @@
 //
 // Structure:
 //  * forward: the ordinary fused steps, saving the 9 state planes every
-//    k_ckpt steps;
+//    k_ckpt steps; the last two segments run through the taped kernels (next
+//    item) at once, so they are not recomputed;
 //  * reverse, one segment of k_ckpt steps at a time: recompute the segment
 //    from its checkpoint with the fused kernels, which also record each
 //    step's start state and its intermediates (water after rain and before
@@ -55,22 +56,32 @@ __device__ __forceinline__ void aadd(float* p, float v) {
 // Block reduction of up to 4 doubles, added by thread 0 to the block's own 4
 // slots of `gpart` (one slot set per workgroup of the launch, so no atomics:
 // same-address atomics from every workgroup serialise in one L2 channel).
-// The host sums the slots in workgroup order at the end of the rollout.
+// Each wave sums its lanes by a fixed butterfly, then thread 0 adds the wave
+// sums in wave order, so the result does not depend on scheduling.  The host
+// sums the slots in workgroup order at the end of the rollout.
 template <int NT>
 __device__ void reduce_coef(double* gpart, double a, double b, double c, double d) {
-    __shared__ double sh[4][NT];
+    static_assert(NT % 64 == 0, "whole waves");
+    constexpr int NW = NT / 64;
+    __shared__ double sh[NW][4];
+    double v[4] = {a, b, c, d};
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] += __shfl_xor(v[j], m, 64);
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
-    sh[0][t] = a; sh[1][t] = b; sh[2][t] = c; sh[3][t] = d;
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sh[t >> 6][j] = v[j];
     __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s)
-            for (int j = 0; j < 4; ++j) sh[j][t] += sh[j][t + s];
-        __syncthreads();
-    }
     if (t == 0) {
         double* q = gpart + 4 * (size_t(blockIdx.y) * gridDim.x + blockIdx.x);
-        for (int j = 0; j < 4; ++j)
-            if (sh[j][0] != 0.0) q[j] += sh[j][0];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double s = sh[0][j];
+            for (int w = 1; w < NW; ++w) s += sh[w][j];
+            if (s != 0.0) q[j] += s;
+        }
     }
 }
 
@@ -115,35 +126,63 @@ __device__ __forceinline__ void stage_walk(int nx, int ny, F&& fn) {
     }
 }
 
+// A thread's cells of the tile + ring, numbered j = threadIdx.x + it * GNT:
+// the GTX * GTY tile cells first, row by row (whole waves, GTIT iterations),
+// then the ring (bottom row, top row, then the left and right columns).
+constexpr int GTIT = GTX * GTY / GNT;
+static_assert(GTX * GTY % GNT == 0, "the tile fills whole iterations");
+static_assert(GIT == GTIT + 1 && GNR - GTX * GTY <= GNT, "the ring fits one iteration");
+__device__ __forceinline__ void ring_cell(int it, int& ex, int& ey) {
+    const int j = int(threadIdx.x) + it * GNT;
+    if (it < GTIT) {
+        ey = j / GTX + 1;
+        ex = j - (ey - 1) * GTX + 1;
+    } else {
+        const int r = min(j - GTX * GTY, GNR - GTX * GTY - 1);   // past the end: the last (loads only)
+        if (r < 2 * GRW) {
+            ey = r < GRW ? 0 : GRH - 1;
+            ex = r < GRW ? r : r - GRW;
+        } else {
+            const int c = r - 2 * GRW;
+            ey = (c >> 1) + 1;
+            ex = (c & 1) ? GRW - 1 : 0;
+        }
+    }
+}
+
 // pre(it, e, i): loads of ring cell e (i its clamped global index) for all of
-// a thread's cells; cell(it, e, px, py, in, tile): phase 1; gather(e, px, py):
-// phase 2 for the tile cells.
+// a thread's cells; cell(it, e, px, py, in, tile): phase 1; gather(it, e, px,
+// py): phase 2 for the tile cells on the grid.  Phase 2 visits the tile cells
+// with the same (thread, it) as phase 1, so what a cell computes for itself
+// can stay in registers indexed by `it` (it < GTIT); only what it sends to
+// its neighbours goes through LDS.
 template <class Pre, class Cell, class Gather>
 __device__ __forceinline__ void tile_gather(int nx, int ny, Pre&& pre, Cell&& cell, Gather&& gather) {
     const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
 #pragma unroll
     for (int it = 0; it < GIT; ++it) {
-        const int e = min(int(threadIdx.x) + it * GNT, GNR - 1);
-        const int ey = e / GRW, ex = e - ey * GRW;
+        int ex, ey;
+        ring_cell(it, ex, ey);
         const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
-        pre(it, e, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx));
+        pre(it, ey * GRW + ex, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx));
     }
     __syncthreads();   // staged LDS regions are complete
 #pragma unroll
     for (int it = 0; it < GIT; ++it) {
-        const int e = threadIdx.x + it * GNT;
-        if (GNR % GNT != 0 && e >= GNR) break;
-        const int ey = e / GRW, ex = e - ey * GRW;
+        if (it == GTIT && int(threadIdx.x) >= GNR - GTX * GTY) break;   // past the end of the ring
+        int ex, ey;
+        ring_cell(it, ex, ey);
         const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
         const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
-        const bool tile = ex >= 1 && ex <= GTX && ey >= 1 && ey <= GTY;
-        cell(it, e, gx, gy, in, tile);
+        cell(it, ey * GRW + ex, gx, gy, in, it < GTIT);
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < GTX * GTY; t += GNT) {
-        const int ty = t / GTX, tx = t - ty * GTX;
-        const int gx = x0 + tx, gy = y0 + ty;
-        if (gx < nx && gy < ny) gather((ty + 1) * GRW + tx + 1, gx, gy);
+#pragma unroll
+    for (int it = 0; it < GTIT; ++it) {
+        int ex, ey;
+        ring_cell(it, ex, ey);
+        const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
+        if (gx < nx && gy < ny) gather(it, ey * GRW + ex, gx, gy);
     }
 }
 
@@ -152,9 +191,10 @@ __device__ __forceinline__ void tile_gather(int nx, int ny, Pre&& pre, Cell&& ce
 // flux outputs (water, flux, velocity).  Writes li's water and flux (fresh)
 // and adds the total-height adjoint to li's rock and dirt.
 __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
-    // per ring/tile cell: outflow adj (4), inflow adj sent to neighbours (4),
-    // total-height adj (own, 4 sent), water adj
-    __shared__ float s_fo[4][GNR], s_fin[4][GNR], s_wn[4][GNR], s_wp[GNR], s_d1[GNR];
+    // sent to neighbours, per ring/tile cell: inflow adj (4), total-height adj (4);
+    // kept per thread: outflow adj (4), own total-height adj, water adj
+    __shared__ float s_fin[4][GNR], s_wn[4][GNR];
+    float o_fo[GTIT][4], o_wp[GTIT], o_d1[GTIT];
     __shared__ float t_w[SRN], t_fl[SRN], t_fr[SRN], t_ft[SRN], t_fb[SRN];
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
@@ -241,24 +281,26 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
                 if (py == 0) { lfin[3] = 0.0f; lwn[3] = 0.0f; }
             }
             for (int j = 0; j < 4; ++j) {
-                s_fo[j][e] = lfo[j];
                 s_fin[j][e] = lfin[j];
                 s_wn[j][e] = lwn[j];
             }
-            s_wp[e] = lwp;
-            s_d1[e] = ld1;
+            if (it < GTIT) {                                  // a tile cell keeps its own part
+                for (int j = 0; j < 4; ++j) o_fo[it][j] = lfo[j];
+                o_wp[it] = lwp;
+                o_d1[it] = ld1;
+            }
         },
-        [&](int e, int px, int py) {
+        [&](int it, int e, int px, int py) {
             const size_t i = at(px, py, nx);
             // neighbour q = p - 1 sends its R-inflow adjoint (index 1) to p's L
             // outflow, q = p + 1 its index 0 to p's R, q = p - nx its index 2
             // to p's B, q = p + nx its index 3 to p's T
-            li.p[HG_P_FLUX_L][i] = s_fo[0][e] + s_fin[1][e - 1];
-            li.p[HG_P_FLUX_R][i] = s_fo[1][e] + s_fin[0][e + 1];
-            li.p[HG_P_FLUX_T][i] = s_fo[2][e] + s_fin[3][e + GRW];
-            li.p[HG_P_FLUX_B][i] = s_fo[3][e] + s_fin[2][e - GRW];
-            const float lw = s_wp[e] + s_wn[0][e + 1] + s_wn[1][e - 1] + s_wn[2][e - GRW] + s_wn[3][e + GRW];
-            li.p[HG_P_WATER][i] = s_d1[e] + lw;                // w = rock + dirt + water
+            li.p[HG_P_FLUX_L][i] = o_fo[it][0] + s_fin[1][e - 1];
+            li.p[HG_P_FLUX_R][i] = o_fo[it][1] + s_fin[0][e + 1];
+            li.p[HG_P_FLUX_T][i] = o_fo[it][2] + s_fin[3][e + GRW];
+            li.p[HG_P_FLUX_B][i] = o_fo[it][3] + s_fin[2][e - GRW];
+            const float lw = o_wp[it] + s_wn[0][e + 1] + s_wn[1][e - 1] + s_wn[2][e - GRW] + s_wn[3][e + GRW];
+            li.p[HG_P_WATER][i] = o_d1[it] + lw;               // w = rock + dirt + water
             li.p[HG_P_ROCK][i] += lw;
             li.p[HG_P_DIRT][i] += lw;
         });
@@ -268,9 +310,11 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
 // Writes li's rock, dirt and sediment (fresh) and adds to li's velocity.
 __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
                                                      double* gpart) {
-    __shared__ float s_dx[GNR], s_dz[GNR], s_r[GNR], s_d[GNR], s_s0[GNR], s_s1[GNR], s_vx[GNR], s_vy[GNR],
-        s_vz[GNR];
+    // sent to neighbours: the slope adjoints; kept per thread: the cell's own
+    // rock, dirt, sediment and velocity adjoints
+    __shared__ float s_dx[GNR], s_dz[GNR];
     __shared__ float t_r[SRN], t_d[SRN];
+    float o_r[GTIT], o_d[GTIT], o_s0[GTIT], o_s1[GTIT], o_vx[GTIT], o_vy[GTIT], o_vz[GTIT];
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
     {
@@ -372,24 +416,26 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
             }
             s_dx[e] = ldx;
             s_dz[e] = ldz;
-            s_r[e] = lr_in;
-            s_d[e] = ld_in;
-            s_s0[e] = ls0_in;
-            s_s1[e] = ls1_in;
-            s_vx[e] = lvx;
-            s_vy[e] = lvy;
-            s_vz[e] = lvz;
+            if (it < GTIT) {                                  // a tile cell keeps its own part
+                o_r[it] = lr_in;
+                o_d[it] = ld_in;
+                o_s0[it] = ls0_in;
+                o_s1[it] = ls1_in;
+                o_vx[it] = lvx;
+                o_vy[it] = lvy;
+                o_vz[it] = lvz;
+            }
         },
-        [&](int e, int px, int py) {
+        [&](int it, int e, int px, int py) {
             const size_t i = at(px, py, nx);
             const float nb = (s_dx[e - 1] - s_dx[e + 1]) + (s_dz[e - GRW] - s_dz[e + GRW]);
-            li.p[HG_P_ROCK][i] = s_r[e] + nb;
-            li.p[HG_P_DIRT][i] = s_d[e] + nb;
-            li.p[HG_P_SED_ROCK][i] = s_s0[e];
-            li.p[HG_P_SED_DIRT][i] = s_s1[e];
-            li.p[HG_P_VEL_U][i] += s_vx[e];
-            li.p[HG_P_VEL_V][i] += s_vy[e];
-            li.p[HG_P_VEL_DEPTH][i] += s_vz[e];
+            li.p[HG_P_ROCK][i] = o_r[it] + nb;
+            li.p[HG_P_DIRT][i] = o_d[it] + nb;
+            li.p[HG_P_SED_ROCK][i] = o_s0[it];
+            li.p[HG_P_SED_DIRT][i] = o_s1[it];
+            li.p[HG_P_VEL_U][i] += o_vx[it];
+            li.p[HG_P_VEL_V][i] += o_vy[it];
+            li.p[HG_P_VEL_DEPTH][i] += o_vz[it];
         });
     reduce_coef<GNT>(gpart, gKc, gKs, gKd, 0.0);
 }
@@ -418,7 +464,12 @@ __device__ __forceinline__ Taps taps_of(int qx, int qy, float vx, float vy, int 
 
 __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
                                                        double* gpart) {
-    __shared__ float s_vx[GNR], s_vy[GNR], s_l0[GNR], s_l1[GNR];
+    // per tile/ring cell q: its 4 bilinear weights, its taps as offsets from
+    // q (bits 0-1: ix - qx + 1, 2-3: ix1 - qx + 1, 4-5: iy - qy + 1, 6-7:
+    // iy1 - qy + 1; 0 when q is off the grid or its taps are not near), and the
+    // sediment adjoints it spreads over them
+    __shared__ float s_w[4][GNR], s_l0[GNR], s_l1[GNR];
+    __shared__ unsigned char s_tap[GNR];
     const float* VX = x.p[HG_P_VEL_U];
     const float* VY = x.p[HG_P_VEL_V];
     double gKe = 0;
@@ -428,31 +479,49 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
             pvx[it] = VX[i]; pvy[it] = VY[i];
             pl0[it] = lo.p[HG_P_SED_ROCK][i]; pl1[it] = lo.p[HG_P_SED_DIRT][i];
         },
-        [&](int it, int e, int, int, bool in, bool) {
-            s_vx[e] = in ? pvx[it] : 0.0f;
-            s_vy[e] = in ? pvy[it] : 0.0f;
+        [&](int it, int e, int qx, int qy, bool in, bool) {
+            unsigned code = 0;
+            float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (in) {
+                const Taps t = taps_of(qx, qy, pvx[it], pvy[it], nx, ny, k.dt);
+                if (t.near) {
+                    code = unsigned(t.ix - qx + 1) | (unsigned(t.ix1 - qx + 1) << 2) | (unsigned(t.iy - qy + 1) << 4) |
+                           (unsigned(t.iy1 - qy + 1) << 6);
+                    w[0] = t.w00; w[1] = t.w10; w[2] = t.w01; w[3] = t.w11;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s_w[j][e] = w[j];
+            s_tap[e] = (unsigned char)code;
             s_l0[e] = in ? pl0[it] : 0.0f;
             s_l1[e] = in ? pl1[it] : 0.0f;
         },
-        [&](int e, int px, int py) {
+        [&](int it, int e, int px, int py) {
             const size_t i = at(px, py, nx);
-            // sediment: gather from the near cells whose taps include p
+            // sediment: gather from the near cells whose taps include p.  A
+            // neighbour q = p + (dx, dy) has p as a tap column when
+            // ix - qx = -dx (or ix1 - qx = -dx), and likewise for rows.  A
+            // near cell's code is never 0: that needs ix1 = ix = qx - 1, and
+            // ix1 = ix only at ix = nx - 1, i.e. qx = nx.
             float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
             for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
                 for (int dx = -1; dx <= 1; ++dx) {
-                    const int qx = px + dx, qy = py + dy;
-                    if (qx < 0 || qy < 0 || qx >= nx || qy >= ny) continue;
                     const int eq = e + dx + dy * GRW;
-                    const Taps t = taps_of(qx, qy, s_vx[eq], s_vy[eq], nx, ny, k.dt);
-                    if (!t.near) continue;
+                    const unsigned code = s_tap[eq];
+                    if (code == 0) continue;
+                    const int cx = 1 - dx, cy = 1 - dy;     // p - q + 1
+                    const bool x0 = int(code & 3u) == cx, x1 = int((code >> 2) & 3u) == cx;
+                    const bool y0 = int((code >> 4) & 3u) == cy, y1 = int((code >> 6) & 3u) == cy;
                     float w = 0.0f;
-                    if (t.iy == py) {
-                        if (t.ix == px) w += t.w00;
-                        if (t.ix1 == px) w += t.w10;
+                    if (y0) {
+                        if (x0) w += s_w[0][eq];
+                        if (x1) w += s_w[1][eq];
                     }
-                    if (t.iy1 == py) {
-                        if (t.ix == px) w += t.w01;
-                        if (t.ix1 == px) w += t.w11;
+                    if (y1) {
+                        if (x0) w += s_w[2][eq];
+                        if (x1) w += s_w[3][eq];
                     }
                     a0 += s_l0[eq] * w;
                     a1 += s_l1[eq] * w;
@@ -460,7 +529,7 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
             li.p[HG_P_SED_ROCK][i] = a0;
             li.p[HG_P_SED_DIRT][i] = a1;
             // own velocity and water
-            const float vx = s_vx[e], vy = s_vy[e];
+            const float vx = pvx[it], vy = pvy[it];
             const float bxr = float(px) - vx * k.dt;
             const float byr = float(py) - vy * k.dt;
             const Taps t = taps_of(px, py, vx, vy, nx, ny, k.dt);
@@ -469,7 +538,7 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
             float bx, by;
             back_trace(px, py, vx, vy, k.dt, nx, ny, bx, by);
             const float fx = bx - float(t.ix), fy = by - float(t.iy);
-            const float l0 = s_l0[e], l1 = s_l1[e];
+            const float l0 = pl0[it], l1 = pl1[it];
             float lbx = 0.0f, lby = 0.0f;
             for (int c = 0; c < 2; ++c) {
                 const float* S = x.p[c == 0 ? HG_P_SED_ROCK : HG_P_SED_DIRT];
@@ -518,10 +587,11 @@ __global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, 
 // sum to p (both rock and dirt for layer 1, whose D_k spans both layers).
 // x: the layer's input rock and dirt.  Writes li's layer plane (fresh) and,
 // for layer 1, adds the rock part to li's rock in place.
-__global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes li, int nx, int ny, int layer,
-                                                     KParams k) {
-    __shared__ float s_lD[8][GNR], s_self[GNR], s_hl[GNR];
+template <int layer>
+__global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
+    __shared__ float s_lD[8][GNR];                // sent to the 8 neighbours
     __shared__ float t_r[SRN], t_d[SRN], t_lh[SRN];
+    float o_self[GTIT], o_hl[GTIT];                 // kept per thread
     constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
     const float* R = x.p[HG_P_ROCK];
@@ -540,7 +610,7 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
     }
     tile_gather(nx, ny,
         [&](int, int, size_t) {},
-        [&](int, int e, int px, int py, bool in, bool) {
+        [&](int it, int e, int px, int py, bool in, bool) {
             float lD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             float lself = 0.0f, lh_layer = 0.0f;
             if (in) {
@@ -554,7 +624,7 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
                     const int o = sc + DX[kk] + DY[kk] * SRW;
                     ok[kk] = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
                     float v = r - t_r[o];
-                    if (layer == 1) v = v + (d - t_d[o]);
+                    if constexpr (layer == 1) v = v + (d - t_d[o]);
                     Dk[kk] = v;
                     lhn[kk] = t_lh[o];
                 }
@@ -594,20 +664,22 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
             }
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = lD[kk];
-            s_self[e] = lself;
-            s_hl[e] = lh_layer;
+            if (it < GTIT) {                                  // a tile cell keeps its own part
+                o_self[it] = lself;
+                o_hl[it] = lh_layer;
+            }
         },
-        [&](int e, int px, int py) {
+        [&](int it, int e, int px, int py) {
             const size_t i = at(px, py, nx);
             float recv = 0.0f;                                // from q = p - dir_k: -dL/dD_k(q)
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) recv -= s_lD[kk][e - (DX[kk] + DY[kk] * GRW)];
-            const float shared = s_self[e] + recv;
+            const float shared = o_self[it] + recv;
             const float lh = t_lh[sidx(e)];
-            if (layer == 0) {
-                li.p[HG_P_ROCK][i] = lh + shared + s_hl[e];
+            if constexpr (layer == 0) {
+                li.p[HG_P_ROCK][i] = lh + shared + o_hl[it];
             } else {
-                li.p[HG_P_DIRT][i] = lh + shared + s_hl[e];
+                li.p[HG_P_DIRT][i] = lh + shared + o_hl[it];
                 li.p[HG_P_ROCK][i] += shared;
             }
         });
@@ -616,12 +688,15 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
 // ---- smoothing -----------------------------------------------------------
 // y = x + m (mean5 - x) at an extremum.  Gather: p keeps its own share and
 // receives 0.2 m dL/dy(q) from each interior neighbour q that is an extremum.
+// The extremum flags of the tile + 1-cell ring are evaluated once into LDS.
 // Writes li's rock and dirt (fresh).
 __global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
     // rock, dirt and their adjoints on the tile + 2-cell region; cells off the
     // grid hold the clamped cell's values, which only feed extremum flags of
     // non-interior cells, and those are not used
     __shared__ float t_r[SRN], t_d[SRN], t_lr[SRN], t_ld[SRN];
+    // per tile/ring cell: bit 0 rock, bit 1 surface flag of an interior extremum
+    __shared__ unsigned char s_f[GNR];
     {
         float a[SIT], b[SIT], c[SIT], d[SIT];
         stage_walk(nx, ny, [&](int it, int, size_t i, bool) {
@@ -632,40 +707,53 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes 
             t_r[s] = a[it]; t_d[s] = b[it]; t_lr[s] = c[it]; t_ld[s] = d[it];
         });
     }
-    __syncthreads();
     const float m = k.smooth_m;
-    auto interior = [&](int xx, int yy) { return xx > 0 && yy > 0 && xx < nx - 1 && yy < ny - 1; };
-    auto flags = [&](int sc, bool& er, bool& eg) {
-        smooth_flags(t_r[sc], t_d[sc], t_r[sc - 1], t_d[sc - 1], t_r[sc + 1], t_d[sc + 1], t_r[sc + SRW],
-                     t_d[sc + SRW], t_r[sc - SRW], t_d[sc - SRW], er, eg);
-    };
-    const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
-    for (int t = threadIdx.x; t < GTX * GTY; t += GNT) {
-        const int ty = t / GTX, tx = t - ty * GTX;
-        const int px = x0 + tx, py = y0 + ty;
-        if (px >= nx || py >= ny) continue;
-        const int sp = (ty + 2) * SRW + tx + 2;
-        const float lr = t_lr[sp], ld = t_ld[sp];
-        float ar = lr, ad = ld;
-        if (interior(px, py)) {
-            bool er, eg;
-            flags(sp, er, eg);
-            if (er) ar = lr * (1.0f - 0.8f * m);
-            if (eg) ad = ld * (1.0f - 0.8f * m);
-        }
-        constexpr int NXo[4] = {-1, 1, 0, 0}, NYo[4] = {0, 0, 1, -1};
+    float o_lr[GTIT], o_ld[GTIT];
+    tile_gather(nx, ny,
+        [&](int, int, size_t) {},
+        [&](int it, int e, int px, int py, bool, bool) {
+            const int sc = sidx(e);
+            unsigned f = 0;
+            if (px > 0 && py > 0 && px < nx - 1 && py < ny - 1) {
+                bool er, eg;
+                smooth_flags(t_r[sc], t_d[sc], t_r[sc - 1], t_d[sc - 1], t_r[sc + 1], t_d[sc + 1], t_r[sc + SRW],
+                             t_d[sc + SRW], t_r[sc - SRW], t_d[sc - SRW], er, eg);
+                f = (er ? 1u : 0u) | (eg ? 2u : 0u);
+            }
+            s_f[e] = (unsigned char)f;
+            if (it < GTIT) {                                  // a tile cell keeps its own share
+                const float lr = t_lr[sc], ld = t_ld[sc];
+                o_lr[it] = (f & 1u) ? lr * (1.0f - 0.8f * m) : lr;
+                o_ld[it] = (f & 2u) ? ld * (1.0f - 0.8f * m) : ld;
+            }
+        },
+        [&](int it, int e, int px, int py) {
+            const int sp = sidx(e);
+            float ar = o_lr[it], ad = o_ld[it];
+            constexpr int NXo[4] = {-1, 1, 0, 0}, NYo[4] = {0, 0, 1, -1};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int sq = sp + NXo[j] + NYo[j] * SRW;
-            bool er, eg;
-            flags(sq, er, eg);
-            const bool use = interior(px + NXo[j], py + NYo[j]);
-            if (use && er) ar += t_lr[sq] * 0.2f * m;
-            if (use && eg) ad += t_ld[sq] * 0.2f * m;
-        }
-        const size_t i = at(px, py, nx);
-        li.p[HG_P_ROCK][i] = ar;
-        li.p[HG_P_DIRT][i] = ad;
+            for (int j = 0; j < 4; ++j) {
+                const unsigned f = s_f[e + NXo[j] + NYo[j] * GRW];
+                const int sq = sp + NXo[j] + NYo[j] * SRW;
+                if (f & 1u) ar += t_lr[sq] * 0.2f * m;
+                if (f & 2u) ad += t_ld[sq] * 0.2f * m;
+            }
+            const size_t i = at(px, py, nx);
+            li.p[HG_P_ROCK][i] = ar;
+            li.p[HG_P_DIRT][i] = ad;
+        });
+}
+
+// Copies the 9 state planes of n floats (n = side^2, side % 8 == 0) in one launch.
+struct StateCopy {
+    const float* s[HG_NUM_STATE_PLANES];
+    float* d[HG_NUM_STATE_PLANES];
+};
+__global__ __launch_bounds__(256) void k_copy_state(StateCopy l, size_t n4) {
+    const size_t total = size_t(HG_NUM_STATE_PLANES) * n4;
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += size_t(gridDim.x) * 256) {
+        const size_t pl = e / n4, j = e - pl * n4;
+        reinterpret_cast<float4*>(l.d[pl])[j] = reinterpret_cast<const float4*>(l.s[pl])[j];
     }
 }
 
@@ -708,8 +796,9 @@ struct AdjWork {
     size_t pn = 0, pitch = 0;
     void* block = nullptr;
     std::vector<float*> ckpt;      // (#checkpoints) x 9 planes
-    // Two sets, so that one segment is recomputed while the previous one is
-    // reversed: (k_ckpt + 1) x 9 planes, each step's start state in a segment
+    // Two sets (one for a single segment), so that one segment is recomputed
+    // while the previous one is reversed: (k + 1) x 9 planes (k the longest
+    // segment), each step's start state in a segment
     // (slot 0 is the checkpoint itself), and k_ckpt x NTAPE planes, each
     // step's intermediates.
     std::vector<float*> seg[2];
@@ -756,6 +845,16 @@ int step_into(hg_ctx* c, float* const* in, float* const* out, float* const* tp, 
     return HG_OK;
 }
 
+int copy_state(hg_ctx* c, float* const* src, float* const* dst, size_t pn) {
+    hgk::StateCopy l;
+    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) { l.s[p] = src[p]; l.d[p] = dst[p]; }
+    const size_t n4 = pn / 4;
+    const unsigned grid = unsigned(std::min<size_t>((HG_NUM_STATE_PLANES * n4 + 255) / 256, 8192));
+    hgk::k_copy_state<<<grid, 256, 0, c->stream>>>(l, n4);
+    HG_HIP(c, hipGetLastError());
+    return HG_OK;
+}
+
 int zero_planes(hg_ctx* c, const std::vector<float*>& planes, size_t pn) {
     for (size_t b = 0; b < planes.size(); b += 48) {
         hgk::PlaneList l{};
@@ -788,24 +887,34 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     const int n = c->n;
     const size_t pn = plane_n(c);
     const uint32_t nck = (nsteps + k_ckpt - 1) / k_ckpt;
+    const uint32_t kseg = std::min(k_ckpt, nsteps);        // longest segment
+    const int nsets = nck > 1 ? 2 : 1;
     AdjWork w;
     w.pn = pn;
     w.pitch = hgk::plane_stride_bytes(pn);
     const size_t n_pool = 64;
     const size_t nplanes = size_t(nck) * HG_NUM_STATE_PLANES +
-                           2 * (size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES + size_t(k_ckpt) * NTAPE) + n_pool + 2;
-    HG_HIP(c, hipMalloc(&w.block, nplanes * w.pitch + 256));
-    struct Free {
-        void* p;
-        ~Free() { if (p) (void)hipFree(p); }
-    } free_block{w.block};
+                           nsets * (size_t(kseg + 1) * HG_NUM_STATE_PLANES + size_t(kseg) * NTAPE) + n_pool + 2;
+    // The workspace stays with the context for the next call (a rollout of
+    // the same shape allocates nothing); a larger one replaces it.
+    const size_t need = nplanes * w.pitch + 256;
+    if (c->adj_bytes < need) {
+        if (c->adj_block) {
+            HG_HIP(c, hipFree(c->adj_block));
+            c->adj_block = nullptr;
+            c->adj_bytes = 0;
+        }
+        HG_HIP(c, hipMalloc(&c->adj_block, need));
+        c->adj_bytes = need;
+    }
+    w.block = c->adj_block;
     char* base = static_cast<char*>(w.block);
     size_t used = 0;
     auto plane = [&]() { float* p = reinterpret_cast<float*>(base + used * w.pitch); ++used; return p; };
     for (size_t i = 0; i < size_t(nck) * HG_NUM_STATE_PLANES; ++i) w.ckpt.push_back(plane());
-    for (int b = 0; b < 2; ++b) {
-        for (size_t i = 0; i < size_t(k_ckpt + 1) * HG_NUM_STATE_PLANES; ++i) w.seg[b].push_back(plane());
-        for (size_t i = 0; i < size_t(k_ckpt) * NTAPE; ++i) w.tape[b].push_back(plane());
+    for (int b = 0; b < nsets; ++b) {
+        for (size_t i = 0; i < size_t(kseg + 1) * HG_NUM_STATE_PLANES; ++i) w.seg[b].push_back(plane());
+        for (size_t i = 0; i < size_t(kseg) * NTAPE; ++i) w.tape[b].push_back(plane());
     }
     for (size_t i = 0; i < n_pool; ++i) w.pool.push_back(plane());
     w.zero = plane();
@@ -816,15 +925,48 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     HG_HIP(c, hipMemsetAsync(w.gpart, 0, nblk * 4 * sizeof(double), c->stream));
     const uint64_t step0 = c->step;
 
-    // ---- forward with checkpoints every k_ckpt steps
+    // ---- forward with checkpoints every k_ckpt steps.  Segment `seg` is
+    // reversed from plane set (nck - 1 - seg) % 2.  The last two segments (the
+    // only one, if there is one) run through the taped kernels straight into
+    // the set their reversal reads, so they are never recomputed; the first of
+    // them starts from its checkpoint, the second from the first's last slot.
+    const uint32_t first_taped = nck - uint32_t(nsets);
     for (uint32_t s = 0; s < nsteps; ++s) {
-        if (s % k_ckpt == 0)
-            for (int p = 0; p < HG_NUM_STATE_PLANES; ++p)
-                HG_HIP(c, hipMemcpyAsync(w.ckpt[(s / k_ckpt) * HG_NUM_STATE_PLANES + p], c->cur[p], pn * sizeof(float),
-                                         hipMemcpyDeviceToDevice, c->stream));
-        const int rc = fused_step(c, rain, thermal, smooth, false);
-        if (rc != HG_OK) return rc;
+        const uint32_t seg = s / k_ckpt, j = s - seg * k_ckpt;
+        if (seg < first_taped) {
+            if (j == 0) {
+                const int rc = copy_state(c, c->cur, &w.ckpt[size_t(seg) * HG_NUM_STATE_PLANES], pn);
+                if (rc != HG_OK) return rc;
+            }
+            const int rc = fused_step(c, rain, thermal, smooth, false);
+            if (rc != HG_OK) return rc;
+        } else {
+            std::vector<float*>& slots = w.seg[(nck - 1 - seg) % 2];
+            if (j == 0) {
+                if (seg == first_taped) {
+                    float* const* ck = &w.ckpt[size_t(seg) * HG_NUM_STATE_PLANES];
+                    const int rc = copy_state(c, c->cur, ck, pn);
+                    if (rc != HG_OK) return rc;
+                    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) slots[size_t(p)] = ck[p];
+                } else {
+                    const std::vector<float*>& prev = w.seg[(nck - seg) % 2];
+                    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p)
+                        slots[size_t(p)] = prev[size_t(k_ckpt) * HG_NUM_STATE_PLANES + p];
+                }
+            }
+            const int rc = step_into(c, &slots[size_t(j) * HG_NUM_STATE_PLANES],
+                                     &slots[size_t(j + 1) * HG_NUM_STATE_PLANES],
+                                     &w.tape[(nck - 1 - seg) % 2][size_t(j) * NTAPE], uint32_t(step0 + s), rain,
+                                     thermal, smooth, c->stream);
+            if (rc != HG_OK) return rc;
+        }
         c->step += 1;
+    }
+    {
+        // the final state, in the last segment's last slot, back into the context
+        const size_t last = nsteps - (nck - 1) * k_ckpt;
+        const int rc = copy_state(c, &w.seg[0][last * HG_NUM_STATE_PLANES], c->cur, pn);
+        if (rc != HG_OK) return rc;
     }
     const uint64_t step_end = c->step;
 
@@ -858,23 +1000,17 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     // neither kernel sequence fills the chip alone.  Segment i (counting from
     // the last) uses plane set i % 2; its recompute waits for the reversal that
     // last used that set, and its reversal waits for its recompute.
-    hipStream_t rs = nullptr;
-    hipEvent_t ev_fwd = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
-    struct Streams {
-        hipStream_t& rs;
-        hipEvent_t &f, *r, *v;
-        ~Streams() {
-            if (rs) { (void)hipStreamSynchronize(rs); (void)hipStreamDestroy(rs); }
-            for (hipEvent_t e : {f, r[0], r[1], v[0], v[1]})
-                if (e) (void)hipEventDestroy(e);
-        }
-    } streams{rs, ev_fwd, ev_rec, ev_rev};
-    HG_HIP(c, hipStreamCreateWithFlags(&rs, hipStreamNonBlocking));
-    HG_HIP(c, hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming));
-    for (int b = 0; b < 2; ++b) {
-        HG_HIP(c, hipEventCreateWithFlags(&ev_rec[b], hipEventDisableTiming));
-        HG_HIP(c, hipEventCreateWithFlags(&ev_rev[b], hipEventDisableTiming));
+    if (!c->adj_stream) {
+        HG_HIP(c, hipStreamCreateWithFlags(&c->adj_stream, hipStreamNonBlocking));
+        for (hipEvent_t& e : c->adj_ev) HG_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    hipStream_t rs = c->adj_stream;
+    hipEvent_t ev_fwd = c->adj_ev[0], ev_rec[2] = {c->adj_ev[1], c->adj_ev[2]},
+               ev_rev[2] = {c->adj_ev[3], c->adj_ev[4]};
+    struct Drain {    // on every return, nothing is left running on the recompute stream
+        hipStream_t rs;
+        ~Drain() { (void)hipStreamSynchronize(rs); }
+    } drain{rs};
     HG_HIP(c, hipEventRecord(ev_fwd, c->stream));     // checkpoints and the zeroed adjoint are written
     HG_HIP(c, hipStreamWaitEvent(rs, ev_fwd, 0));
     // recompute segment `seg` from its checkpoint into plane set b, recording
@@ -894,19 +1030,15 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
         HG_HIP(c, hipEventRecord(ev_rec[b], rs));
         return HG_OK;
     };
-    {
-        const int rc = recompute(int(nck) - 1, 0, false);
-        if (rc != HG_OK) return rc;
-    }
     for (int seg = int(nck) - 1; seg >= 0; --seg) {
         const int i = int(nck) - 1 - seg, b = i % 2;
         const uint32_t s_begin = uint32_t(seg) * k_ckpt;
         const uint32_t s_end = std::min(nsteps, s_begin + k_ckpt);
-        if (seg > 0) {
+        if (seg > 0 && uint32_t(seg - 1) < first_taped) {
             const int rc = recompute(seg - 1, 1 - b, i >= 1);
             if (rc != HG_OK) return rc;
         }
-        HG_HIP(c, hipStreamWaitEvent(c->stream, ev_rec[b], 0));
+        if (uint32_t(seg) < first_taped) HG_HIP(c, hipStreamWaitEvent(c->stream, ev_rec[b], 0));
         std::vector<float*>& slots = w.seg[b];
         // reverse the steps of the segment
         for (int s = int(s_end) - 1; s >= int(s_begin); --s) {
@@ -972,8 +1104,10 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         break;
                     case HG_PASS_THERMAL_TRANSPORT_0:
                     case HG_PASS_THERMAL_TRANSPORT_1:
-                        k_adj_thermal<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n,
-                                                                     pass == HG_PASS_THERMAL_TRANSPORT_0 ? 0 : 1, c->kp);
+                        if (pass == HG_PASS_THERMAL_TRANSPORT_0)
+                            k_adj_thermal<0><<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp);
+                        else
+                            k_adj_thermal<1><<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
                     case HG_PASS_SMOOTH:
                         k_adj_smooth<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp);

@@ -51,6 +51,12 @@ struct hg_ctx {
     hg_ref_particle* refp = nullptr; // the reference particle mode's particles (hg_ref_particles_init)
     uint32_t refp_count = 0;
     unsigned long long* dev_giveups = nullptr;  // in the block's tail
+    // hg_adjoint_rollout's workspace, recompute stream (default priority) and
+    // its events, kept for the next call until hg_destroy
+    void* adj_block = nullptr;
+    size_t adj_bytes = 0;
+    hipStream_t adj_stream = nullptr;
+    hipEvent_t adj_ev[5] = {};
     std::string arch;
     std::string err;
     // timing

@@ -307,6 +307,13 @@ int hg_destroy(hg_ctx* c) {
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipFree(c->block);
     if (c->refp) (void)hipFree(c->refp);
+    if (c->adj_stream) {
+        (void)hipStreamSynchronize(c->adj_stream);
+        (void)hipStreamDestroy(c->adj_stream);
+    }
+    for (hipEvent_t e : c->adj_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->adj_block) (void)hipFree(c->adj_block);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HG_OK;
@@ -697,6 +704,7 @@ int hg_read(hg_ctx* c, int channel, float* rgba, size_t n_floats) {
 
 // ---- heightmap files: NumPy .npy v1.0, little-endian float32, shape (n, n, 4)
 // holding the HEIGHT channel (rock, dirt, water, total) in row-major order.
+}  // extern "C"
 namespace {
 std::string npy_header(size_t n) {
     std::string dict = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(n) + ", " +
@@ -710,6 +718,7 @@ std::string npy_header(size_t n) {
     return h + dict;
 }
 }  // namespace
+extern "C" {
 
 int hg_save_height(hg_ctx* c, const char* path) {
     if (!c) return HG_ERR_INVALID;

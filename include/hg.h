@@ -365,7 +365,11 @@ int hg_ref_particle_stats(hg_ctx* ctx, uint64_t* cas_giveups);
  * dL_dcoef[4], with respect to (Kc, Ks, Kd, Ke) (Ks and Kd shared by both
  * layers).  Clamps, max/min, the talus test and the erosion cases contribute
  * the derivative of the branch the forward run took.  The context is left at
- * the final state, its step counter advanced by nsteps. */
+ * the final state, its step counter advanced by nsteps.  The device workspace
+ * ((#checkpoints + s (k + 1)) x 9 + s k x 11 + 66 planes of n*n floats, with
+ * k = min(k_ckpt, nsteps) and s = 2, or 1 for a single segment) and a second
+ * stream stay with the context for the next call, which reuses them when they
+ * are large enough, until hg_destroy. */
 int hg_adjoint_rollout(hg_ctx* ctx, uint32_t nsteps, uint32_t k_ckpt, uint32_t flags,
                        const float* dL_dH_final, float* dL_dH0, double* dL_dcoef);
 

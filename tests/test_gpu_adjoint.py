@@ -65,10 +65,36 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     n, steps = 32, 9
     h0, tgt = setup_case(n, seed=5)
     ref_h, ref_c, _, _ = gpu_grad(n, h0, tgt, steps, 1)
-    for k in (2, 4, 9, 20):
+    for k in (2, 4, 5, 9, 20):      # 5: two segments, both taped by the forward
         gh, gc, _, _ = gpu_grad(n, h0, tgt, steps, k)
         assert np.abs(gh - ref_h).max() <= 1e-5 * np.abs(ref_h).max(), k
         assert np.allclose(gc, ref_c, rtol=1e-5, atol=1e-12), k
+
+
+def test_repeated_calls_reuse_the_workspace():
+    """The workspace and recompute stream stay with the context: a second
+    call of the same shape, a smaller one (reuse) and a larger one
+    (reallocation) on one context give bit for bit what fresh contexts give."""
+    n = 32
+    h0, tgt = setup_case(n, seed=6)
+    runs = [(9, 4), (9, 4), (5, 2), (9, 9)]
+    fresh = [gpu_grad(n, h0, tgt, s, k)[:2] for s, k in runs]
+    with hg.Terrain(n) as t:
+        t.set_erosion(hg.erosion_settings(Kc=COEF[0], Ks=COEF[1], Kd=COEF[2], Ke=COEF[3]))
+        t.set_rain(hg.RAIN_CONSTANT, amount=0.01)
+        st = np.zeros((hg.NUM_STATE_PLANES, n, n), np.float32)
+        st[hg.P_ROCK] = h0
+        for (steps, k), (ref_h, ref_c) in zip(runs, fresh):
+            t.write_state(st)
+            t.step = 0
+            t.dispatch_grid(steps)
+            h = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
+            g = (2.0 * (h - tgt) / h.size).astype(np.float32)
+            t.write_state(st)
+            t.step = 0
+            gh, gc = t.adjoint_rollout(steps, k, g)
+            assert np.array_equal(gh, ref_h), (steps, k)
+            assert np.array_equal(gc, ref_c), (steps, k)
 
 
 def test_adjoint_leaves_the_final_state():
