@@ -27,6 +27,11 @@ namespace hgk {
 namespace {
 
 __device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * size_t(nx) + size_t(x); }
+// Output and tape planes are written once per launch and not read back by it;
+// written with non-temporal stores the hydro kernel measured 270 -> 262 us at
+// 4096^2 (same results).  Non-temporal loads of the inputs read once per
+// workgroup measured slower and are not used.
+__device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
 
 // ---------------------------------------------------------------------------
 // Eroded sediment of one cell, recomputed from the step's input planes
@@ -277,23 +282,23 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
         es0[it] = s0; es1[it] = s1; evx[it] = vx; evy[it] = vy;
         const bool tile = ex >= HA && ex < HA + TX && ey >= HA && ey < HA + TY;
         if (tile) {
-            io.ro[g] = r;
-            io.dout[g] = d;
-            io.bo[g] = d2 * k.evap;     // sediment_transport.glsl:75
-            io.flo[g] = f[0];
-            io.fro[g] = f[1];
-            io.fto[g] = f[2];
-            io.fbo[g] = f[3];
+            st_stream(&io.ro[g], r);
+            st_stream(&io.dout[g], d);
+            st_stream(&io.bo[g], d2 * k.evap);     // sediment_transport.glsl:75
+            st_stream(&io.flo[g], f[0]);
+            st_stream(&io.fro[g], f[1]);
+            st_stream(&io.fto[g], f[2]);
+            st_stream(&io.fbo[g], f[3]);
             if (EXTRA && io.vx) {
-                io.vx[g] = vx;
-                io.vy[g] = vy;
-                io.vz[g] = vz;
+                st_stream(&io.vx[g], vx);
+                st_stream(&io.vy[g], vy);
+                st_stream(&io.vz[g], vz);
             }
             if (EXTRA && io.tb1) {
-                io.tb1[g] = b;
-                io.tb2[g] = d2;
-                io.ts0[g] = s0;
-                io.ts1[g] = s1;
+                st_stream(&io.tb1[g], b);
+                st_stream(&io.tb2[g], d2);
+                st_stream(&io.ts0[g], s0);
+                st_stream(&io.ts1[g], s1);
             }
         }
     });
@@ -356,8 +361,8 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
             o1 = lerp1(t1, b1, fy);
         }
         const size_t g = at(gx, gy, nx);
-        io.s0o[g] = o0;
-        io.s1o[g] = o1;
+        st_stream(&io.s0o[g], o0);
+        st_stream(&io.s1o[g], o1);
     }
 }
 
@@ -539,11 +544,11 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
         if (gx < nx && gy < ny) {
             const size_t g = at(gx, gy, nx);
             if (EXTRA && io.tr5) {
-                io.tr5[g] = r0;
-                io.td6[g] = d0;
+                st_stream(&io.tr5[g], r0);
+                st_stream(&io.td6[g], d0);
             }
-            io.ro[g] = r;
-            io.dout[g] = d;
+            st_stream(&io.ro[g], r);
+            st_stream(&io.dout[g], d);
         }
     });
 }
