@@ -100,6 +100,7 @@ KParams kparams(const hg_erosion_settings& s, const hg_batch* b) {
     k.rain_mode = b->rain_mode;
     k.seed_lo = uint32_t(b->seed);
     k.seed_hi = uint32_t(b->seed >> 32);
+    k.store_nt = hgk::stream_stores(b->total_floats * sizeof(float));   // every terrain, both buffers
     return k;
 }
 
@@ -181,13 +182,14 @@ void drop_graph(hg_batch* b) {
 int enqueue_steps(hg_batch* b, uint32_t nsteps, uint32_t flags) {
     const bool rain = (flags & HG_STEP_RAIN) && b->rain_mode != HG_RAIN_OFF;
     const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
+    const bool nts = hgk::stream_stores(b->total_floats * sizeof(float)) != 0;   // every terrain, both buffers
     for (uint32_t s = 0; s < nsteps; ++s) {
         BH(b, hgk::launch_batch_hydro(b->hydro_variant, b->dev_terr, b->dev_tiles_h, b->ntiles_h, b->grid_h,
-                                      b->cur, b->dev_step, s, rain, b->stream));
+                                      b->cur, b->dev_step, s, rain, nts, b->stream));
         b->cur ^= (1u << HG_NUM_STATE_PLANES) - 1u;
         if (thermal || smooth) {
             BH(b, hgk::launch_batch_thermal(b->thermal_variant, b->dev_terr, b->dev_tiles_t, b->ntiles_t,
-                                            b->grid_t, b->cur, thermal, smooth, b->stream));
+                                            b->grid_t, b->cur, thermal, smooth, nts, b->stream));
             b->cur ^= (1u << HG_P_ROCK) | (1u << HG_P_DIRT);
         }
     }

@@ -45,6 +45,9 @@ struct KParams {
     // reference rain (HG_RAIN_REFERENCE): Rain_data fields, the explicit time
     // seed and the map's max height
     float rr_thresh, rr_multip, rr_drops, rr_time, rr_max_h;
+    // fused kernels write their outputs with non-temporal stores (the step's
+    // working set exceeds the last-level cache; see stream_stores)
+    int store_nt;
 };
 
 // ---------------------------------------------------------------------------

@@ -27,11 +27,21 @@ namespace hgk {
 namespace {
 
 __device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * size_t(nx) + size_t(x); }
-// Output and tape planes are written once per launch and not read back by it;
-// written with non-temporal stores the hydro kernel measured 270 -> 262 us at
-// 4096^2 (same results).  Non-temporal loads of the inputs read once per
-// workgroup measured slower and are not used.
-__device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
+// Output and tape planes are written once per launch and not read back by it.
+// When the step's state outgrows the last-level cache (KParams::store_nt, a
+// compile-time NTS in the kernels: a run-time choice between two stores of the
+// same value is folded into one plain store) they are written with
+// non-temporal stores: hydro kernel 270 -> 262 us at 4096^2, same results.
+// Below that size ordinary stores are faster (the next launch reads the planes
+// from the cache).  Non-temporal loads of the inputs read once per workgroup
+// measured slower and are not used.
+template <bool NTS>
+__device__ __forceinline__ void st_out(float* p, float v) {
+    if constexpr (NTS)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
 
 // ---------------------------------------------------------------------------
 // Eroded sediment of one cell, recomputed from the step's input planes
@@ -163,7 +173,7 @@ struct HydroTile {
 // EXTRA: also write the optional outputs (velocity, adjoint tape) that are
 // not null; without it their pointers are never read, which keeps them out
 // of the scalar registers of the common case.
-template <int TX, int TY, int HA, int NT, bool EXTRA = false>
+template <int TX, int TY, int HA, int NT, bool EXTRA = false, bool NTS = false>
 __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx, int ny, int bx,
                                            int by, uint32_t step, uint32_t terrain, int rain,
                                            const KParams& k) {
@@ -282,23 +292,23 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
         es0[it] = s0; es1[it] = s1; evx[it] = vx; evy[it] = vy;
         const bool tile = ex >= HA && ex < HA + TX && ey >= HA && ey < HA + TY;
         if (tile) {
-            st_stream(&io.ro[g], r);
-            st_stream(&io.dout[g], d);
-            st_stream(&io.bo[g], d2 * k.evap);     // sediment_transport.glsl:75
-            st_stream(&io.flo[g], f[0]);
-            st_stream(&io.fro[g], f[1]);
-            st_stream(&io.fto[g], f[2]);
-            st_stream(&io.fbo[g], f[3]);
+            st_out<NTS>(&io.ro[g], r);
+            st_out<NTS>(&io.dout[g], d);
+            st_out<NTS>(&io.bo[g], d2 * k.evap);     // sediment_transport.glsl:75
+            st_out<NTS>(&io.flo[g], f[0]);
+            st_out<NTS>(&io.fro[g], f[1]);
+            st_out<NTS>(&io.fto[g], f[2]);
+            st_out<NTS>(&io.fbo[g], f[3]);
             if (EXTRA && io.vx) {
-                st_stream(&io.vx[g], vx);
-                st_stream(&io.vy[g], vy);
-                st_stream(&io.vz[g], vz);
+                st_out<NTS>(&io.vx[g], vx);
+                st_out<NTS>(&io.vy[g], vy);
+                st_out<NTS>(&io.vz[g], vz);
             }
             if (EXTRA && io.tb1) {
-                st_stream(&io.tb1[g], b);
-                st_stream(&io.tb2[g], d2);
-                st_stream(&io.ts0[g], s0);
-                st_stream(&io.ts1[g], s1);
+                st_out<NTS>(&io.tb1[g], b);
+                st_out<NTS>(&io.tb2[g], d2);
+                st_out<NTS>(&io.ts0[g], s0);
+                st_out<NTS>(&io.ts1[g], s1);
             }
         }
     });
@@ -361,8 +371,8 @@ __device__ __forceinline__ void hydro_tile(float* lds, const HydroIO& io, int nx
             o1 = lerp1(t1, b1, fy);
         }
         const size_t g = at(gx, gy, nx);
-        st_stream(&io.s0o[g], o0);
-        st_stream(&io.s1o[g], o1);
+        st_out<NTS>(&io.s0o[g], o0);
+        st_out<NTS>(&io.s1o[g], o1);
     }
 }
 
@@ -389,13 +399,13 @@ __device__ __forceinline__ void tile_of(int b, int nb, int gx, int xcd, int& bx,
     bx = t - by * gx;
 }
 
-template <int TX, int TY, int HA, int NT, bool EXTRA>
+template <int TX, int TY, int HA, int NT, bool EXTRA, bool NTS>
 __global__ __launch_bounds__(NT) void k_fused_hydro(HydroIO io, int nx, int ny, uint32_t step,
                                                     uint32_t terrain, int rain, int xcd, KParams k) {
     __shared__ float lds[6 * HydroTile<TX, TY, HA, NT>::NLDS];
     int bx, by;
     tile_of(blockIdx.x, gridDim.x, (nx + TX - 1) / TX, xcd, bx, by);
-    hydro_tile<TX, TY, HA, NT, EXTRA>(lds, io, nx, ny, bx, by, step, terrain, rain, k);
+    hydro_tile<TX, TY, HA, NT, EXTRA, NTS>(lds, io, nx, ny, bx, by, step, terrain, rain, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -423,7 +433,7 @@ struct ThermalTile {
 };
 
 // One output tile (bx, by) of the thermal kernel; `lds` holds 3*N0 floats.
-template <int TX, int TY, int NT, bool EXTRA = false>
+template <int TX, int TY, int NT, bool EXTRA = false, bool NTS = false>
 __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int nx, int ny, int bx,
                                              int by, int do_thermal, int do_smooth, const KParams& k) {
     constexpr int W0 = ThermalTile<TX, TY>::W0, N0 = ThermalTile<TX, TY>::N0;
@@ -544,22 +554,22 @@ __device__ __forceinline__ void thermal_tile(float* lds, const ThermIO& io, int 
         if (gx < nx && gy < ny) {
             const size_t g = at(gx, gy, nx);
             if (EXTRA && io.tr5) {
-                st_stream(&io.tr5[g], r0);
-                st_stream(&io.td6[g], d0);
+                st_out<NTS>(&io.tr5[g], r0);
+                st_out<NTS>(&io.td6[g], d0);
             }
-            st_stream(&io.ro[g], r);
-            st_stream(&io.dout[g], d);
+            st_out<NTS>(&io.ro[g], r);
+            st_out<NTS>(&io.dout[g], d);
         }
     });
 }
 
-template <int TX, int TY, int NT, bool EXTRA>
+template <int TX, int TY, int NT, bool EXTRA, bool NTS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_fused_thermal(
     ThermIO io, int nx, int ny, int do_thermal, int do_smooth, int xcd, KParams k) {
     __shared__ float lds[3 * ThermalTile<TX, TY>::N0];
     int bx, by;
     tile_of(blockIdx.x, gridDim.x, (nx + TX - 1) / TX, xcd, bx, by);
-    thermal_tile<TX, TY, NT, EXTRA>(lds, io, nx, ny, bx, by, do_thermal, do_smooth, k);
+    thermal_tile<TX, TY, NT, EXTRA, NTS>(lds, io, nx, ny, bx, by, do_thermal, do_smooth, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -583,7 +593,7 @@ __device__ __forceinline__ HydroIO batch_hydro_io(const BatchTerrain& T, uint32_
     return io;
 }
 
-template <int TX, int TY, int HA, int NT>
+template <int TX, int TY, int HA, int NT, bool NTS>
 __global__ __launch_bounds__(NT) void k_batch_hydro(const BatchTerrain* __restrict__ terr,
                                                     const uint32_t* __restrict__ tiles,
                                                     int ntiles, uint32_t cur, const uint32_t* step_ptr,
@@ -600,12 +610,12 @@ __global__ __launch_bounds__(NT) void k_batch_hydro(const BatchTerrain* __restri
         const HydroIO io = batch_hydro_io(T, cur);
         const KParams k = T.k;
         const int n = T.n;
-        hydro_tile<TX, TY, HA, NT>(lds, io, n, n, bx, by, step, uint32_t(ti), rain, k);
+        hydro_tile<TX, TY, HA, NT, false, NTS>(lds, io, n, n, bx, by, step, uint32_t(ti), rain, k);
         __syncthreads();
     }
 }
 
-template <int TX, int TY, int NT>
+template <int TX, int TY, int NT, bool NTS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_thermal(
     const BatchTerrain* __restrict__ terr, const uint32_t* __restrict__ tiles, int ntiles,
     uint32_t cur, int do_thermal,
@@ -624,7 +634,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         io.dout = T.plane[HG_P_DIRT][cd ^ 1];
         const KParams k = T.k;
         const int n = T.n;
-        thermal_tile<TX, TY, NT>(lds, io, n, n, bx, by, do_thermal, do_smooth, k);
+        thermal_tile<TX, TY, NT, false, NTS>(lds, io, n, n, bx, by, do_thermal, do_smooth, k);
         __syncthreads();
     }
 }
@@ -692,12 +702,16 @@ hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, ui
 #define X(i, TX_, TY_, HA_, NT_)                                                              \
     case i: {                                                                                 \
         const unsigned grid = unsigned((nx + TX_ - 1) / TX_) * unsigned((ny + TY_ - 1) / TY_); \
-        if (io.vx || io.tb1)                                                                  \
-            k_fused_hydro<TX_, TY_, HA_, NT_, true><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, \
-                                                                         rain ? 1 : 0, xcd, k);   \
+        const bool ex = io.vx || io.tb1;                                                      \
+        const int r = rain ? 1 : 0;                                                           \
+        if (ex && k.store_nt)                                                                 \
+            k_fused_hydro<TX_, TY_, HA_, NT_, true, true><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, r, xcd, k); \
+        else if (ex)                                                                          \
+            k_fused_hydro<TX_, TY_, HA_, NT_, true, false><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, r, xcd, k); \
+        else if (k.store_nt)                                                                  \
+            k_fused_hydro<TX_, TY_, HA_, NT_, false, true><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, r, xcd, k); \
         else                                                                                  \
-            k_fused_hydro<TX_, TY_, HA_, NT_, false><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, \
-                                                                          rain ? 1 : 0, xcd, k);  \
+            k_fused_hydro<TX_, TY_, HA_, NT_, false, false><<<grid, NT_, 0, s>>>(io, nx, ny, step, terrain, r, xcd, k); \
         break;                                                                                \
     }
         HG_HYDRO_VARIANTS(X)
@@ -709,12 +723,16 @@ hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, ui
 
 hipError_t launch_batch_hydro(int variant, const BatchTerrain* terr, const uint32_t* tiles, int ntiles,
                               int grid, uint32_t cur, const uint32_t* step_ptr, uint32_t step_off,
-                              bool rain, hipStream_t s) {
+                              bool rain, bool nts, hipStream_t s) {
     switch (variant) {
 #define X(i, TX_, TY_, HA_, NT_)                                                                   \
     case i:                                                                                        \
-        k_batch_hydro<TX_, TY_, HA_, NT_><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur, step_ptr, \
-                                                                step_off, rain ? 1 : 0);        \
+        if (nts)                                                                                   \
+            k_batch_hydro<TX_, TY_, HA_, NT_, true><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur, step_ptr, \
+                                                                      step_off, rain ? 1 : 0);    \
+        else                                                                                       \
+            k_batch_hydro<TX_, TY_, HA_, NT_, false><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur, step_ptr, \
+                                                                       step_off, rain ? 1 : 0);   \
         break;
         HG_HYDRO_VARIANTS(X)
 #undef X
@@ -724,12 +742,16 @@ hipError_t launch_batch_hydro(int variant, const BatchTerrain* terr, const uint3
 }
 
 hipError_t launch_batch_thermal(int variant, const BatchTerrain* terr, const uint32_t* tiles, int ntiles,
-                                int grid, uint32_t cur, bool thermal, bool smooth, hipStream_t s) {
+                                int grid, uint32_t cur, bool thermal, bool smooth, bool nts, hipStream_t s) {
     switch (variant) {
 #define X(i, TX_, TY_, NT_)                                                                   \
     case i:                                                                                   \
-        k_batch_thermal<TX_, TY_, NT_><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur,          \
-                                                             thermal ? 1 : 0, smooth ? 1 : 0); \
+        if (nts)                                                                              \
+            k_batch_thermal<TX_, TY_, NT_, true><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur, \
+                                                                   thermal ? 1 : 0, smooth ? 1 : 0); \
+        else                                                                                  \
+            k_batch_thermal<TX_, TY_, NT_, false><<<grid, NT_, 0, s>>>(terr, tiles, ntiles, cur, \
+                                                                    thermal ? 1 : 0, smooth ? 1 : 0); \
         break;
         HG_THERMAL_VARIANTS(X)
 #undef X
@@ -752,7 +774,7 @@ int batch_hydro_blocks_per_cu(int variant) {
     switch (variant) {
 #define X(i, TX_, TY_, HA_, NT_)                                                                  \
     case i:                                                                                       \
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_hydro<TX_, TY_, HA_, NT_>, \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_hydro<TX_, TY_, HA_, NT_, true>, \
                                                          NT_, 0) != hipSuccess)                  \
             nb = 1;                                                                               \
         break;
@@ -768,7 +790,7 @@ int batch_thermal_blocks_per_cu(int variant) {
     switch (variant) {
 #define X(i, TX_, TY_, NT_)                                                                   \
     case i:                                                                                   \
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_thermal<TX_, TY_, NT_>, \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_thermal<TX_, TY_, NT_, true>, \
                                                          NT_, 0) != hipSuccess)              \
             nb = 1;                                                                           \
         break;
@@ -785,12 +807,15 @@ hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, 
 #define X(i, TX_, TY_, NT_)                                                                  \
     case i: {                                                                                \
         const unsigned grid = unsigned((nx + TX_ - 1) / TX_) * unsigned((ny + TY_ - 1) / TY_); \
-        if (io.tr5)                                                                          \
-            k_fused_thermal<TX_, TY_, NT_, true><<<grid, NT_, 0, s>>>(io, nx, ny, thermal ? 1 : 0, \
-                                                                      smooth ? 1 : 0, xcd, k); \
+        const int th = thermal ? 1 : 0, sm = smooth ? 1 : 0;                                 \
+        if (io.tr5 && k.store_nt)                                                            \
+            k_fused_thermal<TX_, TY_, NT_, true, true><<<grid, NT_, 0, s>>>(io, nx, ny, th, sm, xcd, k); \
+        else if (io.tr5)                                                                     \
+            k_fused_thermal<TX_, TY_, NT_, true, false><<<grid, NT_, 0, s>>>(io, nx, ny, th, sm, xcd, k); \
+        else if (k.store_nt)                                                                 \
+            k_fused_thermal<TX_, TY_, NT_, false, true><<<grid, NT_, 0, s>>>(io, nx, ny, th, sm, xcd, k); \
         else                                                                                 \
-            k_fused_thermal<TX_, TY_, NT_, false><<<grid, NT_, 0, s>>>(io, nx, ny, thermal ? 1 : 0, \
-                                                                       smooth ? 1 : 0, xcd, k); \
+            k_fused_thermal<TX_, TY_, NT_, false, false><<<grid, NT_, 0, s>>>(io, nx, ny, th, sm, xcd, k); \
         break;                                                                               \
     }
         HG_THERMAL_VARIANTS(X)

@@ -105,6 +105,7 @@ void update_kparams(hg_ctx* c) {
     k.rr_max_h = c->map_max_height;
     k.seed_lo = uint32_t(c->seed);
     k.seed_hi = uint32_t(c->seed >> 32);
+    k.store_nt = hgk::stream_stores(2 * HG_NUM_STATE_PLANES * hgk::plane_stride_bytes(size_t(c->n) * size_t(c->n)));
 }
 
 int stat_index(hg_ctx* c, const char* name) {
