@@ -44,6 +44,8 @@ def _reset(c1):
 
 
 def test_fused_equals_per_pass_4096(c1):
+    """At this size the fused kernels write their outputs with non-temporal
+    stores and the per-pass kernels with ordinary ones: same bits."""
     t, state = _reset(c1)
     print("fused 2 steps", flush=True)
     t.dispatch_grid(2)
