@@ -1,7 +1,7 @@
-// hg_init.hip — config heightmap initialisation: seeded-permutation 2-D
-// simplex fBm, min-max rescaled into the rock layer.  This replaces the
-// reference's World::gen_heightmap dispatch (src/state.cpp:116-147) for the
-// configs' simplex-fBm terrains.
+// hg_init.hip — simplex-fBm heightmap initialisation: seeded-permutation
+// 2-D simplex fBm, min-max rescaled into the rock layer.  This replaces the
+// reference's World::gen_heightmap dispatch (src/state.cpp:116-147) for
+// terrains defined as "simplex fBm with a seeded permutation".
 //
 // Noise definition (shared with the CPU oracle, written independently there):
 //  * permutation of 0..255 from a Philox4x32-10 driven Fisher-Yates shuffle

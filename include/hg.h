@@ -282,10 +282,11 @@ int hg_reset_timing(hg_ctx* ctx);
 
 /* ---- particle (droplet) erosion ------------------------------------------ */
 
-/* Droplet model parameters (the configs' Beyer-style droplets; the
+/* Droplet model parameters (a mass-conserving Beyer-style model; the
  * reference's own particle mode, glsl/particle.glsl, is a different model
- * that does not conserve mass).  Terrain height is rock + dirt; erosion takes
- * dirt first, then rock; deposits go to dirt. */
+ * that does not conserve mass, and is hg_step_particle_reference below).
+ * Terrain height is rock + dirt; erosion takes dirt first, then rock;
+ * deposits go to dirt. */
 typedef struct hg_droplet_settings {
     uint32_t count;          /* droplets per iteration */
     uint32_t max_lifetime;   /* steps per droplet */

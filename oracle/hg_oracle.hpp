@@ -124,8 +124,8 @@ inline double rain_uniform_u(uint64_t seed, uint64_t step, uint32_t terrain, int
 }
 
 // ---------------------------------------------------------------------------
-// Config heightmap init: seeded-permutation 2-D simplex fBm (this project's
-// own init for the configs' "simplex fBm (seeded permutation)"; the reference
+// Simplex-fBm heightmap init: seeded-permutation 2-D simplex fBm (this
+// project's own init for "simplex fBm (seeded permutation)" terrains; the reference
 // init is glsl/heightmap.glsl:52-158 and is a different noise).
 //  * permutation: Fisher-Yates over 0..255 driven by Philox4x32-10 with
 //    key = seed, ctr = {i, 0x5eed, 0, 0}, j = word0 % (i+1), i = 255..1;
@@ -810,7 +810,7 @@ void grid_step(State<T>& s, const Settings<T>& k, const RainCfg& rc, uint64_t st
 }
 
 // ---------------------------------------------------------------------------
-// Droplet (particle) erosion, the configs' Beyer-style model (H. Beyer,
+// Droplet (particle) erosion, a mass-conserving Beyer-style model (H. Beyer,
 // "Implementation of a method for hydraulic erosion", 2015), restated here as
 // a serial FP64 model: droplets are simulated one after another, in id order.
 // The reference's own particle mode (glsl/particle.glsl:64-136,

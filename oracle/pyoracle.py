@@ -168,7 +168,7 @@ def ref_particle_rand(x: float, y: float, dtype=np.float32) -> float:
 
 def config_settings(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01, talus_deg=35.0, slippage=0.1,
                     G=9.81, dt=0.005, Kconv=0.001, E=1.0) -> ErosionSettings:
-    """The configs' scalar coefficients broadcast to both layers; the unnamed
+    """Scalar coefficients (one per process) broadcast to both layers; the unnamed
     fields take the reference grid defaults (src/state.cpp:81-92)."""
     s = ErosionSettings()
     s.particle_count = 0
