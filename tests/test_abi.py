@@ -81,3 +81,16 @@ def test_create_without_device_fails_cleanly():
         assert e.code in (hg.HG_ERR_HIP, hg.HG_ERR_INVALID)
     else:
         raise AssertionError("hg_create succeeded without a device")
+
+
+def test_missing_library_raises_instead_of_falling_back(monkeypatch, tmp_path):
+    """Without the built HIP library the binding raises: there is no CPU path."""
+    from hydrogen import _abi
+    monkeypatch.setattr(_abi, "_lib", None)
+    monkeypatch.setattr(_abi, "LIB_PATH", str(tmp_path / "libhydrogen.so"))
+    try:
+        _abi.load()
+    except RuntimeError as e:
+        assert "missing" in str(e)
+    else:
+        raise AssertionError("load() succeeded without the library")
