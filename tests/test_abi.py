@@ -7,6 +7,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import hydrogen as hg
 import pyoracle as O
 
@@ -94,3 +96,17 @@ def test_missing_library_raises_instead_of_falling_back(monkeypatch, tmp_path):
         assert "missing" in str(e)
     else:
         raise AssertionError("load() succeeded without the library")
+
+
+@pytest.mark.parametrize("lang,std,cc", [("c", "c11", "gcc"), ("c++", "c++17", "g++")])
+def test_header_is_plain_c_and_cxx(lang, std, cc, tmp_path):
+    """include/hg.h compiles on its own as C11 and as C++17, warnings as errors:
+    the boundary is a C ABI with no C++ or HIP types."""
+    import shutil
+    import subprocess
+    if shutil.which(cc) is None:
+        pytest.skip(f"{cc} not installed")
+    hdr = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "hg.h")
+    r = subprocess.run([cc, f"-std={std}", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-x", lang, hdr],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
