@@ -1124,7 +1124,10 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
         HG_HIP(c, hipEventRecord(ev_rev[b], c->stream));   // plane set b is free again
     }
     c->step = step_end;
-    HG_HIP(c, hipMemcpyAsync(dL_dH0, lam[HG_P_ROCK], pn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    {
+        const int rc = read_pinned(c, lam[HG_P_ROCK], dL_dH0, pn);
+        if (rc != HG_OK) return rc;
+    }
     std::vector<double> part(nblk * 4);
     HG_HIP(c, hipMemcpyAsync(part.data(), w.gpart, nblk * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HG_HIP(c, hipStreamSynchronize(c->stream));

@@ -65,6 +65,11 @@ struct hg_ctx {
     std::vector<PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     std::vector<float> host_tmp;
+    // pinned staging for device-to-host reads (two chunks, allocated on the
+    // first read): copies into pageable memory straight from the device ran
+    // at ~5 GB/s
+    void* pin[2] = {};
+    hipEvent_t pin_ev[2] = {};
 };
 
 namespace hgctx {
@@ -77,6 +82,9 @@ int stat_index(hg_ctx* c, const char* name);
 hipEvent_t take_event(hg_ctx* c);
 int run_pass(hg_ctx* c, int pass);
 int fused_step(hg_ctx* c, bool rain, bool thermal, bool smooth, bool keep_velocity);
+// n_floats from device `src` to host `dst` through the context's pinned chunks
+// (ordered after the work on the context's stream; returns when dst is filled)
+int read_pinned(hg_ctx* c, const float* src, float* dst, size_t n_floats);
 
 // Launch helper: brackets the launch with events when timing is on.
 template <class F>

@@ -9,6 +9,7 @@
 // per plane.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -224,6 +225,40 @@ bool finite_settings(const hg_erosion_settings* s) {
     return true;
 }
 
+// Device-to-host copy through the context's two pinned chunks: the copy of
+// chunk i+1 runs while chunk i is copied out to `dst`.
+constexpr size_t kPinChunk = size_t(16) << 20;
+int read_pinned(hg_ctx* c, const float* src, float* dst, size_t n_floats) {
+    if (!c->pin[0]) {
+        for (int b = 0; b < 2; ++b) {
+            HG_HIP(c, hipHostMalloc(&c->pin[b], kPinChunk, hipHostMallocDefault));
+            HG_HIP(c, hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming));
+        }
+    }
+    const size_t bytes = n_floats * sizeof(float);
+    const char* s = reinterpret_cast<const char*>(src);
+    char* d = reinterpret_cast<char*>(dst);
+    auto issue = [&](size_t off, int b) -> int {
+        const size_t len = std::min(kPinChunk, bytes - off);
+        HG_HIP(c, hipMemcpyAsync(c->pin[b], s + off, len, hipMemcpyDeviceToHost, c->stream));
+        HG_HIP(c, hipEventRecord(c->pin_ev[b], c->stream));
+        return HG_OK;
+    };
+    if (bytes == 0) return HG_OK;
+    int rc = issue(0, 0);
+    if (rc != HG_OK) return rc;
+    for (size_t off = 0, k = 0; off < bytes; off += kPinChunk, ++k) {
+        const int b = int(k & 1);
+        if (off + kPinChunk < bytes) {           // the other chunk was copied out last round
+            rc = issue(off + kPinChunk, 1 - b);
+            if (rc != HG_OK) return rc;
+        }
+        HG_HIP(c, hipEventSynchronize(c->pin_ev[b]));
+        std::memcpy(d + off, c->pin[b], std::min(kPinChunk, bytes - off));
+    }
+    return HG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,6 +350,10 @@ int hg_destroy(hg_ctx* c) {
     for (hipEvent_t e : c->adj_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->adj_block) (void)hipFree(c->adj_block);
+    for (int b = 0; b < 2; ++b) {
+        if (c->pin[b]) (void)hipHostFree(c->pin[b]);
+        if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HG_OK;
@@ -663,9 +702,7 @@ int hg_read_plane(hg_ctx* c, int plane, float* dst, size_t n_floats) {
     if (plane < 0 || plane >= HG_NUM_PLANES) return fail(c, HG_ERR_INVALID, "hg_read_plane: bad plane");
     if (!dst || n_floats != plane_n(c)) return fail(c, HG_ERR_INVALID, "hg_read_plane: bad buffer size");
     DeviceGuard guard(c->device);
-    HG_HIP(c, hipMemcpyAsync(dst, c->cur[plane], n_floats * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HG_HIP(c, hipStreamSynchronize(c->stream));
-    return HG_OK;
+    return read_pinned(c, c->cur[plane], dst, n_floats);
 }
 
 int hg_write_plane(hg_ctx* c, int plane, const float* src, size_t n_floats) {

@@ -432,8 +432,10 @@ class Terrain:
         self._check(self.lib.hg_sync(self.h))
 
     # -- state I/O --
-    def read_plane(self, p: int) -> np.ndarray:
-        a = np.empty((self.n, self.n), dtype=np.float32)
+    def read_plane(self, p: int, out: np.ndarray | None = None) -> np.ndarray:
+        """Plane p (hg_read_plane), into `out` (contiguous float32 n x n) if given."""
+        a = np.empty((self.n, self.n), dtype=np.float32) if out is None else out
+        assert a.dtype == np.float32 and a.shape == (self.n, self.n) and a.flags.c_contiguous
         self._check(self.lib.hg_read_plane(self.h, p, _fptr(a), a.size))
         return a
 
@@ -453,7 +455,10 @@ class Terrain:
         self._check(self.lib.hg_write(self.h, ch, _fptr(a), a.size))
 
     def read_state(self, nplanes: int = NUM_PLANES) -> np.ndarray:
-        return np.stack([self.read_plane(p) for p in range(nplanes)])
+        st = np.empty((nplanes, self.n, self.n), dtype=np.float32)
+        for p in range(nplanes):
+            self.read_plane(p, st[p])
+        return st
 
     def write_state(self, st: np.ndarray):
         for p in range(st.shape[0]):
