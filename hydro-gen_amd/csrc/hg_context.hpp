@@ -70,6 +70,7 @@ struct hg_ctx {
     // at ~5 GB/s
     void* pin[2] = {};
     hipEvent_t pin_ev[2] = {};
+    float* rgba_dev = nullptr;       // 4 n^2 floats for hg_read's texels (allocated on first use)
 };
 
 namespace hgctx {

@@ -331,6 +331,26 @@ hipError_t launch_gen_heightmap(float* rock, float* dirt, int n, const hg_map_se
     return hipGetLastError();
 }
 
+// RGBA32F texel i = (a[i], b[i], c[i], d[i]) for channel reads; a null plane
+// reads 0, and with `total` the fourth component is (a + b) + c, the HEIGHT
+// channel's total height.
+__global__ __launch_bounds__(256) void k_interleave_rgba(float4* out, const float* a, const float* b,
+                                                         const float* c, const float* d, size_t n, int total) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+        const float x = a ? a[i] : 0.0f, y = b ? b[i] : 0.0f, z = c ? c[i] : 0.0f;
+        const float w = total ? (x + y) + z : (d ? d[i] : 0.0f);
+        out[i] = make_float4(x, y, z, w);
+    }
+}
+
+hipError_t launch_interleave_rgba(float* out, const float* const planes[4], size_t n, bool total, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 8192));
+    k_interleave_rgba<<<grid, 256, 0, s>>>(reinterpret_cast<float4*>(out), planes[0], planes[1], planes[2],
+                                           planes[3], n, total ? 1 : 0);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(float* p, size_t n, float value, hipStream_t s) {
     if (n == 0) return hipSuccess;
     k_fill<<<unsigned((n + 255) / 256), 256, 0, s>>>(p, n, value);

@@ -350,6 +350,7 @@ int hg_destroy(hg_ctx* c) {
     for (hipEvent_t e : c->adj_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->adj_block) (void)hipFree(c->adj_block);
+    if (c->rgba_dev) (void)hipFree(c->rgba_dev);
     for (int b = 0; b < 2; ++b) {
         if (c->pin[b]) (void)hipHostFree(c->pin[b]);
         if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
@@ -727,17 +728,13 @@ int hg_read(hg_ctx* c, int channel, float* rgba, size_t n_floats) {
         case HG_CH_SEDIMENT: planes[0] = HG_P_SED_ROCK; planes[1] = HG_P_SED_DIRT; break;
         default: return fail(c, HG_ERR_INVALID, "hg_read: bad channel");
     }
-    c->host_tmp.resize(4 * pn);
-    for (int k = 0; k < 4; ++k) {
-        if (planes[k] < 0) continue;
-        const int rc = hg_read_plane(c, planes[k], c->host_tmp.data() + k * pn, pn);
-        if (rc != HG_OK) return rc;
-    }
-    for (size_t i = 0; i < pn; ++i)
-        for (int k = 0; k < 4; ++k) rgba[4 * i + k] = planes[k] < 0 ? 0.0f : c->host_tmp[k * pn + i];
-    if (channel == HG_CH_HEIGHT)
-        for (size_t i = 0; i < pn; ++i) rgba[4 * i + 3] = rgba[4 * i] + rgba[4 * i + 1] + rgba[4 * i + 2];
-    return HG_OK;
+    // texels assembled on the device, then one copy through the pinned chunks
+    DeviceGuard guard(c->device);
+    if (!c->rgba_dev) HG_HIP(c, hipMalloc(&c->rgba_dev, 4 * pn * sizeof(float)));
+    const float* src[4];
+    for (int k = 0; k < 4; ++k) src[k] = planes[k] < 0 ? nullptr : c->cur[planes[k]];
+    HG_HIP(c, hgk::launch_interleave_rgba(c->rgba_dev, src, pn, channel == HG_CH_HEIGHT, c->stream));
+    return read_pinned(c, c->rgba_dev, rgba, 4 * pn);
 }
 
 // ---- heightmap files: NumPy .npy v1.0, little-endian float32, shape (n, n, 4)
