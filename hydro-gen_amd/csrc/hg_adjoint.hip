@@ -904,7 +904,9 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
             c->adj_block = nullptr;
             c->adj_bytes = 0;
         }
-        HG_HIP(c, hipMalloc(&c->adj_block, need));
+        void* p = nullptr;
+        HG_HIP(c, hipMalloc(&p, need));
+        c->adj_block = p;
         c->adj_bytes = need;
     }
     w.block = c->adj_block;
@@ -1001,9 +1003,16 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     // the last) uses plane set i % 2; its recompute waits for the reversal that
     // last used that set, and its reversal waits for its recompute.
     if (!c->adj_stream) {
-        HG_HIP(c, hipStreamCreateWithFlags(&c->adj_stream, hipStreamNonBlocking));
-        for (hipEvent_t& e : c->adj_ev) HG_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        hipStream_t st = nullptr;
+        HG_HIP(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        c->adj_stream = st;
     }
+    for (hipEvent_t& e : c->adj_ev)
+        if (!e) {
+            hipEvent_t ev = nullptr;
+            HG_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            e = ev;
+        }
     hipStream_t rs = c->adj_stream;
     hipEvent_t ev_fwd = c->adj_ev[0], ev_rec[2] = {c->adj_ev[1], c->adj_ev[2]},
                ev_rev[2] = {c->adj_ev[3], c->adj_ev[4]};

@@ -229,10 +229,16 @@ bool finite_settings(const hg_erosion_settings* s) {
 // chunk i+1 runs while chunk i is copied out to `dst`.
 constexpr size_t kPinChunk = size_t(16) << 20;
 int read_pinned(hg_ctx* c, const float* src, float* dst, size_t n_floats) {
-    if (!c->pin[0]) {
-        for (int b = 0; b < 2; ++b) {
-            HG_HIP(c, hipHostMalloc(&c->pin[b], kPinChunk, hipHostMallocDefault));
-            HG_HIP(c, hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming));
+    for (int b = 0; b < 2; ++b) {            // each piece stored only once it exists
+        if (!c->pin[b]) {
+            void* p = nullptr;
+            HG_HIP(c, hipHostMalloc(&p, kPinChunk, hipHostMallocDefault));
+            c->pin[b] = p;
+        }
+        if (!c->pin_ev[b]) {
+            hipEvent_t e = nullptr;
+            HG_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            c->pin_ev[b] = e;
         }
     }
     const size_t bytes = n_floats * sizeof(float);
@@ -730,7 +736,11 @@ int hg_read(hg_ctx* c, int channel, float* rgba, size_t n_floats) {
     }
     // texels assembled on the device, then one copy through the pinned chunks
     DeviceGuard guard(c->device);
-    if (!c->rgba_dev) HG_HIP(c, hipMalloc(&c->rgba_dev, 4 * pn * sizeof(float)));
+    if (!c->rgba_dev) {
+        void* p = nullptr;
+        HG_HIP(c, hipMalloc(&p, 4 * pn * sizeof(float)));
+        c->rgba_dev = static_cast<float*>(p);
+    }
     const float* src[4];
     for (int k = 0; k < 4; ++k) src[k] = planes[k] < 0 ? nullptr : c->cur[planes[k]];
     HG_HIP(c, hgk::launch_interleave_rgba(c->rgba_dev, src, pn, channel == HG_CH_HEIGHT, c->stream));
