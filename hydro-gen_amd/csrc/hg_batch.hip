@@ -150,8 +150,14 @@ int build_tiles(hg_batch* b) {
     if (b->dev_tiles_h) (void)hipFree(b->dev_tiles_h);
     if (b->dev_tiles_t) (void)hipFree(b->dev_tiles_t);
     b->dev_tiles_h = b->dev_tiles_t = nullptr;
-    BH(b, hipMalloc(&b->dev_tiles_h, lh.size() * sizeof(uint32_t)));
-    BH(b, hipMalloc(&b->dev_tiles_t, lt.size() * sizeof(uint32_t)));
+    {
+        void* ph = nullptr;
+        void* pt = nullptr;
+        BH(b, hipMalloc(&ph, lh.size() * sizeof(uint32_t)));
+        b->dev_tiles_h = static_cast<uint32_t*>(ph);
+        BH(b, hipMalloc(&pt, lt.size() * sizeof(uint32_t)));
+        b->dev_tiles_t = static_cast<uint32_t*>(pt);
+    }
     BH(b, hipMemcpy(b->dev_tiles_h, lh.data(), lh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     BH(b, hipMemcpy(b->dev_tiles_t, lt.data(), lt.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     b->ntiles_h = int(lh.size());
@@ -206,8 +212,11 @@ int per_terrain_steps(hg_batch* b, uint32_t nsteps, uint32_t flags, bool per_pas
     int maxn = 0;
     for (int v : b->n) maxn = std::max(maxn, v);
     const size_t spf = pitch_floats(maxn);
-    if (per_pass && !b->scratch)
-        BH(b, hipMalloc(&b->scratch, spf * sizeof(float) * 2 * (HG_NUM_PLANES - HG_NUM_STATE_PLANES)));
+    if (per_pass && !b->scratch) {
+        void* p = nullptr;
+        BH(b, hipMalloc(&p, spf * sizeof(float) * 2 * (HG_NUM_PLANES - HG_NUM_STATE_PLANES)));
+        b->scratch = p;
+    }
     for (uint32_t s = 0; s < nsteps; ++s) {
         const uint32_t step = uint32_t(b->step + s);
         uint32_t cur_after = b->cur;

@@ -585,7 +585,11 @@ int hg_ref_particles_init(hg_ctx* c, uint32_t count) {
         c->refp = nullptr;
         c->refp_count = 0;
     }
-    HG_HIP(c, hipMalloc(&c->refp, size_t(count) * sizeof(hg_ref_particle)));
+    {
+        void* p = nullptr;
+        HG_HIP(c, hipMalloc(&p, size_t(count) * sizeof(hg_ref_particle)));
+        c->refp = static_cast<hg_ref_particle*>(p);
+    }
     c->refp_count = count;
     HG_HIP(c, hipMemsetAsync(c->refp, 0, size_t(count) * sizeof(hg_ref_particle), c->stream));
     const size_t pn = plane_n(c);
