@@ -64,7 +64,6 @@ struct hg_ctx {
     std::vector<KernelStat> stats;
     std::vector<PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
-    std::vector<float> host_tmp;
     // pinned staging for device-to-host reads (two chunks, allocated on the
     // first read): copies into pageable memory straight from the device ran
     // at ~5 GB/s

@@ -351,6 +351,26 @@ hipError_t launch_interleave_rgba(float* out, const float* const planes[4], size
     return hipGetLastError();
 }
 
+// The inverse for channel writes: plane k (when not null) = component k.
+__global__ __launch_bounds__(256) void k_deinterleave_rgba(const float4* in, float* a, float* b, float* c,
+                                                           float* d, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+        const float4 v = in[i];
+        if (a) a[i] = v.x;
+        if (b) b[i] = v.y;
+        if (c) c[i] = v.z;
+        if (d) d[i] = v.w;
+    }
+}
+
+hipError_t launch_deinterleave_rgba(const float* in, float* const planes[4], size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 8192));
+    k_deinterleave_rgba<<<grid, 256, 0, s>>>(reinterpret_cast<const float4*>(in), planes[0], planes[1], planes[2],
+                                             planes[3], n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(float* p, size_t n, float value, hipStream_t s) {
     if (n == 0) return hipSuccess;
     k_fill<<<unsigned((n + 255) / 256), 256, 0, s>>>(p, n, value);

@@ -131,6 +131,8 @@ hipError_t launch_fill(float* p, size_t n, float value, hipStream_t s);
 // out (4n floats, 16-byte aligned) = RGBA texels of the 4 planes (null: 0);
 // `total`: the fourth component is (p0 + p1) + p2
 hipError_t launch_interleave_rgba(float* out, const float* const planes[4], size_t n, bool total, hipStream_t s);
+// planes[k] (when not null) = component k of the texels in `in` (4n floats)
+hipError_t launch_deinterleave_rgba(const float* in, float* const planes[4], size_t n, hipStream_t s);
 hipError_t launch_gen_heightmap(float* rock, float* dirt, int n, const hg_map_settings& m, hipStream_t s);
 
 }  // namespace hgk

@@ -818,13 +818,18 @@ int hg_write(hg_ctx* c, int channel, const float* rgba, size_t n_floats) {
         case HG_CH_SEDIMENT: planes[0] = HG_P_SED_ROCK; planes[1] = HG_P_SED_DIRT; break;
         default: return fail(c, HG_ERR_INVALID, "hg_write: bad channel");
     }
-    c->host_tmp.resize(pn);
-    for (int k = 0; k < 4; ++k) {
-        if (planes[k] < 0) continue;
-        for (size_t i = 0; i < pn; ++i) c->host_tmp[i] = rgba[4 * i + k];
-        const int rc = hg_write_plane(c, planes[k], c->host_tmp.data(), pn);
-        if (rc != HG_OK) return rc;
+    // texels to the device, split into planes there
+    DeviceGuard guard(c->device);
+    if (!c->rgba_dev) {
+        void* p = nullptr;
+        HG_HIP(c, hipMalloc(&p, 4 * pn * sizeof(float)));
+        c->rgba_dev = static_cast<float*>(p);
     }
+    HG_HIP(c, hipMemcpyAsync(c->rgba_dev, rgba, 4 * pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    float* dst[4];
+    for (int k = 0; k < 4; ++k) dst[k] = planes[k] < 0 ? nullptr : c->cur[planes[k]];
+    HG_HIP(c, hgk::launch_deinterleave_rgba(c->rgba_dev, dst, pn, c->stream));
+    HG_HIP(c, hipStreamSynchronize(c->stream));
     return HG_OK;
 }
 
