@@ -443,8 +443,8 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
 // ---- sediment_transport --------------------------------------------------
 // The bilinear gather's adjoint is a scatter to the four taps of each cell's
 // back-trace.  A cell whose taps lie within one cell of it ("near") is
-// gathered by the taps themselves (k_adj_transport); the few others scatter
-// with atomics afterwards (k_adj_transport_far).
+// gathered by the taps themselves (k_adj_transport); the others scatter with
+// atomics afterwards (k_adj_transport_far).
 struct Taps {
     int ix, iy, ix1, iy1;
     float w00, w10, w01, w11;
