@@ -53,6 +53,7 @@ struct hg_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t graph_steps = 0;
+    uint32_t graph_cur = 0;             // buffer selection baked into the graph's launches
     std::string err;
 };
 
@@ -565,12 +566,19 @@ int hg_batch_capture(hg_batch* b, uint32_t nsteps, uint32_t flags) {
     b->graph = graph;
     BH(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->graph_steps = nsteps;
+    b->graph_cur = cur0;
     return HG_OK;
 }
 
 int hg_batch_run(hg_batch* b) {
     if (!b) return HG_ERR_INVALID;
     if (!b->exec) return bfail(b, HG_ERR_STATE, "hg_batch_run: nothing captured (or settings changed since)");
+    // The captured launches name the plane buffers that were current at
+    // capture time.  An odd number of eager steps since then swapped them.
+    if (b->cur != b->graph_cur)
+        return bfail(b, HG_ERR_STATE,
+                     "hg_batch_run: the buffers were swapped by an odd number of hg_batch_step calls since "
+                     "hg_batch_capture; step once more or capture again");
     Guard g(b->device);
     BH(b, hipGraphLaunch(b->exec, b->stream));
     b->step += b->graph_steps;   // buffers are back at their start: graph_steps is even

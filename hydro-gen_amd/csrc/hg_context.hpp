@@ -50,6 +50,7 @@ struct hg_ctx {
     float map_max_height = 256.0f;   // HG_RAIN_REFERENCE mountain scale (src/settings.hpp:30)
     hg_ref_particle* refp = nullptr; // the reference particle mode's particles (hg_ref_particles_init)
     uint32_t refp_count = 0;
+    int refp_dims[2] = {0, 0};       // hmap_dims of the reference particle mode (0 = n)
     unsigned long long* dev_giveups = nullptr;  // in the block's tail
     // hg_adjoint_rollout's workspace, recompute stream (default priority) and
     // its events, kept for the next call until hg_destroy

@@ -110,6 +110,7 @@ hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams&
 struct RefParticleParams {
     float Kc, Kconv, Ks[2], Kd[2], Ke, G, dt, init_volume, friction, inertia, min_volume, min_velocity;
     uint32_t ttl, particle_count;
+    int dims_x, dims_y;       // hmap_dims of the spawn range and border test
 };
 hipError_t launch_ref_particle_move(hg_ref_particle* parts, uint32_t count, const float* rock, const float* dirt,
                                     const float* water, const float* mx, const float* my, int n,

@@ -3,10 +3,13 @@
 // One thread per droplet; every droplet of an iteration runs concurrently and
 // changes the height planes with float atomic adds (global_atomic_add_f32),
 // so a droplet sees the others' changes in whatever order the memory system
-// applies them.  Height reads are plain loads.  Mass bookkeeping: every amount
-// taken from the map is added to the droplet's load and every amount deposited
-// is subtracted from it; sediment carried over the map edge is summed in
-// `exported`, so sum(rock + dirt) + exported is conserved up to FP32 rounding.
+// applies them.  Height reads are relaxed atomic loads.  Erosion takes from a
+// cell with a returning atomic add and gives back what was not there (take()),
+// so concurrent droplets never drive a layer below 0.  Mass bookkeeping: every
+// amount taken from the map is added to the droplet's load and every amount
+// deposited is subtracted from it; sediment carried over the map edge is
+// summed in `exported`, so sum(rock + dirt) + exported is conserved up to
+// FP32 rounding.
 //
 // Model (Beyer-style droplets; the reference's particle mode,
 // glsl/particle.glsl:64-136 and glsl/particle_erosion.glsl:22-128, is a
@@ -28,6 +31,22 @@ __device__ __forceinline__ float ld(const float* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ float height(const float* R, const float* D, size_t i) { return ld(R + i) + ld(D + i); }
+
+// Removes up to `want` from *p, never more than it holds, and returns what
+// was removed.  The amount a relaxed load says is there is subtracted with a
+// returning atomic add; if other droplets took from the cell in between, the
+// height it replaced is smaller, and the part that was not there is added
+// back.  Each take is then at most max(0, what the cell held at its atomic
+// add, less the give-backs still outstanding), so the takes of all droplets
+// from one cell never exceed what it held and it never ends below 0.
+__device__ __forceinline__ float take(float* p, float want) {
+    const float est = fminf(fmaxf(ld(p), 0.0f), want);
+    if (!(est > 0.0f)) return 0.0f;
+    const float old = atomicAdd(p, -est);
+    const float got = fminf(fmaxf(old, 0.0f), est);
+    if (got < est) atomic_addf(p, est - got);
+    return got;
+}
 
 // Cell of a position inside [0, n-1]^2; n-1 itself maps to cell n-2 (with
 // fraction 1) so the 2x2 stencil stays in the grid.
@@ -125,10 +144,8 @@ __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* 
                     if (!interior && !(x >= 0 && y >= 0 && x < n && y < n)) continue;
                     const size_t i = size_t(y) * n + x;
                     const float want = scale * brush.w[t];
-                    const float td = fminf(fmaxf(ld(D + i), 0.0f), want);
-                    const float tr = fminf(fmaxf(ld(R + i), 0.0f), want - td);
-                    if (td > 0.0f) atomic_addf(D + i, -td);
-                    if (tr > 0.0f) atomic_addf(R + i, -tr);
+                    const float td = take(D + i, want);
+                    const float tr = take(R + i, want - td);
                     sed += td + tr;
                 }
             }

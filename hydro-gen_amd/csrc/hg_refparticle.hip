@@ -3,7 +3,12 @@
 // One Erosion::dispatch_particle (src/erosion.cpp:132-156) is four launches:
 //  * k_refp_move: one thread per particle, glsl/particle.glsl:64-136 (spawn,
 //    terrain normal and momentum sampled bilinearly, velocity and volume
-//    update, capacity, kill tests);
+//    update, capacity, kill tests).  The spawn range and the border test
+//    use Map_settings_data.hmap_dims, which the reference leaves at
+//    (1024, 1024) whatever the map size; here it is the grid's n unless
+//    hg_set_ref_particle_dims sets it.  Particles outside the grid read 0
+//    and their corner updates are dropped, like imageLoad/imageStore
+//    outside an image;
 //  * k_refp_erode: one thread per particle, glsl/particle_erosion.glsl:22-128,
 //    the 4 corners of the particle's quad in the shader's order;
 //  * the fused thermal kernel (both thermal layers and the grid smoothing of
@@ -76,8 +81,8 @@ __global__ __launch_bounds__(256) void k_refp_move(hg_ref_particle* __restrict__
         if (p.sediment[i] < 0.0f || p.iters == 0) p.sediment[i] = 0.0f;
     if (p.iters == 0 || p.to_kill) {                                             // :71-86
         const float ax = fract_(time * 1.37f) * 1000.0f, ay = fract_(time * 7.21f) * 1000.0f;
-        p.position[0] = fdiv(rp_rand(ax, float(id)) * float(n - 4.0f), 1.0f) + 2.0f;
-        p.position[1] = fdiv(rp_rand(ay, float(id) + 3.14f) * float(n - 4.0f), 1.0f) + 2.0f;
+        p.position[0] = fdiv(rp_rand(ax, float(id)) * float(k.dims_x - 4.0f), 1.0f) + 2.0f;
+        p.position[1] = fdiv(rp_rand(ay, float(id) + 3.14f) * float(k.dims_y - 4.0f), 1.0f) + 2.0f;
         p.to_kill = 0u;
         p.velocity[0] = 0.0f;
         p.velocity[1] = 0.0f;
@@ -115,7 +120,7 @@ __global__ __launch_bounds__(256) void k_refp_move(hg_ref_particle* __restrict__
         vy = fdiv(vy, vlen);
     }
     float qx = px + k.dt * vx, qy = py + k.dt * vy;                              // :113
-    if (qx <= 1.0f || qy <= 1.0f || qx * 1.0f >= float(n - 2) || qy * 1.0f >= float(n - 2)) {   // :114-124
+    if (qx <= 1.0f || qy <= 1.0f || qx * 1.0f >= float(k.dims_x - 2) || qy * 1.0f >= float(k.dims_y - 2)) {   // :114-124
         qx = px;
         qy = py;
         vx = 0.0f;

@@ -599,6 +599,17 @@ int hg_ref_particles_init(hg_ctx* c, uint32_t count) {
     return HG_OK;
 }
 
+int hg_set_ref_particle_dims(hg_ctx* c, int32_t dims_x, int32_t dims_y) {
+    if (!c) return HG_ERR_INVALID;
+    const bool both_zero = dims_x == 0 && dims_y == 0;
+    const bool in_range = dims_x >= 8 && dims_y >= 8 && dims_x <= 65536 && dims_y <= 65536;
+    if (!both_zero && !in_range)
+        return fail(c, HG_ERR_INVALID, "hg_set_ref_particle_dims: dims must both be 0 or both in [8, 65536]");
+    c->refp_dims[0] = dims_x;
+    c->refp_dims[1] = dims_y;
+    return HG_OK;
+}
+
 int hg_ref_particles_read(hg_ctx* c, hg_ref_particle* out, uint32_t count) {
     if (!c || !out) return HG_ERR_INVALID;
     if (count != c->refp_count) return fail(c, HG_ERR_INVALID, "hg_ref_particles_read: count differs from hg_ref_particles_init");
@@ -636,6 +647,8 @@ int hg_step_particle_reference(hg_ctx* c, uint32_t iterations, int should_rain, 
     k.particle_count = e.particle_count;
     const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
     const int n = c->n;
+    k.dims_x = c->refp_dims[0] > 0 ? c->refp_dims[0] : n;
+    k.dims_y = c->refp_dims[1] > 0 ? c->refp_dims[1] : n;
     for (uint32_t it = 0; it < iterations; ++it) {
         const float time = time0 + float(it) * time_step;
         int rc = timed_launch(c, "ref_particle_move", [&] {

@@ -38,7 +38,7 @@ EXPORTS = [
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
     "hg_ref_particles_init", "hg_ref_particles_read", "hg_ref_particles_write", "hg_step_particle_reference",
-    "hg_ref_particle_stats",
+    "hg_ref_particle_stats", "hg_set_ref_particle_dims",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
     "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
     "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles", "hg_batch_autotune",
@@ -255,6 +255,7 @@ def load() -> ctypes.CDLL:
     L.hg_step_particle_reference.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                              ctypes.c_uint32]
     L.hg_ref_particle_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.hg_set_ref_particle_dims.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
     ip = ctypes.POINTER(ctypes.c_int)
     L.hg_tile_variant_count.argtypes = [ctypes.c_int]
     L.hg_tile_variant_info.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
@@ -387,6 +388,12 @@ class Terrain:
         """`count` zeroed particles and a zeroed momentum map (hg_ref_particles_init)."""
         self._check(self.lib.hg_ref_particles_init(self.h, count))
         self._refp_count = count
+
+    def set_ref_particle_dims(self, dims_x: int = 0, dims_y: int = 0):
+        """Map_settings_data.hmap_dims of the particle spawn range and border
+        test (hg_set_ref_particle_dims); the reference keeps (1024, 1024) on
+        every map size, (0, 0) uses the grid's n."""
+        self._check(self.lib.hg_set_ref_particle_dims(self.h, dims_x, dims_y))
 
     def read_ref_particles(self):
         arr = (RefParticle * self._refp_count)()

@@ -180,7 +180,9 @@ enum {
 
 /* Replaces World::gen_textures + Erosion::setup_shaders (src/state.cpp:3-44,
  * src/erosion.cpp:21-74): allocates the n x n state on HIP device `device`
- * and a stream.  particle_count > 0 also allocates the droplet buffers.
+ * and a stream.  particle_count only sets the default droplet count of
+ * hg_step_particle (the droplets need no buffers of their own); the
+ * reference particle mode's particle buffer comes from hg_ref_particles_init.
  * The settings start at the reference grid defaults (src/state.cpp:81-92). */
 int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out);
 
@@ -336,6 +338,15 @@ typedef struct hg_ref_particle {
  * The momentum map (the reference's velocity texture in particle mode,
  * rgba) is held in the four FLUX planes: L = x, R = y, T = z, B = w. */
 int hg_ref_particles_init(hg_ctx* ctx, uint32_t count);
+/* Map_settings_data.hmap_dims as glsl/particle.glsl:78-79,111-112 reads it
+ * for the spawn range (2 + rand * (dims - 4)) and the border test
+ * (position >= dims - 2).  The reference leaves it at (1024, 1024) whatever
+ * the map size (src/settings.hpp:34); set (1024, 1024) to keep that quirk on
+ * another grid size.  Particles outside the grid then read heights of 0 and
+ * their corner updates are dropped, like imageLoad / imageStore outside an
+ * image.  (0, 0), the default, uses the grid's n.  Both 0 or both in
+ * [8, 65536]. */
+int hg_set_ref_particle_dims(hg_ctx* ctx, int32_t dims_x, int32_t dims_y);
 int hg_ref_particles_read(hg_ctx* ctx, hg_ref_particle* out, uint32_t count);
 int hg_ref_particles_write(hg_ctx* ctx, const hg_ref_particle* in, uint32_t count);
 /* `iterations` Erosion::dispatch_particle calls (src/erosion.cpp:132-156):
@@ -433,7 +444,11 @@ int hg_batch_set_xcd_order(hg_batch* b, int on);
  * pattern).  All three give identical results. */
 int hg_batch_step(hg_batch* b, uint32_t nsteps, uint32_t flags);
 /* Records nsteps (even) persistent steps into a hipGraph; hg_batch_run
- * replays it, continuing the step counter (kept in device memory). */
+ * replays it, continuing the step counter (kept in device memory).  The
+ * graph's launches name the plane buffers current at capture time: after an
+ * odd number of eager hg_batch_step steps (which swap them) hg_batch_run
+ * returns HG_ERR_STATE until another step restores them or a new capture
+ * replaces the graph. */
 int hg_batch_capture(hg_batch* b, uint32_t nsteps, uint32_t flags);
 int hg_batch_run(hg_batch* b);
 int hg_batch_sync(hg_batch* b);

@@ -968,6 +968,9 @@ template <class T> struct RefParticle {
 template <class T> struct RefParticleParams {
     uint32_t particle_count = 0, ttl = 0;
     T Kc, Kconv, Ks[2], Kd[2], Ke, G, dt, init_volume, friction, inertia, min_volume, min_velocity;
+    // Map_settings_data.hmap_dims (glsl/bindings.glsl:75), which the shader
+    // reads for its spawn range and border test; 0 = the grid's n.
+    int dims_x = 0, dims_y = 0;
 };
 
 template <class T> inline T glsl_fract_t(T x) { return x - T(std::floor(val(x))); }
@@ -997,14 +1000,15 @@ bool ref_particle_move(RefParticle<T>& pw, uint32_t id, const State<T>& s, const
                        const RefParticleParams<T>& k, T time, bool should_rain) {
     using std::sqrt;
     const int n = s.nx;
+    const int dx_ = k.dims_x > 0 ? k.dims_x : n, dy_ = k.dims_y > 0 ? k.dims_y : n;   // hmap_dims
     RefParticle<T> p = pw;
     if (p.iters == 0 && !should_rain) return false;
     for (int i = 0; i < 2; ++i)
         if (val(p.sed[i]) < 0 || p.iters == 0) p.sed[i] = T(0);
     if (p.iters == 0 || p.to_kill) {
         const T ax = glsl_fract_t(time * T(1.37)) * T(1000.0), ay = glsl_fract_t(time * T(7.21)) * T(1000.0);
-        p.px = ref_particle_rand(ax, T(double(id))) * T(double(n) - 4.0) / T(1.0) + T(2.0);
-        p.py = ref_particle_rand(ay, T(double(id)) + T(3.14)) * T(double(n) - 4.0) / T(1.0) + T(2.0);
+        p.px = ref_particle_rand(ax, T(double(id))) * T(double(dx_) - 4.0) / T(1.0) + T(2.0);
+        p.py = ref_particle_rand(ay, T(double(id)) + T(3.14)) * T(double(dy_) - 4.0) / T(1.0) + T(2.0);
         p.to_kill = false;
         p.vx = T(0); p.vy = T(0);
         p.volume = k.init_volume;
@@ -1044,8 +1048,8 @@ bool ref_particle_move(RefParticle<T>& pw, uint32_t id, const State<T>& s, const
     const T ox = p.px, oy = p.py;
     p.px = p.px + k.dt * p.vx;                                                    // :113
     p.py = p.py + k.dt * p.vy;
-    if (val(p.px) <= 1 || val(p.py) <= 1 || val(p.px * T(1.0)) >= double(n - 2) ||
-        val(p.py * T(1.0)) >= double(n - 2)) {                                     // :114-124
+    if (val(p.px) <= 1 || val(p.py) <= 1 || val(p.px * T(1.0)) >= double(dx_ - 2) ||
+        val(p.py * T(1.0)) >= double(dy_ - 2)) {                                   // :114-124
         p.px = ox; p.py = oy;
         p.vx = T(0); p.vy = T(0);
         p.to_kill = true;

@@ -111,12 +111,15 @@ template <class T> orc::RefParticleParams<T> ref_particle_params(const hg_erosio
 // layout (converted to T for the run and rounded back to FP32 at the end).
 template <class T>
 int ref_particle_run(T* planes, int n, hg_ref_particle* parts, uint32_t count, const hg_erosion_settings* s,
-                     uint32_t iterations, int should_rain, float time0, float time_step, unsigned flags) {
+                     uint32_t iterations, int should_rain, float time0, float time_step, unsigned flags,
+                     int dims_x, int dims_y) {
     if (!planes || !s || !parts || n < 8) return -1;
     orc::State<T> st;
     load(st, planes, n, n);
     const orc::Settings<T> k = to_settings<T>(s);
-    const orc::RefParticleParams<T> pk = ref_particle_params<T>(s);
+    orc::RefParticleParams<T> pk = ref_particle_params<T>(s);
+    pk.dims_x = dims_x;
+    pk.dims_y = dims_y;
     std::vector<orc::RefParticle<T>> ps(count);
     for (uint32_t i = 0; i < count; ++i) {
         const hg_ref_particle& q = parts[i];
@@ -314,13 +317,16 @@ int orc_rollout_loss_f64(const double* planes, int nx, int ny, const hg_erosion_
 
 // Dual-number FP64 rollout: the loss and its derivatives with respect to
 // (Kc, Ks, Kd, Ke) and to the initial rock height at `ncells` cells
-// (grad[0..3], then grad[4 + i] for cells[i]).
-int orc_rollout_grad_dual(const double* planes, int nx, int ny, const hg_erosion_settings* s,
-                          const double coef[4], int rain_mode, float rain_amount, uint64_t seed,
-                          int nsteps, unsigned flags, const double* target, const int* cells,
-                          int ncells, double* loss, double* grad) {
-    using D = orc::Dual<DUAL_N>;
-    if (!planes || !s || !coef || !target || !loss || !grad || ncells < 0 || ncells > DUAL_N - 4) return -1;
+// (grad[0..3], then grad[4 + i] for cells[i]).  The dual width is the
+// smallest of 8, 16 and 36 that holds them: every arithmetic operation of
+// the rollout costs one value plus the width in derivatives.
+}  // extern "C"
+template <int N>
+int rollout_grad_dual(const double* planes, int nx, int ny, const hg_erosion_settings* s,
+                      const double coef[4], int rain_mode, float rain_amount, uint64_t seed,
+                      int nsteps, unsigned flags, const double* target, const int* cells,
+                      int ncells, double* loss, double* grad) {
+    using D = orc::Dual<N>;
     orc::State<D> st;
     st.resize(nx, ny);
     const size_t n = size_t(nx) * ny;
@@ -338,18 +344,37 @@ int orc_rollout_grad_dual(const double* planes, int nx, int ny, const hg_erosion
     for (int j = 0; j < 4 + ncells; ++j) grad[j] = L.d[j];
     return 0;
 }
+extern "C" {
+
+int orc_rollout_grad_dual(const double* planes, int nx, int ny, const hg_erosion_settings* s,
+                          const double coef[4], int rain_mode, float rain_amount, uint64_t seed,
+                          int nsteps, unsigned flags, const double* target, const int* cells,
+                          int ncells, double* loss, double* grad) {
+    if (!planes || !s || !coef || !target || !loss || !grad || ncells < 0 || ncells > DUAL_N - 4) return -1;
+    if (ncells <= 4)
+        return rollout_grad_dual<8>(planes, nx, ny, s, coef, rain_mode, rain_amount, seed, nsteps, flags, target,
+                                    cells, ncells, loss, grad);
+    if (ncells <= 12)
+        return rollout_grad_dual<16>(planes, nx, ny, s, coef, rain_mode, rain_amount, seed, nsteps, flags, target,
+                                     cells, ncells, loss, grad);
+    return rollout_grad_dual<DUAL_N>(planes, nx, ny, s, coef, rain_mode, rain_amount, seed, nsteps, flags, target,
+                                     cells, ncells, loss, grad);
+}
 
 
-// The reference's particle mode (orc::ref_particle_dispatch).
+// The reference's particle mode (orc::ref_particle_dispatch); dims_x, dims_y
+// = Map_settings_data.hmap_dims as the shader reads it (0 = n).
 int orc_ref_particle_run_f64(double* planes, int n, hg_ref_particle* parts, uint32_t count,
                              const hg_erosion_settings* s, uint32_t iterations, int should_rain, float time0,
-                             float time_step, unsigned flags) {
-    return ref_particle_run<double>(planes, n, parts, count, s, iterations, should_rain, time0, time_step, flags);
+                             float time_step, unsigned flags, int dims_x, int dims_y) {
+    return ref_particle_run<double>(planes, n, parts, count, s, iterations, should_rain, time0, time_step, flags,
+                                    dims_x, dims_y);
 }
 int orc_ref_particle_run_f32(float* planes, int n, hg_ref_particle* parts, uint32_t count,
                              const hg_erosion_settings* s, uint32_t iterations, int should_rain, float time0,
-                             float time_step, unsigned flags) {
-    return ref_particle_run<float>(planes, n, parts, count, s, iterations, should_rain, time0, time_step, flags);
+                             float time_step, unsigned flags, int dims_x, int dims_y) {
+    return ref_particle_run<float>(planes, n, parts, count, s, iterations, should_rain, time0, time_step, flags,
+                                   dims_x, dims_y);
 }
 float orc_ref_particle_rand_f32(float x, float y) { return orc::ref_particle_rand<float>(x, y); }
 double orc_ref_particle_rand_f64(double x, double y) { return orc::ref_particle_rand<double>(x, y); }

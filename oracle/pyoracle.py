@@ -150,14 +150,16 @@ def reference_particle_defaults(particle_count: int) -> ErosionSettings:
 
 
 def ref_particle_run(planes: np.ndarray, parts, s: ErosionSettings, iterations: int, should_rain: bool,
-                     time0: float, time_step: float, flags: int) -> None:
+                     time0: float, time_step: float, flags: int, dims=(0, 0)) -> None:
     """`iterations` reference particle dispatches in place: planes (NUM_PLANES,
     n, n) float64 or float32 (FLUX planes = momentum map x, y, z, w), parts a
-    ctypes array of RefParticle (rounded to FP32 after the run)."""
+    ctypes array of RefParticle (rounded to FP32 after the run).  `dims` is
+    Map_settings_data.hmap_dims as the shader reads it for the spawn range and
+    border test ((0, 0) = the grid's n)."""
     n = planes.shape[-1]
     fn = lib().orc_ref_particle_run_f64 if planes.dtype == np.float64 else lib().orc_ref_particle_run_f32
     rc = fn(_ptr(planes), n, parts, len(parts), ctypes.byref(s), iterations, int(should_rain), time0, time_step,
-            flags)
+            flags, int(dims[0]), int(dims[1]))
     assert rc == 0
 
 
@@ -234,7 +236,7 @@ def lib():
             getattr(L, f"orc_ref_particle_run_{name}").argtypes = [
                 ctypes.POINTER(t), ctypes.c_int, ctypes.POINTER(RefParticle), ctypes.c_uint32,
                 ctypes.POINTER(ErosionSettings), ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
-                ctypes.c_uint]
+                ctypes.c_uint, ctypes.c_int, ctypes.c_int]
         L.orc_ref_particle_rand_f32.argtypes = [ctypes.c_float, ctypes.c_float]
         L.orc_ref_particle_rand_f32.restype = ctypes.c_float
         L.orc_ref_particle_rand_f64.argtypes = [ctypes.c_double, ctypes.c_double]

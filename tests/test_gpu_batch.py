@@ -110,6 +110,27 @@ def test_graph_replay_matches_stepping(states, persistent):
             assert np.array_equal(got[i][p], eager[i][p]), (i, p, "second replay")
 
 
+def test_graph_replay_after_eager_steps(states):
+    """The captured launches name the buffers current at capture: after an odd
+    number of eager steps the replay is refused, after an even number it
+    continues the step sequence as eager stepping does."""
+    eager = run_mode(states, 4, hg.STEP_DEFAULT)
+    with make_batch(states) as b:
+        b.capture(2)
+        b.step(1)
+        with pytest.raises(hg.HGError) as e:
+            b.run()
+        assert e.value.code == hg.HG_ERR_STATE
+        b.step(1)
+        b.run()
+        b.sync()
+        assert b.step_count == 4
+        got = [b.read_state(i) for i in range(len(SIZES))]
+    for i in range(len(SIZES)):
+        for p in range(hg.NUM_STATE_PLANES):
+            assert np.array_equal(got[i][p], eager[i][p]), (i, p)
+
+
 def test_batched_terrains_match_fp64_oracle(states):
     steps = 6
     got = run_mode(states, steps, hg.STEP_DEFAULT)

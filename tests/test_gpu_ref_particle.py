@@ -156,6 +156,59 @@ def test_spawn_positions_follow_the_hash():
     assert err <= 1e-2 * (n - 4)
 
 
+@pytest.mark.parametrize("dims", [(1024, 1024), (72, 88)])
+def test_hmap_dims_quirk_matches_fp32_oracle_bitwise(dims):
+    """Map_settings_data.hmap_dims bounds the particles (glsl/particle.glsl:111-112)
+    whatever the grid: (1024, 1024) on a 96^2 grid, the reference's own
+    setting, lets particles move on outside the grid (heights read 0 there and
+    their corner updates are dropped); (72, 88) kills them at x >= 70 or
+    y >= 86 inside it.  Separated particles follow the FP32 oracle run with
+    the same dims bit for bit, and differ from a run with dims = n."""
+    n = 96
+    specs = [(10.3, 12.1, 0.2, 0.1), (80.4, 30.2, 0.4, 0.0), (30.6, 91.7, 0.0, 0.3), (93.9, 93.2, 0.3, 0.3),
+             (120.2, 40.5, -0.2, 0.1), (50.1, 60.9, 0.1, -0.2), (69.5, 20.3, 0.5, 0.0), (20.2, 85.6, 0.0, 0.5)]
+    s = _settings(len(specs), d_t=0.05)
+    results = {}
+    for d in (dims, (0, 0)):
+        t, parts = _setup(n, specs, s)
+        t.set_ref_particle_dims(*d)
+        st = _oracle_state(t)
+        t.dispatch_particle_reference(10, should_rain=False, time0=1.0, time_step=0.5, flags=0)
+        got = t.read_ref_particles()
+        gst = t.read_state(hg.NUM_PLANES)
+        t.close()
+        op = _to_oracle(parts)
+        O.ref_particle_run(st, op, _os(s), 10, False, 1.0, 0.5, 0, dims=d)
+        for a, b in zip(got, op):
+            assert _fields(a) == _fields(b), d
+        for k in (hg.P_ROCK, hg.P_DIRT, hg.P_WATER, hg.P_FLUX_T, hg.P_FLUX_B):
+            assert np.array_equal(gst[k], st[k]), (d, k)
+        results[d] = [_fields(a) for a in got]
+    assert results[dims] != results[(0, 0)]
+
+
+def test_hmap_dims_spawn_range():
+    """With the reference's hmap_dims (1024, 1024) on a 128^2 grid, spawns fall
+    in [2, 1022) (glsl/particle.glsl:78-79), most of them outside the grid."""
+    n, count = 128, 4096
+    s = _settings(count)
+    with _terrain(n) as t:
+        t.set_erosion(s)
+        t.ref_particles_init(count)
+        t.set_ref_particle_dims(1024, 1024)
+        t.dispatch_particle_reference(1, should_rain=True, time0=3.21, time_step=0.0, flags=0)
+        got = t.read_ref_particles()
+        with pytest.raises(hg.HGError):
+            t.set_ref_particle_dims(1024, 0)
+        with pytest.raises(hg.HGError):
+            t.set_ref_particle_dims(4, 4)
+    xy = np.array([[p.position[0], p.position[1]] for p in got])
+    print(f"spawn range x {xy[:, 0].min():.2f}..{xy[:, 0].max():.2f}, "
+          f"inside the grid {np.mean((xy < n).all(axis=1)):.3f}", flush=True)
+    assert (xy >= 1.0).all() and (xy < 1022.0).all()
+    assert xy.max() > 900.0 and np.mean((xy < n).all(axis=1)) < 0.1
+
+
 def test_many_particles_distribution_vs_oracle():
     """4096 particles on 128^2 for 10 dispatches with thermal erosion and
     smoothing, from the same random start: heights and water against the

@@ -181,6 +181,50 @@ def test_flux_single_column_by_hand():
     assert out2[O.VX, 2, 1] == pytest.approx(-f * K / vz, rel=1e-12)
 
 
+def test_flux_velocity_uses_old_fluxes_by_hand():
+    """glsl/hydro_flux.glsl:146-158: the velocity reads get_flux() of the pass's
+    input image, so both the cell's own outflow and its neighbours' are the OLD
+    fluxes.  Asymmetric, non-zero old fluxes whose new values differ make the
+    old-flux and new-flux formulas disagree, and only the old one may match."""
+    s, k = settings()
+    dt, G, E = k["dt"], k["G"], k["E"]
+    st = O.new_state(5)
+    st[O.ROCK] = np.arange(25, dtype=np.float64).reshape(5, 5) * 0.01   # a slope, no symmetry
+    st[O.WATER, 1:4, 1:4] = [[0.30, 0.20, 0.10], [0.25, 0.50, 0.05], [0.15, 0.35, 0.45]]
+    old = {O.FL: 0.7, O.FR: 0.3, O.FT: 0.9, O.FB: 0.2}
+    rng = np.random.default_rng(5)
+    for p, v in old.items():
+        st[p, 1:4, 1:4] = v * (1.0 + rng.uniform(-0.5, 0.5, (3, 3)))
+    out = one_pass(st, s, O.PASS_FLUX)
+    y, x = 2, 2
+    F = lambda p, yy, xx: st[p, yy, xx]        # noqa: E731  old flux
+    N = lambda p, yy, xx: out[p, yy, xx]       # noqa: E731  new flux
+    # the new fluxes of the centre, by hand: E*f + dt*G*dw, clamped at 0, then K
+    w = st[O.ROCK] + st[O.DIRT] + st[O.WATER]
+    dw = [w[y, x] - w[y, x - 1], w[y, x] - w[y, x + 1], w[y, x] - w[y + 1, x], w[y, x] - w[y - 1, x]]
+    f = [max(0.0, E * F(p, y, x) + dt * G * d) for p, d in zip((O.FL, O.FR, O.FT, O.FB), dw)]
+    K = min(1.0, st[O.WATER, y, x] / (sum(f) * dt))
+    for p, v in zip((O.FL, O.FR, O.FT, O.FB), f):
+        assert N(p, y, x) == pytest.approx(v * K, rel=1e-12)
+    sum_in = F(O.FR, y, x - 1) + F(O.FL, y, x + 1) + F(O.FB, y + 1, x) + F(O.FT, y - 1, x)
+    d1 = st[O.WATER, y, x]
+    d2 = max(0.0, d1 + dt * (sum_in - K * sum(f)))
+    vz = d1 + d2
+    assert out[O.VZ, y, x] == pytest.approx(vz, rel=1e-12)
+    vx_old = (F(O.FR, y, x - 1) - F(O.FL, y, x) + F(O.FR, y, x) - F(O.FL, y, x + 1)) / vz
+    vy_old = (F(O.FT, y - 1, x) - F(O.FB, y, x) + F(O.FT, y, x) - F(O.FB, y + 1, x)) / vz
+    assert out[O.VX, y, x] == pytest.approx(vx_old, rel=1e-12)
+    assert out[O.VY, y, x] == pytest.approx(vy_old, rel=1e-12)
+    # the plausible slips: the cell's own outflow taken from the new fluxes,
+    # or every flux taken from the new fluxes
+    vx_own_new = (F(O.FR, y, x - 1) - N(O.FL, y, x) + N(O.FR, y, x) - F(O.FL, y, x + 1)) / vz
+    vy_own_new = (F(O.FT, y - 1, x) - N(O.FB, y, x) + N(O.FT, y, x) - F(O.FB, y + 1, x)) / vz
+    vx_all_new = (N(O.FR, y, x - 1) - N(O.FL, y, x) + N(O.FR, y, x) - N(O.FL, y, x + 1)) / vz
+    for wrong_x, wrong_y in [(vx_own_new, vy_own_new), (vx_all_new, vy_own_new)]:
+        assert abs(wrong_x - vx_old) > 1e-3 * abs(vx_old) + 1e-6
+        assert abs(wrong_y - vy_old) > 1e-3 * abs(vy_old) + 1e-6
+
+
 def random_hydro_state(nx, ny, seed):
     rng = np.random.default_rng(seed)
     st = O.new_state(nx, ny=ny)
