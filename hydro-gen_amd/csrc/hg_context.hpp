@@ -45,6 +45,7 @@ struct hg_ctx {
     unsigned long long* dev_dsteps = nullptr;
     int hydro_variant = 0;           // fused kernel tile variants (see hg_autotune)
     int thermal_variant = 0;
+    int sweep_variant = 0;           // one-kernel step variant; -1: the hydro and thermal kernels
     int xcd_order = 2;               // fused kernels' tile order (hg_set_xcd_order)
     float rain_time = 0.0f;          // HG_RAIN_REFERENCE time seed
     float map_max_height = 256.0f;   // HG_RAIN_REFERENCE mountain scale (src/settings.hpp:30)

@@ -44,49 +44,6 @@ __device__ __forceinline__ void st_out(float* p, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Eroded sediment of one cell, recomputed from the step's input planes
-// (rain -> flux -> erosion at q).  Used for back-trace taps outside the halo.
-// ---------------------------------------------------------------------------
-__device__ void eroded_sediment_global(const HydroIO& io, int qx, int qy, int nx, int ny,
-                                       uint32_t step, uint32_t terrain, bool rain, const KParams& k,
-                                       float& s0o, float& s1o) {
-    const float* R = io.r;
-    const float* D = io.d;
-    const float* B = io.b;
-    auto water = [&](int x, int y) -> float {
-        float b = B[at(x, y, nx)];
-        if (rain) b = b + rain_incr(x, y, step, terrain, k);
-        return b;
-    };
-    auto total = [&](int x, int y) -> float {
-        if (x < 0 || y < 0 || x >= nx || y >= ny) return HG_OOB_HEIGHT;
-        const size_t j = at(x, y, nx);
-        return (R[j] + D[j]) + water(x, y);
-    };
-    auto rd = [&](int x, int y) -> float {
-        if (x < 0 || y < 0 || x >= nx || y >= ny) return 0.0f;
-        const size_t j = at(x, y, nx);
-        return R[j] + D[j];
-    };
-    const size_t i = at(qx, qy, nx);
-    float f[4] = {io.fl[i], io.fr[i], io.ft[i], io.fb[i]};
-    const float b = water(qx, qy);
-    const float w = (R[i] + D[i]) + b;
-    const float wn[4] = {total(qx - 1, qy), total(qx + 1, qy), total(qx, qy + 1), total(qx, qy - 1)};
-    const float fin[4] = {qx > 0 ? io.fr[i - 1] : 0.0f, qx < nx - 1 ? io.fl[i + 1] : 0.0f,
-                          qy < ny - 1 ? io.fb[i + nx] : 0.0f, qy > 0 ? io.ft[i - nx] : 0.0f};
-    const unsigned wall = (qx == 0 ? 1u : 0u) | (qx == nx - 1 ? 2u : 0u) | (qy == 0 ? 4u : 0u) |
-                          (qy == ny - 1 ? 8u : 0u);
-    float vx, vy, vz;
-    (void)flux_cell(f, b, w, wn, fin, wall, k, vx, vy, vz);
-    float r = R[i], d = D[i];
-    s0o = io.s0[i];
-    s1o = io.s1[i];
-    erosion_cell(r, d, s0o, s1o, vx, vy, vz, rd(qx - 1, qy), rd(qx + 1, qy), rd(qx, qy + 1),
-                 rd(qx, qy - 1), k);
-}
-
-// ---------------------------------------------------------------------------
 // Cells of the region of radius RAD around a TX x TY tile.  When the tile
 // width is a multiple of 64, the region is split into its core columns (the
 // tile's own: one wave per 64-wide row chunk, so lanes touch 64 consecutive

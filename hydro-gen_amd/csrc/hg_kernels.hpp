@@ -64,6 +64,49 @@ struct ThermIO {
     // optional tape: rock after thermal layer 0, dirt after layer 1 (before smoothing)
     float *tr5 = nullptr, *td6 = nullptr;
 };
+// ---------------------------------------------------------------------------
+// Eroded sediment of one cell, recomputed from the step's input planes
+// (rain -> flux -> erosion at q).  Used for back-trace taps outside the halo.
+// ---------------------------------------------------------------------------
+__device__ inline void eroded_sediment_global(const HydroIO& io, int qx, int qy, int nx, int ny,
+                                       uint32_t step, uint32_t terrain, bool rain, const KParams& k,
+                                       float& s0o, float& s1o) {
+    const float* R = io.r;
+    const float* D = io.d;
+    const float* B = io.b;
+    auto water = [&](int x, int y) -> float {
+        float b = B[size_t(y) * size_t(nx) + size_t(x)];
+        if (rain) b = b + rain_incr(x, y, step, terrain, k);
+        return b;
+    };
+    auto total = [&](int x, int y) -> float {
+        if (x < 0 || y < 0 || x >= nx || y >= ny) return HG_OOB_HEIGHT;
+        const size_t j = size_t(y) * size_t(nx) + size_t(x);
+        return (R[j] + D[j]) + water(x, y);
+    };
+    auto rd = [&](int x, int y) -> float {
+        if (x < 0 || y < 0 || x >= nx || y >= ny) return 0.0f;
+        const size_t j = size_t(y) * size_t(nx) + size_t(x);
+        return R[j] + D[j];
+    };
+    const size_t i = size_t(qy) * size_t(nx) + size_t(qx);
+    float f[4] = {io.fl[i], io.fr[i], io.ft[i], io.fb[i]};
+    const float b = water(qx, qy);
+    const float w = (R[i] + D[i]) + b;
+    const float wn[4] = {total(qx - 1, qy), total(qx + 1, qy), total(qx, qy + 1), total(qx, qy - 1)};
+    const float fin[4] = {qx > 0 ? io.fr[i - 1] : 0.0f, qx < nx - 1 ? io.fl[i + 1] : 0.0f,
+                          qy < ny - 1 ? io.fb[i + nx] : 0.0f, qy > 0 ? io.ft[i - nx] : 0.0f};
+    const unsigned wall = (qx == 0 ? 1u : 0u) | (qx == nx - 1 ? 2u : 0u) | (qy == 0 ? 4u : 0u) |
+                          (qy == ny - 1 ? 8u : 0u);
+    float vx, vy, vz;
+    (void)flux_cell(f, b, w, wn, fin, wall, k, vx, vy, vz);
+    float r = R[i], d = D[i];
+    s0o = io.s0[i];
+    s1o = io.s1[i];
+    erosion_cell(r, d, s0o, s1o, vx, vy, vz, rd(qx - 1, qy), rd(qx + 1, qy), rd(qx, qy + 1),
+                 rd(qx, qy - 1), k);
+}
+
 int hydro_variant_count();
 int thermal_variant_count();
 bool hydro_variant_info(int v, int* tx, int* ty, int* halo, int* nt);
@@ -72,6 +115,13 @@ hipError_t launch_fused_hydro(int variant, const HydroIO& io, int nx, int ny, ui
                               uint32_t terrain, bool rain, int xcd, const KParams& k, hipStream_t s);
 hipError_t launch_fused_thermal(int variant, const ThermIO& io, int nx, int ny, bool thermal,
                                 bool smooth, int xcd, const KParams& k, hipStream_t s);
+
+// the whole step in one kernel (hg_sweep.hip): strips of columns swept row
+// by row; rain / thermal / smoothing as the step's flags say
+int sweep_variant_count();
+bool sweep_variant_info(int v, int* strip_w, int* rows, int* threads, int* seg_rows);
+hipError_t launch_step_sweep(int variant, const HydroIO& io, int n, uint32_t step, uint32_t terrain, bool rain,
+                             bool thermal, bool smooth, const KParams& k, hipStream_t s);
 
 // batched persistent kernels (hg_fused.hip)
 struct BatchTerrain {

@@ -38,7 +38,9 @@ __device__ __forceinline__ float height(const float* R, const float* D, size_t i
 // height it replaced is smaller, and the part that was not there is added
 // back.  Each take is then at most max(0, what the cell held at its atomic
 // add, less the give-backs still outstanding), so the takes of all droplets
-// from one cell never exceed what it held and it never ends below 0.
+// from one cell never exceed what it held and it never ends below 0 in exact
+// arithmetic; in FP32 the subtraction and the give-back round, which can
+// leave a residue of a few ulp of the amounts involved below 0.
 __device__ __forceinline__ float take(float* p, float want) {
     const float est = fminf(fmaxf(ld(p), 0.0f), want);
     if (!(est > 0.0f)) return 0.0f;

@@ -29,9 +29,13 @@ thread_local std::string g_create_error;
 
 // Autotuner results per (grid size, device architecture): {hydro, thermal}.
 std::mutex g_tune_mu;
-std::map<std::pair<int, std::string>, std::pair<int, int>> g_tune_cache;
+struct TunedTiles {
+    int hydro, thermal, sweep;
+};
+std::map<std::pair<int, std::string>, TunedTiles> g_tune_cache;
 constexpr int DEFAULT_HYDRO_VARIANT = 0;
 constexpr int DEFAULT_THERMAL_VARIANT = 1;
+constexpr int DEFAULT_SWEEP_VARIANT = -1;
 
 }  // namespace
 
@@ -194,6 +198,17 @@ int fused_step(hg_ctx* c, bool rain, bool thermal, bool smooth, bool keep_veloci
     io.vy = keep_velocity ? c->alt[HG_P_VEL_V] : nullptr;
     io.vz = keep_velocity ? c->alt[HG_P_VEL_DEPTH] : nullptr;
     const uint32_t step = uint32_t(c->step);
+    if (c->sweep_variant >= 0 && !keep_velocity) {
+        // the whole step in one kernel; rock and dirt come out of its last pass
+        const int rc = timed_launch(c, "step_sweep", [&] {
+            return hgk::launch_step_sweep(c->sweep_variant, io, c->n, step, 0u, rain, thermal, smooth, c->kp,
+                                          c->stream);
+        });
+        if (rc != HG_OK) return rc;
+        swap_planes(c, {HG_P_ROCK, HG_P_DIRT, HG_P_WATER, HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
+                        HG_P_SED_ROCK, HG_P_SED_DIRT});
+        return HG_OK;
+    }
     int rc = timed_launch(c, "fused_hydro", [&] {
         return hgk::launch_fused_hydro(c->hydro_variant, io, c->n, c->n, step, 0u, rain, c->xcd_order, c->kp,
                                        c->stream);
@@ -329,12 +344,14 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->arch = prop.gcnArchName;
     c->hydro_variant = DEFAULT_HYDRO_VARIANT;
     c->thermal_variant = DEFAULT_THERMAL_VARIANT;
+    c->sweep_variant = DEFAULT_SWEEP_VARIANT;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find({c->n, c->arch});
         if (it != g_tune_cache.end()) {
-            c->hydro_variant = it->second.first;
-            c->thermal_variant = it->second.second;
+            c->hydro_variant = it->second.hydro;
+            c->thermal_variant = it->second.thermal;
+            c->sweep_variant = it->second.sweep;
         }
     }
     *out = c;
@@ -887,6 +904,7 @@ int hg_kernel_stats(hg_ctx* c, int i, const char** name, uint64_t* launches, dou
 int hg_tile_variant_count(int kernel) {
     if (kernel == HG_KERNEL_HYDRO) return hgk::hydro_variant_count();
     if (kernel == HG_KERNEL_THERMAL) return hgk::thermal_variant_count();
+    if (kernel == HG_KERNEL_SWEEP) return hgk::sweep_variant_count();
     return HG_ERR_INVALID;
 }
 
@@ -895,6 +913,7 @@ int hg_tile_variant_info(int kernel, int variant, int* tile_x, int* tile_y, int*
     bool ok = false;
     if (kernel == HG_KERNEL_HYDRO) ok = hgk::hydro_variant_info(variant, &tx, &ty, &h, &nt);
     else if (kernel == HG_KERNEL_THERMAL) ok = hgk::thermal_variant_info(variant, &tx, &ty, &h, &nt);
+    else if (kernel == HG_KERNEL_SWEEP) ok = hgk::sweep_variant_info(variant, &tx, &h, &nt, &ty);
     if (!ok) return HG_ERR_INVALID;
     if (tile_x) *tile_x = tx;
     if (tile_y) *tile_y = ty;
@@ -916,6 +935,20 @@ int hg_get_tiles(const hg_ctx* c, int* hydro_variant, int* thermal_variant) {
     if (!c) return HG_ERR_INVALID;
     if (hydro_variant) *hydro_variant = c->hydro_variant;
     if (thermal_variant) *thermal_variant = c->thermal_variant;
+    return HG_OK;
+}
+
+int hg_set_sweep(hg_ctx* c, int variant) {
+    if (!c) return HG_ERR_INVALID;
+    if (variant < -1 || variant >= hgk::sweep_variant_count())
+        return fail(c, HG_ERR_INVALID, "hg_set_sweep: variant is -1 or a HG_KERNEL_SWEEP variant");
+    c->sweep_variant = variant;
+    return HG_OK;
+}
+
+int hg_get_sweep(const hg_ctx* c, int* variant) {
+    if (!c || !variant) return HG_ERR_INVALID;
+    *variant = c->sweep_variant;
     return HG_OK;
 }
 
@@ -945,9 +978,10 @@ int hg_autotune(hg_ctx* c, uint32_t steps, uint32_t flags, int* hydro_best, int*
     int rc = resolve_timing(c);
     std::vector<KernelStat> saved_stats = c->stats;
     c->timing = true;
-    const int h0 = c->hydro_variant, t0 = c->thermal_variant;
+    const int h0 = c->hydro_variant, t0 = c->thermal_variant, s0 = c->sweep_variant;
     double best[2] = {1e300, 1e300};
     int pick[2] = {h0, t0};
+    c->sweep_variant = -1;
     for (int kind = 0; kind < 2 && rc == HG_OK; ++kind) {
         const int nv = kind == 0 ? hgk::hydro_variant_count() : hgk::thermal_variant_count();
         for (int v = 0; v < nv && rc == HG_OK; ++v) {
@@ -966,6 +1000,25 @@ int hg_autotune(hg_ctx* c, uint32_t steps, uint32_t flags, int* hydro_best, int*
         }
         if (kind == 0) c->hydro_variant = pick[0]; else c->thermal_variant = pick[1];
     }
+    // one-kernel steps: each variant's step time against the two kernels'
+    int sweep_pick = -1;
+    {
+        double best_step = (thermal || smooth) ? best[0] + best[1] : best[0];
+        for (int v = 0; v < hgk::sweep_variant_count() && rc == HG_OK; ++v) {
+            c->sweep_variant = v;
+            rc = fused_step(c, rain, thermal, smooth, false);   // warm-up
+            if (rc == HG_OK) rc = resolve_timing(c);
+            c->stats.clear();
+            for (uint32_t s2 = 0; s2 < steps && rc == HG_OK; ++s2) rc = fused_step(c, rain, thermal, smooth, false);
+            if (rc == HG_OK) rc = resolve_timing(c);
+            for (auto& st : c->stats)
+                if (st.name == "step_sweep" && st.launches && st.total_ms / st.launches < best_step) {
+                    best_step = st.total_ms / st.launches;
+                    sweep_pick = v;
+                }
+        }
+        c->sweep_variant = sweep_pick;
+    }
     c->stats = saved_stats;
     c->timing = saved_timing;
     for (int p = 0; p < HG_NUM_STATE_PLANES; ++p)
@@ -976,11 +1029,12 @@ int hg_autotune(hg_ctx* c, uint32_t steps, uint32_t flags, int* hydro_best, int*
     if (rc != HG_OK) {
         c->hydro_variant = h0;
         c->thermal_variant = t0;
+        c->sweep_variant = s0;
         return rc;
     }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[{c->n, c->arch}] = {pick[0], pick[1]};
+        g_tune_cache[{c->n, c->arch}] = {pick[0], pick[1], sweep_pick};
     }
     if (hydro_best) *hydro_best = pick[0];
     if (thermal_best) *thermal_best = pick[1];

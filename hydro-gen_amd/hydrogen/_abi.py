@@ -40,13 +40,14 @@ EXPORTS = [
     "hg_ref_particles_init", "hg_ref_particles_read", "hg_ref_particles_write", "hg_step_particle_reference",
     "hg_ref_particle_stats", "hg_set_ref_particle_dims",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
+    "hg_set_sweep", "hg_get_sweep",
     "hg_batch_create", "hg_batch_destroy", "hg_batch_last_error", "hg_batch_count",
     "hg_batch_set_erosion", "hg_batch_set_rain", "hg_batch_init_fbm", "hg_batch_set_tiles", "hg_batch_autotune",
     "hg_batch_set_persistent", "hg_batch_set_xcd_order",
     "hg_batch_step", "hg_batch_capture", "hg_batch_run", "hg_batch_sync", "hg_batch_get_step",
     "hg_batch_read_plane", "hg_batch_write_plane",
 ]
-KERNEL_HYDRO, KERNEL_THERMAL = 0, 1
+KERNEL_HYDRO, KERNEL_THERMAL, KERNEL_SWEEP = 0, 1, 2
 
 
 class ErosionSettings(ctypes.Structure):
@@ -260,6 +261,8 @@ def load() -> ctypes.CDLL:
     L.hg_tile_variant_count.argtypes = [ctypes.c_int]
     L.hg_tile_variant_info.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
     L.hg_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hg_set_sweep.argtypes = [vp, ctypes.c_int]
+    L.hg_get_sweep.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.hg_get_tiles.argtypes = [vp, ip, ip]
     L.hg_set_xcd_order.argtypes = [vp, ctypes.c_int]
     L.hg_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ip, ip]
@@ -510,6 +513,16 @@ class Terrain:
         a, b = ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.hg_get_tiles(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_sweep(self, variant: int):
+        """One-kernel steps (hg_set_sweep): a KERNEL_SWEEP variant, or -1 for
+        the hydro and thermal kernels."""
+        self._check(self.lib.hg_set_sweep(self.h, variant))
+
+    def get_sweep(self) -> int:
+        v = ctypes.c_int()
+        self._check(self.lib.hg_get_sweep(self.h, ctypes.byref(v)))
+        return v.value
 
     def set_xcd_order(self, mode: int):
         """Workgroup-to-tile order of the fused kernels (hg_set_xcd_order):

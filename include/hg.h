@@ -317,6 +317,7 @@ int hg_step_particle(hg_ctx* ctx, uint32_t iterations, uint32_t flags);
  * number of droplet steps simulated. */
 int hg_particle_stats(hg_ctx* ctx, double* exported_mass, uint64_t* droplet_steps);
 
+
 /* ---- the reference's particle mode ---------------------------------------- */
 
 /* Particle of glsl/bindings.glsl:101-111 in its std430 layout (48 bytes), the
@@ -393,8 +394,11 @@ int hg_adjoint_rollout(hg_ctx* ctx, uint32_t nsteps, uint32_t k_ckpt, uint32_t f
  * per (grid size, device architecture) for contexts created later in the
  * process.  These have no counterpart in the reference (its workgroup is a
  * fixed 8x8, glsl/bindings.glsl:9-10). */
-enum { HG_KERNEL_HYDRO = 0, HG_KERNEL_THERMAL = 1 };
+enum { HG_KERNEL_HYDRO = 0, HG_KERNEL_THERMAL = 1, HG_KERNEL_SWEEP = 2 };
 int hg_tile_variant_count(int kernel);
+/* For HG_KERNEL_SWEEP (the one-kernel step, hg_set_sweep): tile_x = strip
+ * width in columns, tile_y = rows of the segment a workgroup sweeps, halo =
+ * rows per iteration of the sweep. */
 int hg_tile_variant_info(int kernel, int variant, int* tile_x, int* tile_y, int* halo, int* threads);
 /* -1 keeps the current choice. */
 int hg_set_tiles(hg_ctx* ctx, int hydro_variant, int thermal_variant);
@@ -406,8 +410,19 @@ int hg_get_tiles(const hg_ctx* ctx, int* hydro_variant, int* thermal_variant);
  * shares with its horizontal and vertical neighbours are read through one
  * L2 (the default).  Returns HG_ERR_INVALID for other modes. */
 int hg_set_xcd_order(hg_ctx* ctx, int mode);
+/* One-kernel steps.  With a HG_KERNEL_SWEEP variant (the default is 0),
+ * hg_step_grid runs each fused step as one kernel: each workgroup sweeps a
+ * strip of columns row by row and runs every pass of the step a few rows
+ * behind the one it depends on, so the step reads and writes its 9 state
+ * planes once.  -1 runs the hydro kernel and then the thermal kernel.  Both
+ * give the per-pass result bit for bit; steps with HG_STEP_KEEP_VELOCITY
+ * always use the two kernels. */
+int hg_set_sweep(hg_ctx* ctx, int variant);
+int hg_get_sweep(const hg_ctx* ctx, int* variant);
 /* Times `steps` fused steps per variant (state and step counter are restored
- * afterwards) and selects the fastest of each kernel. */
+ * afterwards) and selects the fastest of each kernel, then the one-kernel
+ * variant if one is faster than the two kernels together (else -1, see
+ * hg_set_sweep). */
 int hg_autotune(hg_ctx* ctx, uint32_t steps, uint32_t flags, int* hydro_best, int* thermal_best);
 
 /* ---- batches of terrains -------------------------------------------------- */

@@ -60,4 +60,38 @@ template <int N> inline Dual<N> atan(const Dual<N>& x) {
 template <int N> inline Dual<N> floor(const Dual<N>& x) { return Dual<N>(std::floor(x.v)); }
 template <int N> inline Dual<N> fabs(const Dual<N>& x) { return x.v < 0 ? -x : x; }
 
+// Shadow<K>: a value computed along with K perturbed copies of it.  Every
+// comparison of the model reads the base value (orc::val), so the copies
+// follow the branches of the unperturbed run: a central difference of two
+// copies is the difference quotient of the rollout with its branches frozen,
+// whose limit is the derivative of the branch taken, the gradient that the
+// dual numbers (and the adjoint) compute.
+template <int K> struct Shadow {
+    double v = 0;
+    double c[K] = {};
+    Shadow() = default;
+    Shadow(double x) : v(x) { for (int i = 0; i < K; ++i) c[i] = x; }   // NOLINT
+    Shadow(float x) : Shadow(double(x)) {}                                // NOLINT
+    Shadow(int x) : Shadow(double(x)) {}                                  // NOLINT
+    Shadow& operator+=(const Shadow& o) { v += o.v; for (int i = 0; i < K; ++i) c[i] += o.c[i]; return *this; }
+    Shadow& operator-=(const Shadow& o) { v -= o.v; for (int i = 0; i < K; ++i) c[i] -= o.c[i]; return *this; }
+    Shadow& operator*=(const Shadow& o) { v *= o.v; for (int i = 0; i < K; ++i) c[i] *= o.c[i]; return *this; }
+    Shadow& operator/=(const Shadow& o) { v /= o.v; for (int i = 0; i < K; ++i) c[i] /= o.c[i]; return *this; }
+    friend Shadow operator+(Shadow a, const Shadow& b) { a += b; return a; }
+    friend Shadow operator-(Shadow a, const Shadow& b) { a -= b; return a; }
+    friend Shadow operator*(Shadow a, const Shadow& b) { a *= b; return a; }
+    friend Shadow operator/(Shadow a, const Shadow& b) { a /= b; return a; }
+    friend Shadow operator-(const Shadow& a) { Shadow r; r.v = -a.v; for (int i = 0; i < K; ++i) r.c[i] = -a.c[i]; return r; }
+};
+template <int K> inline double val(const Shadow<K>& x) { return x.v; }
+template <int K> inline Shadow<K> sqrt(const Shadow<K>& x) {
+    Shadow<K> r; r.v = std::sqrt(x.v); for (int i = 0; i < K; ++i) r.c[i] = std::sqrt(x.c[i] > 0 ? x.c[i] : 0.0); return r;
+}
+template <int K> inline Shadow<K> atan(const Shadow<K>& x) {
+    Shadow<K> r; r.v = std::atan(x.v); for (int i = 0; i < K; ++i) r.c[i] = std::atan(x.c[i]); return r;
+}
+// floor and |.| are branches too: the copies take the base run's.
+template <int K> inline Shadow<K> floor(const Shadow<K>& x) { return Shadow<K>(std::floor(x.v)); }
+template <int K> inline Shadow<K> fabs(const Shadow<K>& x) { return x.v < 0 ? -x : x; }
+
 }  // namespace orc

@@ -35,6 +35,8 @@ inline double val(double x) { return x; }
 inline float val(float x) { return x; }
 template <int N> struct Dual;
 template <int N> double val(const Dual<N>& x);
+template <int K> struct Shadow;
+template <int K> double val(const Shadow<K>& x);
 
 template <class T> inline T tmax(T a, T b) { return val(a) < val(b) ? b : a; }   // GLSL max(a,b)
 template <class T> inline T tmin(T a, T b) { return val(b) < val(a) ? b : a; }   // GLSL min(a,b)

@@ -362,6 +362,64 @@ int orc_rollout_grad_dual(const double* planes, int nx, int ny, const hg_erosion
 }
 
 
+// Central differences of the rollout loss with every branch frozen at the
+// unperturbed run (orc::Shadow): fd[j] = (L(+h_j) - L(-h_j)) / (2 h_j) for
+// the directions (Kc, Ks, Kd, Ke) with h_j = eps_coef * coef[j], then the
+// initial rock height at cells[i] with h = eps_cell.  Their limit as h -> 0
+// is the derivative of the branch taken, what orc_rollout_grad_dual returns.
+}  // extern "C"
+template <int NC>
+int rollout_fd_frozen(const double* planes, int nx, int ny, const hg_erosion_settings* s,
+                      const double coef[4], int rain_mode, float rain_amount, uint64_t seed, int nsteps,
+                      unsigned flags, const double* target, const int* cells, int ncells, double eps_coef,
+                      double eps_cell, double* fd) {
+    constexpr int K = 2 * (4 + NC);
+    using S = orc::Shadow<K>;
+    orc::State<S> st;
+    st.resize(nx, ny);
+    const size_t n = size_t(nx) * ny;
+    auto ps = st.planes();
+    for (int p = 0; p < ORC_NUM_PLANES; ++p)
+        for (size_t i = 0; i < n; ++i) (*ps[p])[i] = S(planes[p * n + i]);
+    if (ncells > NC) return -1;
+    for (int c = 0; c < NC && c < ncells; ++c) {
+        if (cells[c] < 0 || size_t(cells[c]) >= n) return -1;
+        S& r = st.r[size_t(cells[c])];
+        r.c[2 * (4 + c)] += eps_cell;
+        r.c[2 * (4 + c) + 1] -= eps_cell;
+    }
+    S cf[4];
+    for (int j = 0; j < 4; ++j) {
+        cf[j] = S(coef[j]);
+        cf[j].c[2 * j] = coef[j] + eps_coef * coef[j];
+        cf[j].c[2 * j + 1] = coef[j] - eps_coef * coef[j];
+    }
+    const S L = rollout_loss<S>(st, s, cf, rain_mode, double(rain_amount), seed, nsteps, flags, target);
+    for (int j = 0; j < 4 + NC && j < 4 + ncells; ++j) {
+        const double h = j < 4 ? (cf[j].c[2 * j] - cf[j].c[2 * j + 1]) / 2 : eps_cell;
+        fd[j] = (L.c[2 * j] - L.c[2 * j + 1]) / (2 * h);
+    }
+    return 0;
+}
+extern "C" {
+
+int orc_rollout_fd_frozen(const double* planes, int nx, int ny, const hg_erosion_settings* s,
+                          const double coef[4], int rain_mode, float rain_amount, uint64_t seed, int nsteps,
+                          unsigned flags, const double* target, const int* cells, int ncells, double eps_coef,
+                          double eps_cell, double* fd) {
+    if (!planes || !s || !coef || !target || !fd || ncells < 0 || ncells > 12 || !(eps_coef > 0) ||
+        !(eps_cell > 0))
+        return -1;
+    if (ncells == 0)
+        return rollout_fd_frozen<0>(planes, nx, ny, s, coef, rain_mode, rain_amount, seed, nsteps, flags, target,
+                                    cells, ncells, eps_coef, eps_cell, fd);
+    if (ncells <= 4)
+        return rollout_fd_frozen<4>(planes, nx, ny, s, coef, rain_mode, rain_amount, seed, nsteps, flags, target,
+                                    cells, ncells, eps_coef, eps_cell, fd);
+    return rollout_fd_frozen<12>(planes, nx, ny, s, coef, rain_mode, rain_amount, seed, nsteps, flags, target,
+                                 cells, ncells, eps_coef, eps_cell, fd);
+}
+
 // The reference's particle mode (orc::ref_particle_dispatch); dims_x, dims_y
 // = Map_settings_data.hmap_dims as the shader reads it (0 = n).
 int orc_ref_particle_run_f64(double* planes, int n, hg_ref_particle* parts, uint32_t count,

@@ -247,6 +247,10 @@ def lib():
         L.orc_rollout_grad_dual.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), dp,
                                             ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int,
                                             ctypes.c_uint, dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, dp, dp]
+        L.orc_rollout_fd_frozen.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ErosionSettings), dp,
+                                            ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int,
+                                            ctypes.c_uint, dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double, dp]
         for name, t in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
             getattr(L, f"orc_gen_heightmap_{name}").argtypes = [ctypes.POINTER(t), ctypes.POINTER(t), ctypes.c_int,
                                                                  ctypes.POINTER(MapSettings)]
@@ -407,6 +411,25 @@ def rollout_loss(state: np.ndarray, settings: ErosionSettings, coef, nsteps: int
                                   seed, nsteps, flags, _ptr(tg), ctypes.byref(out)) != 0:
         raise ValueError("orc_rollout_loss_f64 failed")
     return out.value
+
+
+def rollout_fd_frozen(state: np.ndarray, settings: ErosionSettings, coef, nsteps: int, target: np.ndarray,
+                      cells, eps_coef: float = 1e-3, eps_cell: float = 1e-3, flags: int = STEP_DEFAULT,
+                      rain_mode: int = RAIN_CONSTANT, rain_amount: float = 0.01, seed: int = 0):
+    """FP64 central differences of rollout_loss with every branch frozen at the
+    unperturbed run: (dL/d(Kc,Ks,Kd,Ke) at steps eps_coef * coef, dL/drock0
+    at cells with step eps_cell).  At most 12 cells."""
+    _, ny, nx = state.shape
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    tg = np.ascontiguousarray(target, dtype=np.float64)
+    cl = np.ascontiguousarray(cells, dtype=np.int32)
+    g = np.zeros(4 + len(cl))
+    if lib().orc_rollout_fd_frozen(_ptr(st), nx, ny, ctypes.byref(settings), _ptr(c), rain_mode, rain_amount,
+                                   seed, nsteps, flags, _ptr(tg), cl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   len(cl), eps_coef, eps_cell, _ptr(g)) != 0:
+        raise ValueError("orc_rollout_fd_frozen failed")
+    return g[:4], g[4:]
 
 
 def rollout_grad_dual(state: np.ndarray, settings: ErosionSettings, coef, nsteps: int, target: np.ndarray,

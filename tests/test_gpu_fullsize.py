@@ -43,11 +43,14 @@ def _reset(c1):
     return t, state
 
 
-def test_fused_equals_per_pass_4096(c1):
-    """At this size the fused kernels write their outputs with non-temporal
-    stores and the per-pass kernels with ordinary ones: same bits."""
+@pytest.mark.parametrize("sweep", [-1] + list(range(len(hg.tile_variants(hg.KERNEL_SWEEP)))))
+def test_fused_equals_per_pass_4096(c1, sweep):
+    """At this size the fused kernels (the two-kernel step, sweep = -1, and the
+    one-kernel step's variants) write their outputs with non-temporal stores
+    and the per-pass kernels with ordinary ones: same bits."""
     t, state = _reset(c1)
-    print("fused 2 steps", flush=True)
+    print(f"fused 2 steps, sweep {sweep}", flush=True)
+    t.set_sweep(sweep)
     t.dispatch_grid(2)
     fused = t.read_state(9)
     t, _ = _reset(c1)

@@ -49,3 +49,23 @@ def test_dual_is_linear_in_the_seed(case):
     d1 = (f(1e-6) - f(-1e-6)) / 2e-6
     d2 = (f(2e-6) - f(-2e-6)) / 4e-6
     assert abs(d1 - gc[0]) <= 1e-4 * abs(gc[0]) and abs(d2 - gc[0]) <= 1e-4 * abs(gc[0])
+
+
+def test_frozen_branch_differences(case):
+    """orc::Shadow: copies of the rollout with perturbed inputs that follow the
+    unperturbed run's branches.  Their central difference is the derivative
+    of the branch taken up to the step's truncation and FP64 cancellation, so
+    it meets the dual numbers at 1e-3 with a step of 1e-3 (plain differences
+    of a rollout with branch switches need not)."""
+    n, steps, st, tgt, s, coef = case
+    cells = [3 * n + 4, 12 * n + 12, 20 * n + 5, n * n - 2]
+    _, gc, gh = O.rollout_grad_dual(st, s, coef, steps, tgt, cells)
+    for e in (1e-2, 1e-3):
+        fc, fh = O.rollout_fd_frozen(st, s, coef, steps, tgt, cells, eps_coef=e, eps_cell=e)
+        floor = 1e-3 * np.abs(gc).max()
+        assert (np.abs(fc - gc) <= 1e-3 * np.maximum(np.abs(gc), floor)).all(), (e, fc, gc)
+        assert (np.abs(fh - gh) <= 1e-3 * np.abs(gh).max()).all(), (e, fh, gh)
+    # a wrong step size in the quotient (or a copy that missed its
+    # perturbation) would be off by O(1)
+    fc, _ = O.rollout_fd_frozen(st, s, coef, steps, tgt, cells, eps_coef=1e-3, eps_cell=1e-3)
+    assert abs(fc[0] / gc[0] - 1.0) < 1e-3 and abs(fc[1] / gc[1] - 1.0) < 1e-3
