@@ -155,6 +155,17 @@ struct Brush {
 Brush make_brush(int r);
 hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
                            uint32_t iteration, double* exported, unsigned long long* steps, hipStream_t s);
+// Per-workgroup sums of one droplet launch (256 droplets per workgroup).
+struct DropletPartial {
+    double exported;
+    unsigned long long steps;
+};
+// The droplet kernel writing one DropletPartial per workgroup instead of
+// adding to counters, and the fixed-order sum of those partials.
+hipError_t launch_droplets_partials(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
+                                    uint32_t iteration, DropletPartial* partials, hipStream_t s);
+hipError_t launch_droplet_totals(const DropletPartial* partials, uint32_t nblocks, double* exported,
+                                 uint64_t* steps, bool accumulate, hipStream_t s);
 
 // the reference's particle mode (hg_refparticle.hip)
 struct RefParticleParams {

@@ -65,10 +65,16 @@ __device__ __forceinline__ void deposit(float* D, int n, float px, float py, flo
     atomic_addf(D + i + n + 1, amount * u * v);
 }
 
+// PARTIALS = false: the workgroup's exported mass and droplet steps are added
+// atomically to *exported_out and *steps_out.  PARTIALS = true: they are
+// written to partials[blockIdx.x], for k_droplet_totals to sum in a fixed
+// order.
+template <bool PARTIALS>
 __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* __restrict__ D, int n,
                                                   DropletParams p, Brush brush, uint32_t iteration,
                                                   double* exported_out,
-                                                  unsigned long long* steps_out) {
+                                                  unsigned long long* steps_out,
+                                                  DropletPartial* partials) {
     __shared__ double s_exp[256];
     __shared__ unsigned s_steps[256];
     const uint32_t id = blockIdx.x * 256 + threadIdx.x;
@@ -167,8 +173,43 @@ __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (s_exp[0] != 0.0) unsafeAtomicAdd(exported_out, s_exp[0]);
-        atomicAdd(steps_out, (unsigned long long)s_steps[0]);
+        if constexpr (PARTIALS) {
+            partials[blockIdx.x].exported = s_exp[0];
+            partials[blockIdx.x].steps = s_steps[0];
+        } else {
+            if (s_exp[0] != 0.0) unsafeAtomicAdd(exported_out, s_exp[0]);
+            atomicAdd(steps_out, (unsigned long long)s_steps[0]);
+        }
+    }
+}
+
+// Sums partials[0, nblocks) in a fixed order (thread t: blocks t, t + 256, ...;
+// then a fixed tree) into *exported_out and *steps_out (either may be null),
+// replacing them, or adding to them with `accumulate`.
+__global__ __launch_bounds__(256) void k_droplet_totals(const DropletPartial* __restrict__ partials,
+                                                        uint32_t nblocks, double* exported_out,
+                                                        uint64_t* steps_out, int accumulate) {
+    __shared__ double s_exp[256];
+    __shared__ unsigned long long s_steps[256];
+    double e = 0.0;
+    unsigned long long st = 0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
+        e += partials[b].exported;
+        st += partials[b].steps;
+    }
+    s_exp[threadIdx.x] = e;
+    s_steps[threadIdx.x] = st;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            s_exp[threadIdx.x] += s_exp[threadIdx.x + h];
+            s_steps[threadIdx.x] += s_steps[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (exported_out) *exported_out = (accumulate ? *exported_out : 0.0) + s_exp[0];
+        if (steps_out) *steps_out = (accumulate ? *steps_out : 0ull) + uint64_t(s_steps[0]);
     }
 }
 
@@ -199,7 +240,22 @@ Brush make_brush(int r) {
 hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
                            uint32_t iteration, double* exported, unsigned long long* steps, hipStream_t s) {
     if (p.count == 0) return hipSuccess;
-    k_droplets<<<(p.count + 255) / 256, 256, 0, s>>>(rock, dirt, n, p, brush, iteration, exported, steps);
+    k_droplets<false><<<(p.count + 255) / 256, 256, 0, s>>>(rock, dirt, n, p, brush, iteration, exported, steps,
+                                                             nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_droplets_partials(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
+                                    uint32_t iteration, DropletPartial* partials, hipStream_t s) {
+    if (p.count == 0) return hipSuccess;
+    k_droplets<true><<<(p.count + 255) / 256, 256, 0, s>>>(rock, dirt, n, p, brush, iteration, nullptr, nullptr,
+                                                            partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_droplet_totals(const DropletPartial* partials, uint32_t nblocks, double* exported,
+                                 uint64_t* steps, bool accumulate, hipStream_t s) {
+    k_droplet_totals<<<1, 256, 0, s>>>(partials, nblocks, exported, steps, accumulate ? 1 : 0);
     return hipGetLastError();
 }
 

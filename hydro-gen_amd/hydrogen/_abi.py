@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "libhydrogen.so")
 
 # ---- enums mirrored from include/hg.h -------------------------------------
 HG_OK, HG_ERR_INVALID, HG_ERR_HIP, HG_ERR_OOM, HG_ERR_STATE, HG_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+HG_DROPLETS_NOALLOC_MAX = 4194304   # droplets per iteration of hg_droplets_noalloc
 RAIN_OFF, RAIN_CONSTANT, RAIN_UNIFORM, RAIN_REFERENCE = 0, 1, 2, 3
 CH_HEIGHT, CH_FLUX, CH_VELOCITY, CH_SEDIMENT = 0, 1, 2, 3
 (P_ROCK, P_DIRT, P_WATER, P_FLUX_L, P_FLUX_R, P_FLUX_T, P_FLUX_B, P_SED_ROCK, P_SED_DIRT,
@@ -36,7 +37,7 @@ EXPORTS = [
     "hg_set_rain", "hg_set_rain_time", "hg_init_fbm", "hg_gen_heightmap", "hg_rain", "hg_step_grid", "hg_run_pass", "hg_get_step",
     "hg_set_step", "hg_read", "hg_write", "hg_save_height", "hg_load_height", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
-    "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_adjoint_rollout",
+    "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_droplets_noalloc", "hg_adjoint_rollout",
     "hg_ref_particles_init", "hg_ref_particles_read", "hg_ref_particles_write", "hg_step_particle_reference",
     "hg_ref_particle_stats", "hg_set_ref_particle_dims",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
@@ -250,6 +251,8 @@ def load() -> ctypes.CDLL:
                                      ctypes.POINTER(ctypes.c_double)]
     L.hg_step_particle.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.hg_particle_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
+    L.hg_droplets_noalloc.argtypes = [vp, vp, ctypes.c_uint32, ctypes.POINTER(DropletSettings), ctypes.c_uint32,
+                                      ctypes.c_uint32, vp, vp, vp]
     L.hg_ref_particles_init.argtypes = [vp, ctypes.c_uint32]
     L.hg_ref_particles_read.argtypes = [vp, ctypes.POINTER(RefParticle), ctypes.c_uint32]
     L.hg_ref_particles_write.argtypes = [vp, ctypes.POINTER(RefParticle), ctypes.c_uint32]
@@ -544,6 +547,19 @@ def tile_variants(kernel: int) -> list:
         if L.hg_tile_variant_info(kernel, v, *[ctypes.byref(x) for x in a]) == HG_OK:
             out.append(tuple(x.value for x in a))
     return out
+
+
+def droplets_noalloc(rock_ptr: int, dirt_ptr: int, n: int, s: "DropletSettings", first_iteration: int,
+                     iterations: int, stream: int = 0, exported_ptr: int = 0, steps_ptr: int = 0) -> None:
+    """hg_droplets_noalloc: droplet iterations on caller-owned device planes
+    (device addresses as ints), allocating nothing; exported_ptr / steps_ptr
+    are device addresses of a float64 / uint64 receiving the totals (0 = none)."""
+    L = load()
+    rc = L.hg_droplets_noalloc(ctypes.c_void_p(rock_ptr), ctypes.c_void_p(dirt_ptr), n, ctypes.byref(s),
+                               first_iteration, iterations, ctypes.c_void_p(stream or None),
+                               ctypes.c_void_p(exported_ptr or None), ctypes.c_void_p(steps_ptr or None))
+    if rc != HG_OK:
+        raise HGError(rc, L.hg_last_error(None).decode())
 
 
 class Batch:

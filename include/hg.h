@@ -189,8 +189,8 @@ int hg_create(int device, uint32_t n, uint32_t particle_count, hg_ctx** out);
 /* Replaces World::delete_textures (src/state.cpp:46-55).  NULL is a no-op. */
 int hg_destroy(hg_ctx* ctx);
 
-/* Message of the last failing call on ctx (or of the last failing hg_create
- * when ctx is NULL).  Never NULL. */
+/* Message of the last failing call on ctx (or, when ctx is NULL, of the last
+ * failing hg_create or hg_droplets_noalloc on this thread).  Never NULL. */
 const char* hg_last_error(const hg_ctx* ctx);
 
 /* Grid side length. */
@@ -317,6 +317,27 @@ int hg_step_particle(hg_ctx* ctx, uint32_t iterations, uint32_t flags);
  * number of droplet steps simulated. */
 int hg_particle_stats(hg_ctx* ctx, double* exported_mass, uint64_t* droplet_steps);
 
+/* Droplet iterations that allocate nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * droplet iterations of hg_step_particle (without the thermal and smoothing
+ * flags) on planes the caller owns, enqueued on `stream` (a hipStream_t; NULL
+ * is the default stream) on the current HIP device:
+ *   rock, dirt      device pointers to n x n FP32 planes, eroded in place;
+ *   s               the droplet model (hg_droplet_settings); at most
+ *                   HG_DROPLETS_NOALLOC_MAX droplets per iteration;
+ *   first_iteration the first iteration's index (the start positions'
+ *                   counter), then first_iteration + 1, ...;
+ *   exported_mass, droplet_steps   device pointers (either may be NULL) that
+ *                   receive this call's totals, summed in a fixed order.
+ * Each workgroup's sums go to a fixed 256 KiB device array of the library (one
+ * per GPU, held until the process ends); nothing is allocated, freed or
+ * cleared through the runtime.  Calls that overlap on one GPU share that
+ * array and race.  Errors: hg_last_error(NULL). */
+#define HG_DROPLETS_NOALLOC_MAX 4194304u
+int hg_droplets_noalloc(float* rock, float* dirt, uint32_t n, const hg_droplet_settings* s,
+                        uint32_t first_iteration, uint32_t iterations, void* stream,
+                        double* exported_mass, uint64_t* droplet_steps);
 
 /* ---- the reference's particle mode ---------------------------------------- */
 
