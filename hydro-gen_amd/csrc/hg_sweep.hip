@@ -417,6 +417,259 @@ __global__ __launch_bounds__(NT) void k_step_sweep(SweepArgs a, KParams k) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same sweep with one wave per strip and no LDS stencils.  Lane l of a
+// wave holds column xs - 6 + l of a strip of 52 output columns (lanes 6..57),
+// and the wave walks its segment one row per iteration.  Every pass keeps the
+// rows it and its consumers need in registers (a window that moves up one
+// row per iteration); a cell's left and right neighbours come from the lanes
+// beside it through DPP wave shifts.  Passes evaluate lanes 1..62 (flux and
+// erosion) down to 6..57 (smoothing), one lane fewer each side per pass.
+// The eroded sediment that the transport's bilinear taps read sits in a ring
+// of 8 rows x 64 lanes of LDS private to the wave (taps within 2 rows and
+// inside the wave's columns; the rest recompute exactly).  No pass waits for
+// another wave, so the kernel has no barrier.
+// ---------------------------------------------------------------------------
+constexpr int WV_OUT = 52, WV_XL = 6, WV_WAVES = 4;
+
+// value of the lane to the left (lane - 1) / right (lane + 1) of this one
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+template <int N>
+__device__ __forceinline__ void push(float (&w)[N], float v) {
+#pragma unroll
+    for (int i = N - 1; i > 0; --i) w[i] = w[i - 1];
+    w[0] = v;
+}
+
+template <int RSEG, int OCC, bool NTS>
+__global__ __launch_bounds__(64 * WV_WAVES) __attribute__((amdgpu_waves_per_eu(OCC))) void k_step_wave(
+    SweepArgs a, KParams k) {
+    __shared__ float esh[WV_WAVES][2][8][64];
+    const HydroIO& io = a.io;
+    const int n = a.n;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int gw = blockIdx.x * WV_WAVES + wv;
+    const int strip = gw % a.nstrips, seg = gw / a.nstrips;
+    if (seg >= a.nsegs) return;                       // the whole wave
+    const int xs = strip * WV_OUT, xl0 = xs - WV_XL;  // grid x of lane 0
+    const int x = xl0 + lane;
+    const bool x_in = x >= 0 && x < n;
+    const int y0 = seg * RSEG, R = min(RSEG, n - y0);
+    const bool out_lane = lane >= WV_XL && lane < WV_XL + WV_OUT && x < n;
+    float* e0ring = &esh[wv][0][0][0];
+    float* e1ring = &esh[wv][1][0][0];
+
+    // windows: index 0 is the newest row
+    float rd[3] = {}, bw[3] = {}, wfl[2] = {}, wfr[2] = {}, wft[3] = {}, wfb[2] = {};
+    float wr[2] = {}, wd[2] = {}, ws0[2] = {}, ws1[2] = {};
+    float r1[4] = {}, d1[6] = {}, vx3[3] = {}, vy3[3] = {}, g0[3] = {}, r2[5] = {}, g1[3] = {}, dd[3] = {};
+    // prefetched row
+    float pr = 0.f, pd = 0.f, pb = 0.f, pfl = 0.f, pfr = 0.f, pft = 0.f, pfb = 0.f, ps0 = 0.f, ps1 = 0.f;
+    auto load_row = [&](int y) {
+        pr = pd = pb = pfl = pfr = pft = pfb = ps0 = ps1 = 0.f;
+        if (x_in && y >= 0 && y < n) {
+            const size_t g = gat(x, y, n);
+            pr = io.r[g]; pd = io.d[g]; pb = io.b[g];
+            pfl = io.fl[g]; pfr = io.fr[g]; pft = io.ft[g]; pfb = io.fb[g];
+            ps0 = io.s0[g]; ps1 = io.s1[g];
+        }
+    };
+    const int niter = R + 12;
+    int f = y0 - 6;
+    load_row(f);
+    for (int t = 0; t < niter; ++t, ++f) {
+        // ---- row f enters the input windows (out of the grid: rock + dirt 0,
+        // total height HG_OOB_HEIGHT through the water, no flux)
+        {
+            const bool inside = x_in && f >= 0 && f < n;
+            float b = inside ? pb : HG_OOB_HEIGHT;
+            if (inside && a.rain) b = b + rain_incr(x, f, a.step, a.terrain, k);
+            push(rd, pr + pd);
+            push(bw, b);
+            push(wfl, pfl); push(wfr, pfr); push(wft, pft); push(wfb, pfb);
+            push(wr, pr); push(wd, pd); push(ws0, ps0); push(ws1, ps1);
+        }
+        if (t + 1 < niter) load_row(f + 1);
+
+        // ---- flux + erosion, row f - 1 (inputs: [0] row f, [1] row f - 1, [2] row f - 2)
+        {
+            const int gy = f - 1;
+            const float w0 = rd[1] + bw[1];
+            const float wl = from_left(w0), wrt = from_right(w0);
+            const float hl = from_left(rd[1]), hr = from_right(rd[1]);
+            const float fin_l = from_left(wfr[1]), fin_r = from_right(wfl[1]);
+            float nr1 = HG_OOB_HEIGHT, nd1 = HG_OOB_HEIGHT, ne0 = 0.0f, ne1 = 0.0f, nvx = 0.0f, nvy = 0.0f;
+            if (lane >= 1 && lane < 63 && gy >= y0 - 5 && gy < y0 + R + 5 && x_in && gy >= 0 && gy < n) {
+                float f4[4] = {wfl[1], wfr[1], wft[1], wfb[1]};
+                const float wn[4] = {wl, wrt, rd[0] + bw[0], rd[2] + bw[2]};
+                const float fin[4] = {fin_l, fin_r, wfb[0], wft[2]};
+                const unsigned wall = (x == 0 ? 1u : 0u) | (x == n - 1 ? 2u : 0u) | (gy == 0 ? 4u : 0u) |
+                                      (gy == n - 1 ? 8u : 0u);
+                float vz;
+                const float d2 = flux_cell(f4, bw[1], w0, wn, fin, wall, k, nvx, nvy, vz);
+                float r = wr[1], d = wd[1], s0 = ws0[1], s1 = ws1[1];
+                erosion_cell(r, d, s0, s1, nvx, nvy, vz, hl, hr, rd[0], rd[2], k);
+                nr1 = r; nd1 = d; ne0 = s0; ne1 = s1;
+                if (out_lane && gy >= y0 && gy < y0 + R) {
+                    const size_t g = gat(x, gy, n);
+                    st_out<NTS>(&io.bo[g], d2 * k.evap);     // sediment_transport.glsl:75
+                    st_out<NTS>(&io.flo[g], f4[0]);
+                    st_out<NTS>(&io.fro[g], f4[1]);
+                    st_out<NTS>(&io.fto[g], f4[2]);
+                    st_out<NTS>(&io.fbo[g], f4[3]);
+                }
+            }
+            push(r1, nr1); push(d1, nd1); push(vx3, nvx); push(vy3, nvy);
+            const int sl = ((gy + 64) & 7) * 64 + lane;
+            e0ring[sl] = ne0;
+            e1ring[sl] = ne1;
+        }
+
+        // ---- thermal layer 0 outflow scale, row f - 2 (r1: [0] f-1 .. [3] f-4)
+        {
+            const int gy = f - 2;
+            const float c = r1[1];
+            const float D[8] = {c - from_left(r1[1]), c - from_right(r1[1]), c - r1[0], c - r1[2],
+                                c - from_left(r1[0]), c - from_right(r1[0]), c - from_left(r1[2]),
+                                c - from_right(r1[2])};
+            float ng = 0.0f;
+            if (a.do_thermal && lane >= 2 && lane < 62 && gy >= y0 - 4 && gy < y0 + R + 4) {
+                const float g = thermal_g(D, c, 0, k);
+                ng = (x_in && gy >= 0 && gy < n) ? g : 0.0f;
+            }
+            push(g0, ng);
+        }
+
+        // ---- sediment transport, row f - 3 (velocity [2]; eroded sediment rows
+        // f-8 .. f-1 in the ring)
+        {
+            const int gy = f - 3;
+            if (out_lane && gy >= y0 && gy < y0 + R) {
+                float bx, by;
+                back_trace(x, gy, vx3[2], vy3[2], k.dt, n, n, bx, by);
+                const int ix = int(bx), iy = int(by);
+                const float fx = bx - float(ix), fy = by - float(iy);
+                const int ix1 = min(ix + 1, n - 1), iy1 = min(iy + 1, n - 1);
+                const int l0 = ix - xl0, l1 = ix1 - xl0;
+                float o0, o1;
+                if (l0 >= 1 && l1 <= 62 && iy >= gy - 2 && iy1 <= gy + 2) {
+                    const int a0 = ((iy + 64) & 7) * 64, a1 = ((iy1 + 64) & 7) * 64;
+                    o0 = lerp1(lerp1(e0ring[a0 + l0], e0ring[a0 + l1], fx), lerp1(e0ring[a1 + l0], e0ring[a1 + l1], fx), fy);
+                    o1 = lerp1(lerp1(e1ring[a0 + l0], e1ring[a0 + l1], fx), lerp1(e1ring[a1 + l0], e1ring[a1 + l1], fx), fy);
+                } else {
+                    float p0 = 0.0f, p1 = 0.0f, t0 = 0.0f, t1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
+#pragma unroll 1
+                    for (int tap = 0; tap < 4; ++tap) {
+                        const int qx = (tap & 1) ? ix1 : ix, qy = (tap & 2) ? iy1 : iy;
+                        float v0, v1;
+                        eroded_sediment_global(io, qx, qy, n, n, a.step, a.terrain, a.rain != 0, k, v0, v1);
+                        if ((tap & 1) == 0) {
+                            p0 = v0; p1 = v1;
+                        } else if (tap == 1) {
+                            t0 = lerp1(p0, v0, fx); t1 = lerp1(p1, v1, fx);
+                        } else {
+                            b0 = lerp1(p0, v0, fx); b1 = lerp1(p1, v1, fx);
+                        }
+                    }
+                    o0 = lerp1(t0, b0, fy);
+                    o1 = lerp1(t1, b1, fy);
+                }
+                const size_t g = gat(x, gy, n);
+                st_out<NTS>(&io.s0o[g], o0);
+                st_out<NTS>(&io.s1o[g], o1);
+            }
+        }
+
+        // ---- thermal layer 0 transport, row f - 3 (r1 [1..3], g0 [0..2])
+        {
+            const int gy = f - 3;
+            const float r = r1[2], g = g0[1];
+            const float nbr[8] = {from_left(r1[2]), from_right(r1[2]), r1[1], r1[3],
+                                  from_left(r1[1]), from_right(r1[1]), from_left(r1[3]), from_right(r1[3])};
+            const float nbg[8] = {from_left(g0[1]), from_right(g0[1]), g0[0], g0[2],
+                                  from_left(g0[0]), from_right(g0[0]), from_left(g0[2]), from_right(g0[2])};
+            float nr = r;
+            if (a.do_thermal && lane >= 3 && lane < 61 && gy >= y0 - 3 && gy < y0 + R + 3 && x_in && gy >= 0 &&
+                gy < n) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    acc = acc + thermal_term(r - nbr[kk], g, nbg[kk], kk < 4 ? k.tan_alpha[0] : k.thr_diag[0]);
+                nr = r + acc;
+            }
+            push(r2, nr);
+        }
+
+        // ---- thermal layer 1 outflow scale, row f - 4 (r2 [0..2], d1 [2..4])
+        {
+            const int gy = f - 4;
+            const float r = r2[1], d = d1[3];
+            const float D[8] = {
+                (r - from_left(r2[1])) + (d - from_left(d1[3])), (r - from_right(r2[1])) + (d - from_right(d1[3])),
+                (r - r2[0]) + (d - d1[2]), (r - r2[2]) + (d - d1[4]),
+                (r - from_left(r2[0])) + (d - from_left(d1[2])), (r - from_right(r2[0])) + (d - from_right(d1[2])),
+                (r - from_left(r2[2])) + (d - from_left(d1[4])), (r - from_right(r2[2])) + (d - from_right(d1[4]))};
+            float ng = 0.0f;
+            if (a.do_thermal && lane >= 4 && lane < 60 && gy >= y0 - 2 && gy < y0 + R + 2) {
+                const float g = thermal_g(D, d, 1, k);
+                ng = (x_in && gy >= 0 && gy < n) ? g : 0.0f;
+            }
+            push(g1, ng);
+        }
+
+        // ---- thermal layer 1 transport, row f - 5 (r2 [1..3], d1 [3..5], g1 [0..2])
+        {
+            const int gy = f - 5;
+            const float r = r2[2], d = d1[4], g = g1[1];
+            const float nbr[8] = {from_left(r2[2]), from_right(r2[2]), r2[1], r2[3],
+                                  from_left(r2[1]), from_right(r2[1]), from_left(r2[3]), from_right(r2[3])};
+            const float nbd[8] = {from_left(d1[4]), from_right(d1[4]), d1[3], d1[5],
+                                  from_left(d1[3]), from_right(d1[3]), from_left(d1[5]), from_right(d1[5])};
+            const float nbg[8] = {from_left(g1[1]), from_right(g1[1]), g1[0], g1[2],
+                                  from_left(g1[0]), from_right(g1[0]), from_left(g1[2]), from_right(g1[2])};
+            float nd = d;
+            if (a.do_thermal && lane >= 5 && lane < 59 && gy >= y0 - 1 && gy < y0 + R + 1 && x_in && gy >= 0 &&
+                gy < n) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk)
+                    acc = acc + thermal_term((r - nbr[kk]) + (d - nbd[kk]), g, nbg[kk],
+                                             kk < 4 ? k.tan_alpha[1] : k.thr_diag[1]);
+                nd = d + acc;
+            }
+            push(dd, nd);
+        }
+
+        // ---- smoothing, row f - 6 (r2 [2..4], dd [0..2]); write rock and dirt
+        {
+            const int gy = f - 6;
+            float r = r2[3], d = dd[1];
+            const float lr = from_left(r2[3]), lg = from_left(dd[1]), rrt = from_right(r2[3]), rg = from_right(dd[1]);
+            if (out_lane && gy >= y0 && gy < y0 + R) {
+                if (a.do_smooth && x > 0 && gy > 0 && x < n - 1 && gy < n - 1)
+                    smooth_cell(r, d, lr, lg, rrt, rg, r2[2], dd[0], r2[4], dd[2], k.smooth_m);
+                const size_t g = gat(x, gy, n);
+                st_out<NTS>(&io.ro[g], r);
+                st_out<NTS>(&io.dout[g], d);
+            }
+        }
+    }
+}
+
+// rows per segment, waves per SIMD the registers are sized for
+#define HG_WAVE_VARIANTS(X) \
+    X(0, 64, 4)             \
+    X(1, 96, 4)             \
+    X(2, 128, 4)            \
+    X(3, 64, 3)             \
+    X(4, 128, 3)
+
 //                      W   H   NT  HA  rows per segment
 #define HG_SWEEP_VARIANTS(X)   \
     X(0, 112, 3, 384, 2, 256)  \
@@ -427,14 +680,33 @@ __global__ __launch_bounds__(NT) void k_step_sweep(SweepArgs a, KParams k) {
 
 }  // namespace
 
+// Variants 0 .. (wave variants - 1) are the one-wave-per-strip kernel, the
+// rest the workgroup kernel with LDS rings.
 int sweep_variant_count() {
-#define X(i, w, h, nt, ha, rs) +1
-    return 0 HG_SWEEP_VARIANTS(X);
+#define X(i, ...) +1
+    return 0 HG_WAVE_VARIANTS(X) HG_SWEEP_VARIANTS(X);
 #undef X
 }
 
+namespace {
+constexpr int kWaveVariants = 0
+#define X(i, ...) +1
+    HG_WAVE_VARIANTS(X)
+#undef X
+    ;
+}  // namespace
+
 bool sweep_variant_info(int v, int* strip_w, int* rows, int* threads, int* seg_rows) {
-    switch (v) {
+    if (v < kWaveVariants) {
+        switch (v) {
+#define X(i, RS_, OCC_) \
+    case i: *strip_w = WV_OUT; *rows = 1; *threads = 64 * WV_WAVES; *seg_rows = RS_; return true;
+            HG_WAVE_VARIANTS(X)
+#undef X
+            default: return false;
+        }
+    }
+    switch (v - kWaveVariants) {
 #define X(i, W_, H_, NT_, HA_, RS_) \
     case i: *strip_w = W_; *rows = H_; *threads = NT_; *seg_rows = RS_; return true;
         HG_SWEEP_VARIANTS(X)
@@ -445,14 +717,35 @@ bool sweep_variant_info(int v, int* strip_w, int* rows, int* threads, int* seg_r
 
 hipError_t launch_step_sweep(int variant, const HydroIO& io, int n, uint32_t step, uint32_t terrain, bool rain,
                              bool thermal, bool smooth, const KParams& k, hipStream_t s) {
-    switch (variant) {
+    SweepArgs a;
+    a.io = io; a.n = n;
+    a.step = step; a.terrain = terrain;
+    a.rain = rain ? 1 : 0; a.do_thermal = thermal ? 1 : 0; a.do_smooth = smooth ? 1 : 0;
+    if (variant < kWaveVariants) {
+        switch (variant) {
+#define X(i, RS_, OCC_)                                                                            \
+    case i: {                                                                                     \
+        a.seg_rows = RS_;                                                                         \
+        a.nstrips = (n + WV_OUT - 1) / WV_OUT; a.nsegs = (n + RS_ - 1) / RS_;                     \
+        const unsigned waves = unsigned(a.nstrips) * unsigned(a.nsegs);                           \
+        const unsigned grid = (waves + WV_WAVES - 1) / WV_WAVES;                                  \
+        if (k.store_nt)                                                                           \
+            k_step_wave<RS_, OCC_, true><<<grid, 64 * WV_WAVES, 0, s>>>(a, k);                    \
+        else                                                                                      \
+            k_step_wave<RS_, OCC_, false><<<grid, 64 * WV_WAVES, 0, s>>>(a, k);                   \
+        break;                                                                                    \
+    }
+            HG_WAVE_VARIANTS(X)
+#undef X
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    switch (variant - kWaveVariants) {
 #define X(i, W_, H_, NT_, HA_, RS_)                                                               \
     case i: {                                                                                    \
-        SweepArgs a;                                                                             \
-        a.io = io; a.n = n; a.seg_rows = RS_;                                                    \
+        a.seg_rows = RS_;                                                                        \
         a.nstrips = (n + W_ - 1) / W_; a.nsegs = (n + RS_ - 1) / RS_;                            \
-        a.step = step; a.terrain = terrain;                                                      \
-        a.rain = rain ? 1 : 0; a.do_thermal = thermal ? 1 : 0; a.do_smooth = smooth ? 1 : 0;     \
         const unsigned grid = unsigned(a.nstrips) * unsigned(a.nsegs);                           \
         if (k.store_nt)                                                                          \
             k_step_sweep<W_, H_, NT_, HA_, true><<<grid, NT_, 0, s>>>(a, k);                     \
