@@ -35,7 +35,9 @@ HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 # read once and written once):
 #   fused_hydro  : rock, dirt, water, flux L/R/T/B, sediment rock/dirt -> 9 x (4+4) = 72 B
 #   fused_thermal: rock, dirt -> 2 x (4+4) = 16 B
-ALGO_BYTES = {"fused_hydro": 72, "fused_thermal": 16}
+#   step_sweep   : the whole step in one kernel, the 9 planes once -> 72 B
+ALGO_BYTES = {"fused_hydro": 72, "fused_thermal": 16, "step_sweep": 72}
+KERNEL_INDEX = {"fused_hydro": 0, "fused_thermal": 1, "step_sweep": 2}
 STEP_ALGO_BYTES = 72         # minimum for a step that materialises the 9 planes once
 
 
@@ -81,7 +83,7 @@ def batched_workload(seed: int = 2026, count: int = 64):
     return sides, coeffs
 
 
-def bench_batched(steps: int, base_steps: int) -> dict:
+def bench_batched(steps: int, base_steps: int, graph_steps: int = 100) -> dict:
     sides, coeffs = batched_workload()
     cells = float(sum(n * n for n in sides))
     b = hg.Batch(sides)
@@ -94,7 +96,8 @@ def bench_batched(steps: int, base_steps: int) -> dict:
     hv, tv = b.autotune(4)
     out["tile_variants"] = {"hydro": hv, "thermal": tv}
     log(f"batched tiles: hydro {hv}, thermal {tv}")
-    # baselines: per-terrain launches, single-terrain fused kernels / one kernel per pass
+    # baselines: per-terrain launches, single-terrain fused kernels / one kernel
+    # per pass, eager
     for key, flags, k in (("per_terrain_fused", hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN, base_steps),
                           ("per_terrain_per_pass",
                            hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN | hg.STEP_PER_PASS, base_steps)):
@@ -105,6 +108,19 @@ def bench_batched(steps: int, base_steps: int) -> dict:
         b.sync()
         dt = time.perf_counter() - t0
         out[key] = {"steps": k, "cell_steps_per_s": cells * k / dt, "ms_per_step": dt / k * 1e3}
+    # baseline: per-terrain launches of the same fused kernels (each over its
+    # terrain's tiles), `graph_steps` steps captured in one hipGraph, so launch
+    # overhead is off the clock for the baseline as well
+    b.set_persistent(False)
+    b.capture(graph_steps, hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN)
+    b.run()
+    b.sync()
+    t0 = time.perf_counter()
+    b.run()
+    b.sync()
+    dt = time.perf_counter() - t0
+    out["per_terrain_fused_graph"] = {"steps": graph_steps, "cell_steps_per_s": cells * graph_steps / dt,
+                                      "ms_per_step": dt / graph_steps * 1e3}
     # batched: one launch per pass over all terrains' tiles, `steps` steps
     # captured in one hipGraph; flat grid (a workgroup per tile) and persistent grid
     best = None
@@ -125,6 +141,7 @@ def bench_batched(steps: int, base_steps: int) -> dict:
     v = best
     out["speedup_vs_per_terrain_per_pass"] = v / out["per_terrain_per_pass"]["cell_steps_per_s"]
     out["speedup_vs_per_terrain_fused"] = v / out["per_terrain_fused"]["cell_steps_per_s"]
+    out["speedup_vs_per_terrain_fused_graph"] = v / out["per_terrain_fused_graph"]["cell_steps_per_s"]
     h = b.read_plane(len(sides) - 1, hg.P_ROCK)
     out["state_finite"] = bool(np.isfinite(h).all())
     b.close()
@@ -341,8 +358,10 @@ def main() -> int:
     flags = hg.STEP_DEFAULT | (hg.STEP_PER_PASS if args.per_pass else 0)
     t = build_terrain(n)
     tiles = None
+    sweep = -1
     if args.tune and not args.per_pass:
         tiles = t.autotune(10, flags)
+        sweep = t.get_sweep()
     t.dispatch_grid(args.warmup, flags)
     t.sync()
 
@@ -366,7 +385,8 @@ def main() -> int:
     if per_cell is not None:
         ach = per_cell * n * n / dom_avg_s
         hv, tv = t.get_tiles()
-        variant = hg.tile_variants(0)[hv] if dom == "fused_hydro" else hg.tile_variants(1)[tv]
+        pick = {"fused_hydro": hv, "fused_thermal": tv, "step_sweep": sweep}[dom]
+        variant = hg.tile_variants(KERNEL_INDEX[dom])[pick]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": ach / HBM_PEAK,
                     "traffic": profiled_traffic(dom, n, variant),
@@ -414,7 +434,7 @@ def main() -> int:
             "path": "per-pass" if args.per_pass else "fused",
             "step_hbm_roofline_frac": value * STEP_ALGO_BYTES / HBM_PEAK,
             "kernels": kern,
-            "tile_variants": {"hydro": tiles[0], "thermal": tiles[1]} if tiles else None,
+            "tile_variants": {"hydro": tiles[0], "thermal": tiles[1], "sweep": sweep} if tiles else None,
             "state_finite": finite,
             "batched": batched,
             "particles": particles,

@@ -48,6 +48,11 @@ struct hg_batch {
     uint32_t* dev_tiles_h = nullptr;
     uint32_t* dev_tiles_t = nullptr;
     int ntiles_h = 0, ntiles_t = 0;
+    // the work-lists in natural (terrain, row, column) order, and each
+    // terrain's first entry and count in them: per-terrain launches
+    uint32_t* dev_nat_h = nullptr;
+    uint32_t* dev_nat_t = nullptr;
+    std::vector<int> first_h, count_h, first_t, count_t;
     int grid_h = 0, grid_t = 0;
     bool tiles_dirty = true, terr_dirty = true;
     hipGraph_t graph = nullptr;
@@ -126,14 +131,24 @@ int upload_terr(hg_batch* b) {
     return HG_OK;
 }
 
-// XCD-interleaved work-list of all tiles for a TX x TY tiling.
-std::vector<uint32_t> work_list(const hg_batch* b, int tx, int ty) {
+// Work-list of all tiles for a TX x TY tiling in natural order; `first` and
+// `count` receive each terrain's range in it.
+std::vector<uint32_t> natural_list(const hg_batch* b, int tx, int ty, std::vector<int>* first,
+                                   std::vector<int>* count) {
     std::vector<uint32_t> nat;
     for (int t = 0; t < b->count; ++t) {
+        if (first) first->push_back(int(nat.size()));
         const int nbx = (b->n[t] + tx - 1) / tx, nby = (b->n[t] + ty - 1) / ty;
         for (int y = 0; y < nby; ++y)
             for (int x = 0; x < nbx; ++x) nat.push_back((uint32_t(t) << 24) | (uint32_t(y) << 12) | uint32_t(x));
+        if (count) count->push_back(nbx * nby);
     }
+    return nat;
+}
+
+// XCD-interleaved work-list of all tiles for a TX x TY tiling.
+std::vector<uint32_t> work_list(const hg_batch* b, int tx, int ty) {
+    std::vector<uint32_t> nat = natural_list(b, tx, ty, nullptr, nullptr);
     if (!b->xcd_order) return nat;
     const size_t N = nat.size(), chunk = (N + 7) / 8;
     std::vector<uint32_t> out(chunk * 8, 0xffffffffu);
@@ -163,6 +178,24 @@ int build_tiles(hg_batch* b) {
     BH(b, hipMemcpy(b->dev_tiles_t, lt.data(), lt.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     b->ntiles_h = int(lh.size());
     b->ntiles_t = int(lt.size());
+    {
+        b->first_h.clear(); b->count_h.clear(); b->first_t.clear(); b->count_t.clear();
+        hgk::hydro_variant_info(b->hydro_variant, &tx, &ty, &h, &nt);
+        std::vector<uint32_t> nh = natural_list(b, tx, ty, &b->first_h, &b->count_h);
+        hgk::thermal_variant_info(b->thermal_variant, &tx, &ty, &h, &nt);
+        std::vector<uint32_t> ntl = natural_list(b, tx, ty, &b->first_t, &b->count_t);
+        if (b->dev_nat_h) (void)hipFree(b->dev_nat_h);
+        if (b->dev_nat_t) (void)hipFree(b->dev_nat_t);
+        b->dev_nat_h = b->dev_nat_t = nullptr;
+        void* ph = nullptr;
+        void* pt = nullptr;
+        BH(b, hipMalloc(&ph, nh.size() * sizeof(uint32_t)));
+        b->dev_nat_h = static_cast<uint32_t*>(ph);
+        BH(b, hipMalloc(&pt, ntl.size() * sizeof(uint32_t)));
+        b->dev_nat_t = static_cast<uint32_t*>(pt);
+        BH(b, hipMemcpy(b->dev_nat_h, nh.data(), nh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        BH(b, hipMemcpy(b->dev_nat_t, ntl.data(), ntl.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     // persistent grids: resident workgroups, a multiple of 8 (one per XCD slot)
     b->grid_h = b->cu_count * hgk::batch_hydro_blocks_per_cu(b->hydro_variant);
     b->grid_t = b->cu_count * hgk::batch_thermal_blocks_per_cu(b->thermal_variant);
@@ -184,19 +217,36 @@ void drop_graph(hg_batch* b) {
     b->graph_steps = 0;
 }
 
-// Enqueue the persistent launches of `nsteps` fused steps starting at offset
-// 0 from *dev_step, then advance *dev_step.  Updates b->cur.
-int enqueue_steps(hg_batch* b, uint32_t nsteps, uint32_t flags) {
+// Enqueue the launches of `nsteps` fused steps starting at offset 0 from
+// *dev_step, then advance *dev_step.  Updates b->cur.  One launch of each
+// fused kernel per step covers every terrain; with `per_terrain` each terrain
+// gets launches of its own (the same kernels on its range of the natural
+// work-list), the per-terrain pattern the batch is measured against.
+int enqueue_steps(hg_batch* b, uint32_t nsteps, uint32_t flags, bool per_terrain = false) {
     const bool rain = (flags & HG_STEP_RAIN) && b->rain_mode != HG_RAIN_OFF;
     const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
     const bool nts = hgk::stream_stores(b->total_floats * sizeof(float)) != 0;   // every terrain, both buffers
     for (uint32_t s = 0; s < nsteps; ++s) {
-        BH(b, hgk::launch_batch_hydro(b->hydro_variant, b->dev_terr, b->dev_tiles_h, b->ntiles_h, b->grid_h,
-                                      b->cur, b->dev_step, s, rain, nts, b->stream));
+        if (per_terrain) {
+            for (int t = 0; t < b->count; ++t)
+                BH(b, hgk::launch_batch_hydro(b->hydro_variant, b->dev_terr, b->dev_nat_h + b->first_h[t],
+                                              b->count_h[t], b->count_h[t], b->cur, b->dev_step, s, rain, nts,
+                                              b->stream));
+        } else {
+            BH(b, hgk::launch_batch_hydro(b->hydro_variant, b->dev_terr, b->dev_tiles_h, b->ntiles_h, b->grid_h,
+                                          b->cur, b->dev_step, s, rain, nts, b->stream));
+        }
         b->cur ^= (1u << HG_NUM_STATE_PLANES) - 1u;
         if (thermal || smooth) {
-            BH(b, hgk::launch_batch_thermal(b->thermal_variant, b->dev_terr, b->dev_tiles_t, b->ntiles_t,
-                                            b->grid_t, b->cur, thermal, smooth, nts, b->stream));
+            if (per_terrain) {
+                for (int t = 0; t < b->count; ++t)
+                    BH(b, hgk::launch_batch_thermal(b->thermal_variant, b->dev_terr, b->dev_nat_t + b->first_t[t],
+                                                    b->count_t[t], b->count_t[t], b->cur, thermal, smooth, nts,
+                                                    b->stream));
+            } else {
+                BH(b, hgk::launch_batch_thermal(b->thermal_variant, b->dev_terr, b->dev_tiles_t, b->ntiles_t,
+                                                b->grid_t, b->cur, thermal, smooth, nts, b->stream));
+            }
             b->cur ^= (1u << HG_P_ROCK) | (1u << HG_P_DIRT);
         }
     }
@@ -360,6 +410,8 @@ int hg_batch_destroy(hg_batch* b) {
     drop_graph(b);
     if (b->dev_tiles_h) (void)hipFree(b->dev_tiles_h);
     if (b->dev_tiles_t) (void)hipFree(b->dev_tiles_t);
+    if (b->dev_nat_h) (void)hipFree(b->dev_nat_h);
+    if (b->dev_nat_t) (void)hipFree(b->dev_nat_t);
     if (b->scratch) (void)hipFree(b->scratch);
     (void)hipFree(b->block);
     (void)hipFree(b->dev_terr);
@@ -545,8 +597,8 @@ int hg_batch_capture(hg_batch* b, uint32_t nsteps, uint32_t flags) {
     if (!b) return HG_ERR_INVALID;
     if (nsteps == 0 || (nsteps % 2) != 0)
         return bfail(b, HG_ERR_INVALID, "hg_batch_capture: nsteps must be even and > 0 (buffers return to their start every 2 steps)");
-    if (flags & ~(HG_STEP_RAIN | HG_STEP_THERMAL | HG_STEP_SMOOTH))
-        return bfail(b, HG_ERR_INVALID, "hg_batch_capture: only rain/thermal/smooth flags");
+    if (flags & ~(HG_STEP_RAIN | HG_STEP_THERMAL | HG_STEP_SMOOTH | HG_STEP_PER_TERRAIN))
+        return bfail(b, HG_ERR_INVALID, "hg_batch_capture: only rain/thermal/smooth/per-terrain flags");
     Guard g(b->device);
     int rc = upload_terr(b);
     if (rc == HG_OK) rc = build_tiles(b);
@@ -554,7 +606,7 @@ int hg_batch_capture(hg_batch* b, uint32_t nsteps, uint32_t flags) {
     drop_graph(b);
     const uint32_t cur0 = b->cur;
     BH(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_steps(b, nsteps, flags);
+    rc = enqueue_steps(b, nsteps, flags, (flags & HG_STEP_PER_TERRAIN) != 0);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &graph);
     b->cur = cur0;   // capture does not execute anything

@@ -480,7 +480,11 @@ int hg_batch_set_xcd_order(hg_batch* b, int on);
  * pattern).  All three give identical results. */
 int hg_batch_step(hg_batch* b, uint32_t nsteps, uint32_t flags);
 /* Records nsteps (even) persistent steps into a hipGraph; hg_batch_run
- * replays it, continuing the step counter (kept in device memory).  The
+ * replays it, continuing the step counter (kept in device memory).  With
+ * HG_STEP_PER_TERRAIN in flags the graph instead holds one launch of each
+ * fused kernel per terrain and step, over that terrain's tiles only (the
+ * pattern of eroding the terrains one by one, for comparison; same results).
+ * The
  * graph's launches name the plane buffers current at capture time: after an
  * odd number of eager hg_batch_step steps (which swap them) hg_batch_run
  * returns HG_ERR_STATE until another step restores them or a new capture

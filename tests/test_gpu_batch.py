@@ -89,12 +89,12 @@ def test_batch_autotune_restores_state(states):
             assert all(np.array_equal(st[p], ref[i][p]) for p in range(hg.NUM_STATE_PLANES)), (hv, tv, i)
 
 
-@pytest.mark.parametrize("persistent", [False, True])
-def test_graph_replay_matches_stepping(states, persistent):
+@pytest.mark.parametrize("persistent,per_terrain", [(False, False), (True, False), (False, True)])
+def test_graph_replay_matches_stepping(states, persistent, per_terrain):
     eager = run_mode(states, 8, hg.STEP_DEFAULT)
     with make_batch(states) as b:
         b.set_persistent(persistent)
-        b.capture(4)
+        b.capture(4, hg.STEP_DEFAULT | (hg.STEP_PER_TERRAIN if per_terrain else 0))
         b.run()
         b.sync()
         assert b.step_count == 4
