@@ -346,6 +346,9 @@ def main() -> int:
                     help="steps of the differentiable-rollout workload (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy (PMC counter calibration)")
+    ap.add_argument("--sweep", type=int, default=None,
+                    help="force the step kernel: -1 two kernels, v >= 0 one-kernel variant v "
+                         "(default: the autotuner's choice)")
     ap.add_argument("--tune", action="store_true", default=True)
     ap.add_argument("--no-tune", dest="tune", action="store_false")
     args = ap.parse_args()
@@ -362,6 +365,9 @@ def main() -> int:
     if args.tune and not args.per_pass:
         tiles = t.autotune(10, flags)
         sweep = t.get_sweep()
+    if args.sweep is not None and not args.per_pass:
+        t.set_sweep(args.sweep)
+        sweep = args.sweep
     t.dispatch_grid(args.warmup, flags)
     t.sync()
 
@@ -434,7 +440,8 @@ def main() -> int:
             "path": "per-pass" if args.per_pass else "fused",
             "step_hbm_roofline_frac": value * STEP_ALGO_BYTES / HBM_PEAK,
             "kernels": kern,
-            "tile_variants": {"hydro": tiles[0], "thermal": tiles[1], "sweep": sweep} if tiles else None,
+            "tile_variants": {"hydro": tiles[0], "thermal": tiles[1]} if tiles else None,
+            "sweep_variant": sweep,      # -1: hydro + thermal kernels; v >= 0: one-kernel variant v
             "state_finite": finite,
             "batched": batched,
             "particles": particles,

@@ -732,6 +732,28 @@ template <class T> void pass_thermal_transport(State<T>& s, int layer) {
 // ---------------------------------------------------------------------------
 // smoothing, grid branch (particle_count == 0): glsl/smoothing.glsl:22-74,97-103.
 // ---------------------------------------------------------------------------
+// Extremum flags of interior cell (x, y) for the rock (er) and surface (eg)
+// channels (glsl/smoothing.glsl:40-73).
+template <class T> void smooth_flags_at(const State<T>& s, int x, int y, bool& er, bool& eg) {
+    const size_t i = s.at(x, y);
+    const T tr = s.r[i], tg = s.d[i];
+    const size_t il = s.at(x - 1, y), ir = s.at(x + 1, y), it = s.at(x, y + 1), ib = s.at(x, y - 1);
+    const T lr = s.r[il], lg = s.d[il], rr = s.r[ir], rg = s.d[ir];
+    const T t_r = s.r[it], t_g = s.d[it], br = s.r[ib], bg = s.d[ib];
+    T dlr = tr - lr, dlg = tg - lg; dlg += dlr;                                 // :40-48
+    T drr = tr - rr, drg = tg - rg; drg += drr;
+    T dtr = tr - t_r, dtg = tg - t_g; dtg += dtr;
+    T dbr = tr - br, dbg = tg - bg; dbg += dbr;
+    const T g_hdiff = fabs((dlg + drg + dtg + dbg) / T(4));                     // :50-55
+    const T r_hdiff = fabs((dlr + drr + dtr + dbr) / T(4));
+    const T xcr = dlr * drr, xcg = dlg * drg;                                   // :60-61
+    const T ycr = dtr * dbr, ycg = dtg * dbg;
+    er = ((val(-dlr) > val(r_hdiff) || val(-drr) > val(r_hdiff)) && val(xcr) > 0) ||   // :63
+         ((val(-dtr) > val(r_hdiff) || val(-dbr) > val(r_hdiff)) && val(ycr) > 0);
+    eg = ((val(-dlg) > val(g_hdiff) || val(-drg) > val(g_hdiff)) && val(xcg) > 0) ||   // :69
+         ((val(-dtg) > val(g_hdiff) || val(-dbg) > val(g_hdiff)) && val(ycg) > 0);
+}
+
 template <class T> void pass_smooth(State<T>& s, const Settings<T>& k) {
     std::vector<T>&orr = s.scratch(0), &od = s.scratch(1);
     const int nx = s.nx, ny = s.ny;
@@ -743,25 +765,14 @@ template <class T> void pass_smooth(State<T>& s, const Settings<T>& k) {
                 orr[i] = s.r[i]; od[i] = s.d[i];
                 continue;
             }
-            T tr = s.r[i], tg = s.d[i];
-            size_t il = s.at(x - 1, y), ir = s.at(x + 1, y), it = s.at(x, y + 1), ib = s.at(x, y - 1);
-            T lr = s.r[il], lg = s.d[il], rr = s.r[ir], rg = s.d[ir];
-            T t_r = s.r[it], t_g = s.d[it], br = s.r[ib], bg = s.d[ib];
-            T dlr = tr - lr, dlg = tg - lg; dlg += dlr;                             // :40-48
-            T drr = tr - rr, drg = tg - rg; drg += drr;
-            T dtr = tr - t_r, dtg = tg - t_g; dtg += dtr;
-            T dbr = tr - br, dbg = tg - bg; dbg += dbr;
-            T g_hdiff = fabs((dlg + drg + dtg + dbg) / T(4));                       // :50-55
-            T r_hdiff = fabs((dlr + drr + dtr + dbr) / T(4));
-            T xcr = dlr * drr, xcg = dlg * drg;                                     // :60-61
-            T ycr = dtr * dbr, ycg = dtg * dbg;
+            bool er, eg;
+            smooth_flags_at(s, x, y, er, eg);
+            const T tr = s.r[i], tg = s.d[i];
             T nr = tr, ng = tg;
-            if (((val(-dlr) > val(r_hdiff) || val(-drr) > val(r_hdiff)) && val(xcr) > 0) ||
-                ((val(-dtr) > val(r_hdiff) || val(-dbr) > val(r_hdiff)) && val(ycr) > 0))
-                nr = (tr + lr + rr + t_r + br) / T(5);                              // :63-67
-            if (((val(-dlg) > val(g_hdiff) || val(-drg) > val(g_hdiff)) && val(xcg) > 0) ||
-                ((val(-dtg) > val(g_hdiff) || val(-dbg) > val(g_hdiff)) && val(ycg) > 0))
-                ng = (tg + lg + rg + t_g + bg) / T(5);                              // :69-73
+            if (er) nr = (tr + s.r[s.at(x - 1, y)] + s.r[s.at(x + 1, y)] + s.r[s.at(x, y + 1)] +   // :63-67
+                          s.r[s.at(x, y - 1)]) / T(5);
+            if (eg) ng = (tg + s.d[s.at(x - 1, y)] + s.d[s.at(x + 1, y)] + s.d[s.at(x, y + 1)] +   // :69-73
+                          s.d[s.at(x, y - 1)]) / T(5);
             T m = tclamp(k.Kspeed[1] * k.dt, T(0), T(1));                           // :75
             orr[i] = m * nr + (T(1) - m) * s.r[i];                                  // :100
             od[i] = m * ng + (T(1) - m) * s.d[i];
@@ -1158,12 +1169,19 @@ template <class T> void ref_particle_smooth_extra(State<T>& s, std::vector<T> mo
 // particle, erosion of every particle (id order), thermal erosion of both
 // layers and smoothing (grid branch for rock/dirt, particle branch for water
 // and momentum).  `thermal` / `smooth` switch those passes off for tests.
+// The reference's per-pixel lock (glsl/particle_erosion.glsl:89-99) makes each
+// particle's update of a corner one transaction, in an order the GPU picks;
+// id order is one such order, and `reverse_order` (erosion in descending id
+// order) another, so two runs bound how much the order alone moves a result.
 template <class T>
 void ref_particle_dispatch(State<T>& s, std::vector<T> mom[4], std::vector<RefParticle<T>>& parts,
                            const RefParticleParams<T>& pk, const Settings<T>& k, T time, bool should_rain,
-                           bool thermal, bool smooth) {
+                           bool thermal, bool smooth, bool reverse_order = false) {
     for (size_t id = 0; id < parts.size(); ++id) ref_particle_move(parts[id], uint32_t(id), s, mom, pk, time, should_rain);
-    for (auto& p : parts) ref_particle_erode(p, s, mom, pk);
+    if (reverse_order)
+        for (size_t id = parts.size(); id-- > 0;) ref_particle_erode(parts[id], s, mom, pk);
+    else
+        for (auto& p : parts) ref_particle_erode(p, s, mom, pk);
     if (thermal) {
         pass_thermal_flux(s, k, 0);
         pass_thermal_transport(s, 0);

@@ -131,7 +131,8 @@ int ref_particle_run(T* planes, int n, hg_ref_particle* parts, uint32_t count, c
     std::vector<T> mom[4] = {st.fl, st.fr, st.ft, st.fb};
     for (uint32_t it = 0; it < iterations; ++it) {
         const float time = time0 + float(it) * time_step;
-        orc::ref_particle_dispatch(st, mom, ps, pk, k, T(time), should_rain != 0, (flags & 2u) != 0, (flags & 4u) != 0);
+        orc::ref_particle_dispatch(st, mom, ps, pk, k, T(time), should_rain != 0, (flags & 2u) != 0, (flags & 4u) != 0,
+                                   (flags & 0x100u) != 0);
     }
     st.fl = mom[0]; st.fr = mom[1]; st.ft = mom[2]; st.fb = mom[3];
     for (uint32_t i = 0; i < count; ++i) {
@@ -214,6 +215,31 @@ int orc_pass_f32(float* planes, int nx, int ny, const hg_erosion_settings* s, in
     if (pass < 0 || pass >= HG_NUM_PASSES) return -1;
     return run<float>(planes, nx, ny, s, rain_mode, double(rain_amount), seed, terrain, step, 0, 0,
                       pass);
+}
+
+}  // extern "C"
+
+// Smoothing extremum flags of every cell (glsl/smoothing.glsl:40-73): bit 0
+// rock, bit 1 surface; 0 on the border, which the pass skips.
+template <class T> int smooth_flags(const T* planes, int nx, int ny, uint8_t* flags) {
+    if (!planes || !flags || nx < 3 || ny < 3) return -1;
+    orc::State<T> st;
+    load(st, planes, nx, ny);
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            bool er = false, eg = false;
+            if (x > 0 && y > 0 && x < nx - 1 && y < ny - 1) orc::smooth_flags_at(st, x, y, er, eg);
+            flags[size_t(y) * nx + x] = uint8_t((er ? 1 : 0) | (eg ? 2 : 0));
+        }
+    return 0;
+}
+
+extern "C" {
+int orc_smooth_flags_f64(const double* planes, int nx, int ny, uint8_t* flags) {
+    return smooth_flags<double>(planes, nx, ny, flags);
+}
+int orc_smooth_flags_f32(const float* planes, int nx, int ny, uint8_t* flags) {
+    return smooth_flags<float>(planes, nx, ny, flags);
 }
 
 // One droplet iteration (serial, droplets in id order) on rock/dirt planes.

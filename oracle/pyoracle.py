@@ -25,6 +25,7 @@ PASS_RAIN, PASS_FLUX, PASS_EROSION, PASS_TRANSPORT, PASS_TFLUX0, PASS_TTRANS0, P
     PASS_TTRANS1, PASS_SMOOTH = range(9)
 STEP_RAIN, STEP_THERMAL, STEP_SMOOTH = 1, 2, 4
 STEP_DEFAULT = 7
+REFP_REVERSE_ORDER = 0x100   # reference particle mode: erode in descending id order
 RAIN_OFF, RAIN_CONSTANT, RAIN_UNIFORM = 0, 1, 2
 
 
@@ -155,7 +156,8 @@ def ref_particle_run(planes: np.ndarray, parts, s: ErosionSettings, iterations: 
     n, n) float64 or float32 (FLUX planes = momentum map x, y, z, w), parts a
     ctypes array of RefParticle (rounded to FP32 after the run).  `dims` is
     Map_settings_data.hmap_dims as the shader reads it for the spawn range and
-    border test ((0, 0) = the grid's n)."""
+    border test ((0, 0) = the grid's n).  flags: STEP_THERMAL, STEP_SMOOTH, and
+    REFP_REVERSE_ORDER to erode in descending particle id order."""
     n = planes.shape[-1]
     fn = lib().orc_ref_particle_run_f64 if planes.dtype == np.float64 else lib().orc_ref_particle_run_f32
     rc = fn(_ptr(planes), n, parts, len(parts), ctypes.byref(s), iterations, int(should_rain), time0, time_step,
@@ -237,6 +239,9 @@ def lib():
                 ctypes.POINTER(t), ctypes.c_int, ctypes.POINTER(RefParticle), ctypes.c_uint32,
                 ctypes.POINTER(ErosionSettings), ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                 ctypes.c_uint, ctypes.c_int, ctypes.c_int]
+        for name, ct in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            getattr(L, f"orc_smooth_flags_{name}").argtypes = [ctypes.POINTER(ct), ctypes.c_int, ctypes.c_int,
+                                                               ctypes.POINTER(ctypes.c_uint8)]
         L.orc_ref_particle_rand_f32.argtypes = [ctypes.c_float, ctypes.c_float]
         L.orc_ref_particle_rand_f32.restype = ctypes.c_float
         L.orc_ref_particle_rand_f64.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -353,6 +358,19 @@ def run_pass(state: np.ndarray, settings: ErosionSettings, pass_id: int, rain_mo
     if rc != 0:
         raise ValueError("orc_pass failed")
     return state
+
+
+def smooth_flags(state: np.ndarray) -> np.ndarray:
+    """Extremum flags of the smoothing pass per cell (bit 0 rock, bit 1
+    surface; border cells 0), from the state's rock and dirt planes."""
+    _, ny, nx = state.shape
+    out = np.zeros((ny, nx), np.uint8)
+    fn = lib().orc_smooth_flags_f64 if state.dtype == np.float64 else lib().orc_smooth_flags_f32
+    ct = ctypes.c_double if state.dtype == np.float64 else ctypes.c_float
+    st = np.ascontiguousarray(state)
+    rc = fn(st.ctypes.data_as(ctypes.POINTER(ct)), nx, ny, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    assert rc == 0
+    return out
 
 
 def ref_simplex(x: float, y: float, dtype=np.float64) -> float:

@@ -248,6 +248,46 @@ def test_many_particles_distribution_vs_oracle():
     assert abs(w_gpu - w_64) <= 0.05 * w_64
 
 
+def test_dense_particles_atomics_vs_lock_orders():
+    """The reference serialises each particle's update of a corner with a
+    per-pixel lock (glsl/particle_erosion.glsl:89-99); here each layer is
+    updated atomically on its own, so on a dense case (4 particles per cell,
+    64^2) updates of one pixel interleave.  The distance of the GPU heights to
+    the serial FP64 oracle (id order) is set beside the distance between two
+    lock-consistent orders (FP64 in id and in descending id order) and beside
+    the GPU's own run-to-run distance."""
+    n, count, iters = 64, 16384, 4
+    rng = np.random.default_rng(11)
+    specs = [(float(x), float(y), 0.0, 0.0) for x, y in rng.uniform(3.0, n - 4.0, (count, 2))]
+    s = _settings(count)
+    runs = []
+    for _ in range(2):
+        t, parts = _setup(n, specs, s, momentum=False)
+        st = _oracle_state(t)
+        t.dispatch_particle_reference(iters, should_rain=False, time0=1.0, time_step=0.5, flags=0)
+        runs.append(t.read_state(hg.NUM_PLANES))
+        t.close()
+    fwd, rev, f32 = st.astype(np.float64), st.astype(np.float64), st.copy()
+    O.ref_particle_run(fwd, _to_oracle(parts), _os(s), iters, False, 1.0, 0.5, 0)
+    O.ref_particle_run(rev, _to_oracle(parts), _os(s), iters, False, 1.0, 0.5, O.REFP_REVERSE_ORDER)
+    O.ref_particle_run(f32, _to_oracle(parts), _os(s), iters, False, 1.0, 0.5, 0)
+
+    def height(x):
+        return x[O.ROCK].astype(np.float64) + x[O.DIRT]
+
+    def rl2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    h0, hf, hr, h32 = height(st), height(fwd), height(rev), height(f32)
+    g1, g2 = height(runs[0]), height(runs[1])
+    change, order, fp32 = rl2(h0, hf), rl2(hr, hf), rl2(h32, hf)
+    gpu, gpu_rev, gpu_gpu = rl2(g1, hf), rl2(g1, hr), rl2(g1, g2)
+    print(f"dense particles ({count} on {n}^2, {iters} dispatches): change {change:.3e}; lock orders "
+          f"{order:.3e}; fp32 oracle {fp32:.3e}; gpu vs id order {gpu:.3e}, vs descending {gpu_rev:.3e}; "
+          f"gpu run to run {gpu_gpu:.3e}", flush=True)
+    assert change > 0 and order > 0
+    assert gpu <= 3.0 * max(order, fp32) and gpu_rev <= 3.0 * max(order, fp32)
+
+
 def test_reference_particle_api_errors():
     with hg.Terrain(32) as t:
         with pytest.raises(hg.HGError):

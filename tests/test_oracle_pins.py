@@ -477,6 +477,40 @@ def test_smoothing_ramp_spike_border():
     st[O.ROCK, 2, 2] = 6.0
     out = one_pass(st, s, O.PASS_SMOOTH)
     assert np.array_equal(out[O.ROCK], st[O.ROCK])
+    assert not O.smooth_flags(st).any()
+
+
+def test_smoothing_flags_entry_follows_the_pass():
+    """orc_smooth_flags reports the extremum tests that pass_smooth applies:
+    the hand-built pit alone, and on a rough random state exactly the cells
+    that the pass moves to the 5-point mean (FP64, same blend)."""
+    s, k = settings(slippage=40.0)
+    m = min(1.0, f32(40.0) * f32(0.005))
+    nx, ny = 8, 7
+    st = O.new_state(nx, ny=ny)
+    st[O.ROCK] = 2.0
+    st[O.ROCK, 3, 4] = 1.0
+    st[O.ROCK, 3, 3] = 11.0
+    st[O.DIRT] = 0.1
+    f = O.smooth_flags(st)
+    assert np.argwhere(f & 1).tolist() == [[3, 4]]
+    # surface = rock + dirt with constant dirt: the same single pit
+    assert np.argwhere(f & 2).tolist() == [[3, 4]]
+    rng = np.random.default_rng(5)
+    nx = ny = 24
+    st = O.new_state(nx, ny=ny)
+    st[O.ROCK] = rng.uniform(0, 5, (ny, nx))
+    st[O.DIRT] = rng.uniform(0, 1.5, (ny, nx))
+    f = O.smooth_flags(st)
+    out = one_pass(st, s, O.PASS_SMOOTH)
+    for plane, bit in ((O.ROCK, 1), (O.DIRT, 2)):
+        x = st[plane]
+        mean = (x[1:-1, 1:-1] + x[1:-1, :-2] + x[1:-1, 2:] + x[2:, 1:-1] + x[:-2, 1:-1]) / 5
+        flagged = (f[1:-1, 1:-1] & bit) != 0
+        want = np.where(flagged, m * mean + (1 - m) * x[1:-1, 1:-1], m * x[1:-1, 1:-1] + (1 - m) * x[1:-1, 1:-1])
+        assert 0.1 < flagged.mean() < 0.9
+        assert np.allclose(out[plane][1:-1, 1:-1], want, rtol=1e-14, atol=0)
+    assert not f[0].any() and not f[-1].any() and not f[:, 0].any() and not f[:, -1].any()
 
 
 # ---------------------------------------------------------------------------
