@@ -204,6 +204,22 @@ def bench_particles(iterations: int, oracle_iters: int) -> dict:
     out["ms_per_iteration"] = dt / (iterations - oracle_iters) * 1e3
     out["mass_residual_rel"] = (float(h.sum()) + ex - m0) / m0
     out["exported_mass"] = ex
+    # the same workload with the droplets sorted by start tile (an option:
+    # faster, statistically further from the serial model)
+    with hg.Terrain(n) as ts:
+        ts.init_fbm(hg.fbm_params(seed=3, base_freq=1.0 / 512, hi=256.0))
+        ts.set_droplets(d)
+        ts.set_droplet_order(1)
+        ts.dispatch_particle(1)
+        ts.sync()
+        k = max(1, min(5, iterations))
+        t0 = time.perf_counter()
+        ts.dispatch_particle(k)
+        ts.sync()
+        dts = time.perf_counter() - t0
+        _, steps_s = ts.particle_stats()
+        out["sorted_order"] = {"iterations": k, "ms_per_iteration": dts / k * 1e3,
+                               "droplet_steps_per_s": steps_s * k / (k + 1) / dts}
     if oracle_iters > 0:
         od = O.droplet_settings(count=1 << 20, seed=42)
         r64, d64 = h0.astype(np.float64), np.zeros((n, n))

@@ -153,8 +153,19 @@ struct Brush {
     float w[HG_BRUSH_MAX];   // normalised to sum 1 over the full brush
 };
 Brush make_brush(int r);
+// order: null, or the droplet id each thread slot runs (a permutation of
+// [0, count), from launch_droplet_order).
 hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
-                           uint32_t iteration, double* exported, unsigned long long* steps, hipStream_t s);
+                           uint32_t iteration, const uint32_t* order, double* exported,
+                           unsigned long long* steps, hipStream_t s);
+// Droplet ordering: a counting sort of an iteration's droplets by the tile of
+// their start cell (tiles of droplet_tile_side(n) cells, row-major), so the
+// 64 droplets of a wave start close together and share cache lines.
+// tiles: droplet_tile_count(n) words of scratch; order: count words.
+int droplet_tile_side(int n);
+int droplet_tile_count(int n);
+hipError_t launch_droplet_order(int n, const DropletParams& p, uint32_t iteration, uint32_t* tiles,
+                                uint32_t* order, hipStream_t s);
 // Per-workgroup sums of one droplet launch (256 droplets per workgroup).
 struct DropletPartial {
     double exported;

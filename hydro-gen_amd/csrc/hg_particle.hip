@@ -14,6 +14,7 @@
 // Model (Beyer-style droplets; the reference's particle mode,
 // glsl/particle.glsl:64-136 and glsl/particle_erosion.glsl:22-128, is a
 // different model): see hg_droplet_settings in include/hg.h.
+#include <algorithm>
 #include <cmath>
 
 #include "hg_kernels.hpp"
@@ -65,6 +66,16 @@ __device__ __forceinline__ void deposit(float* D, int n, float px, float py, flo
     atomic_addf(D + i + n + 1, amount * u * v);
 }
 
+// Start position of droplet `id` in an iteration: uniform over [0, n-1)^2.
+__device__ __forceinline__ void droplet_start(uint32_t id, uint32_t iteration, const DropletParams& p, int n,
+                                              float& px, float& py) {
+    uint32_t o[4];
+    philox4x32_10(id, iteration, 0x64726f70u, 0u, p.seed_lo, p.seed_hi, o);
+    const float span = float(n - 1);
+    px = u24(o[0]) * span;
+    py = u24(o[1]) * span;
+}
+
 // PARTIALS = false: the workgroup's exported mass and droplet steps are added
 // atomically to *exported_out and *steps_out.  PARTIALS = true: they are
 // written to partials[blockIdx.x], for k_droplet_totals to sum in a fixed
@@ -72,19 +83,19 @@ __device__ __forceinline__ void deposit(float* D, int n, float px, float py, flo
 template <bool PARTIALS>
 __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* __restrict__ D, int n,
                                                   DropletParams p, Brush brush, uint32_t iteration,
-                                                  double* exported_out,
+                                                  const uint32_t* __restrict__ order, double* exported_out,
                                                   unsigned long long* steps_out,
                                                   DropletPartial* partials) {
     __shared__ double s_exp[256];
     __shared__ unsigned s_steps[256];
-    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
     double exported = 0.0;
     unsigned steps = 0;
-    if (id < p.count) {
-        uint32_t o[4];
-        philox4x32_10(id, iteration, 0x64726f70u, 0u, p.seed_lo, p.seed_hi, o);
+    if (slot < p.count) {
+        const uint32_t id = order ? order[slot] : slot;
         const float span = float(n - 1);
-        float px = u24(o[0]) * span, py = u24(o[1]) * span;
+        float px, py;
+        droplet_start(id, iteration, p, n, px, py);
         float dx = 0.0f, dy = 0.0f, speed = p.init_speed, water = p.init_water, sed = 0.0f;
         bool alive = true;
         for (uint32_t life = 0; life < p.max_lifetime; ++life) {
@@ -183,6 +194,59 @@ __global__ __launch_bounds__(256) void k_droplets(float* __restrict__ R, float* 
     }
 }
 
+// ---- droplet ordering (counting sort by start tile) --------------------------
+
+__device__ __forceinline__ uint32_t start_tile(uint32_t id, uint32_t iteration, const DropletParams& p, int n,
+                                               int side, int ntx) {
+    float px, py;
+    droplet_start(id, iteration, p, n, px, py);
+    return uint32_t(int(py) / side) * uint32_t(ntx) + uint32_t(int(px) / side);
+}
+
+__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* a, uint32_t count) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) a[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_drop_count(DropletParams p, int n, int side, int ntx, uint32_t iteration,
+                                                    uint32_t* tiles) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id < p.count) atomicAdd(&tiles[start_tile(id, iteration, p, n, side, ntx)], 1u);
+}
+
+// Exclusive prefix sum of tiles[0, count) in place, one workgroup of 1024
+// threads: each thread sums a contiguous run, the run totals are scanned in
+// LDS, then each run is rewritten as offsets.
+__global__ __launch_bounds__(1024) void k_drop_scan(uint32_t* tiles, uint32_t count) {
+    __shared__ uint32_t s[1024];
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per, e = min(b + per, count);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += tiles[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {          // inclusive Hillis-Steele scan
+        const uint32_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;                      // exclusive
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t c = tiles[i];
+        tiles[i] = run;
+        run += c;
+    }
+}
+
+// Places droplet id at its sorted position (its tile's next free offset).
+__global__ __launch_bounds__(256) void k_drop_scatter(DropletParams p, int n, int side, int ntx, uint32_t iteration,
+                                                      uint32_t* tiles, uint32_t* order) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= p.count) return;
+    order[atomicAdd(&tiles[start_tile(id, iteration, p, n, side, ntx)], 1u)] = id;
+}
+
 // Sums partials[0, nblocks) in a fixed order (thread t: blocks t, t + 256, ...;
 // then a fixed tree) into *exported_out and *steps_out (either may be null),
 // replacing them, or adding to them with `accumulate`.
@@ -238,10 +302,30 @@ Brush make_brush(int r) {
 }
 
 hipError_t launch_droplets(float* rock, float* dirt, int n, const DropletParams& p, const Brush& brush,
-                           uint32_t iteration, double* exported, unsigned long long* steps, hipStream_t s) {
+                           uint32_t iteration, const uint32_t* order, double* exported,
+                           unsigned long long* steps, hipStream_t s) {
     if (p.count == 0) return hipSuccess;
-    k_droplets<false><<<(p.count + 255) / 256, 256, 0, s>>>(rock, dirt, n, p, brush, iteration, exported, steps,
-                                                             nullptr);
+    k_droplets<false><<<(p.count + 255) / 256, 256, 0, s>>>(rock, dirt, n, p, brush, iteration, order, exported,
+                                                             steps, nullptr);
+    return hipGetLastError();
+}
+
+int droplet_tile_side(int n) { return std::max(16, (n + 127) / 128); }
+
+int droplet_tile_count(int n) {
+    const int side = droplet_tile_side(n), nt = (n + side - 1) / side;
+    return nt * nt;
+}
+
+hipError_t launch_droplet_order(int n, const DropletParams& p, uint32_t iteration, uint32_t* tiles,
+                                uint32_t* order, hipStream_t s) {
+    if (p.count == 0) return hipSuccess;
+    const int side = droplet_tile_side(n), ntx = (n + side - 1) / side;
+    const uint32_t nt = uint32_t(ntx) * uint32_t(ntx), g = (p.count + 255) / 256;
+    k_zero_u32<<<(nt + 255) / 256, 256, 0, s>>>(tiles, nt);
+    k_drop_count<<<g, 256, 0, s>>>(p, n, side, ntx, iteration, tiles);
+    k_drop_scan<<<1, 1024, 0, s>>>(tiles, nt);
+    k_drop_scatter<<<g, 256, 0, s>>>(p, n, side, ntx, iteration, tiles, order);
     return hipGetLastError();
 }
 
@@ -249,7 +333,7 @@ hipError_t launch_droplets_partials(float* rock, float* dirt, int n, const Dropl
                                     uint32_t iteration, DropletPartial* partials, hipStream_t s) {
     if (p.count == 0) return hipSuccess;
     k_droplets<true><<<(p.count + 255) / 256, 256, 0, s>>>(rock, dirt, n, p, brush, iteration, nullptr, nullptr,
-                                                            partials);
+                                                            nullptr, partials);
     return hipGetLastError();
 }
 

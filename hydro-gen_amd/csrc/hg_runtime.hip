@@ -374,6 +374,7 @@ int hg_destroy(hg_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->adj_block) (void)hipFree(c->adj_block);
     if (c->rgba_dev) (void)hipFree(c->rgba_dev);
+    if (c->drop_order) (void)hipFree(c->drop_order);
     for (int b = 0; b < 2; ++b) {
         if (c->pin[b]) (void)hipHostFree(c->pin[b]);
         if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
@@ -555,6 +556,19 @@ int hg_set_droplets(hg_ctx* c, const hg_droplet_settings* s) {
     return HG_OK;
 }
 
+int hg_set_droplet_order(hg_ctx* c, int mode) {
+    if (!c) return HG_ERR_INVALID;
+    if (mode < 0 || mode > 1) return fail(c, HG_ERR_INVALID, "hg_set_droplet_order: mode must be 0 or 1");
+    c->droplet_order = mode;
+    return HG_OK;
+}
+
+int hg_get_droplet_order(const hg_ctx* c, int* mode) {
+    if (!c || !mode) return HG_ERR_INVALID;
+    *mode = c->droplet_order;
+    return HG_OK;
+}
+
 int hg_step_particle(hg_ctx* c, uint32_t iterations, uint32_t flags) {
     if (!c) return HG_ERR_INVALID;
     if (flags & ~(HG_STEP_THERMAL | HG_STEP_SMOOTH)) return fail(c, HG_ERR_INVALID, "hg_step_particle: only thermal/smooth flags");
@@ -567,11 +581,36 @@ int hg_step_particle(hg_ctx* c, uint32_t iterations, uint32_t flags) {
     p.seed_lo = uint32_t(c->ds.seed); p.seed_hi = uint32_t(c->ds.seed >> 32);
     const hgk::Brush brush = hgk::make_brush(c->ds.brush_radius);
     const bool thermal = (flags & HG_STEP_THERMAL) != 0, smooth = (flags & HG_STEP_SMOOTH) != 0;
+    const bool ordered = c->droplet_order > 0 && p.count > 0;
+    uint32_t* tiles = nullptr;
+    if (ordered) {
+        // slot -> id for `count` droplets, then the per-tile counts / offsets
+        const uint32_t need = p.count + uint32_t(hgk::droplet_tile_count(c->n));
+        if (need > c->drop_cap) {
+            HG_HIP(c, hipStreamSynchronize(c->stream));
+            if (c->drop_order) (void)hipFree(c->drop_order);
+            c->drop_order = nullptr;
+            c->drop_cap = 0;
+            void* q = nullptr;
+            HG_HIP(c, hipMalloc(&q, size_t(need) * sizeof(uint32_t)));
+            c->drop_order = static_cast<uint32_t*>(q);
+            c->drop_cap = need;
+        }
+        tiles = c->drop_order + p.count;
+    }
     for (uint32_t it = 0; it < iterations; ++it) {
         const uint32_t iter = c->particle_iter;
-        int rc = timed_launch(c, "droplets", [&] {
+        int rc = HG_OK;
+        if (ordered) {
+            rc = timed_launch(c, "droplet_order", [&] {
+                return hgk::launch_droplet_order(c->n, p, iter, tiles, c->drop_order, c->stream);
+            });
+            if (rc != HG_OK) return rc;
+        }
+        rc = timed_launch(c, "droplets", [&] {
             return hgk::launch_droplets(c->cur[HG_P_ROCK], c->cur[HG_P_DIRT], c->n, p, brush, iter,
-                                        c->dev_exported, c->dev_dsteps, c->stream);
+                                        ordered ? c->drop_order : nullptr, c->dev_exported, c->dev_dsteps,
+                                        c->stream);
         });
         if (rc != HG_OK) return rc;
         if (thermal || smooth) {

@@ -38,6 +38,7 @@ EXPORTS = [
     "hg_set_step", "hg_read", "hg_write", "hg_save_height", "hg_load_height", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_droplets_noalloc", "hg_adjoint_rollout",
+    "hg_set_droplet_order", "hg_get_droplet_order",
     "hg_ref_particles_init", "hg_ref_particles_read", "hg_ref_particles_write", "hg_step_particle_reference",
     "hg_ref_particle_stats", "hg_set_ref_particle_dims",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
@@ -264,6 +265,8 @@ def load() -> ctypes.CDLL:
     L.hg_tile_variant_count.argtypes = [ctypes.c_int]
     L.hg_tile_variant_info.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
     L.hg_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hg_set_droplet_order.argtypes = [vp, ctypes.c_int]
+    L.hg_get_droplet_order.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.hg_set_sweep.argtypes = [vp, ctypes.c_int]
     L.hg_get_sweep.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.hg_get_tiles.argtypes = [vp, ip, ip]
@@ -384,6 +387,16 @@ class Terrain:
 
     def set_droplets(self, d: DropletSettings):
         self._check(self.lib.hg_set_droplets(self.h, ctypes.byref(d)))
+
+    def set_droplet_order(self, mode: int):
+        """hg_set_droplet_order: 0 thread j runs droplet j, 1 droplets sorted
+        by the tile of their start cell."""
+        self._check(self.lib.hg_set_droplet_order(self.h, mode))
+
+    def get_droplet_order(self) -> int:
+        v = ctypes.c_int()
+        self._check(self.lib.hg_get_droplet_order(self.h, ctypes.byref(v)))
+        return v.value
 
     def dispatch_particle(self, iterations: int = 1, flags: int = 0):
         """Droplet iterations (Erosion::dispatch_particle counterpart)."""

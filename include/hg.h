@@ -313,6 +313,23 @@ int hg_set_droplets(hg_ctx* ctx, const hg_droplet_settings* s);
  * the thermal and smoothing passes after each iteration, like the reference's
  * particle dispatch. */
 int hg_step_particle(hg_ctx* ctx, uint32_t iterations, uint32_t flags);
+/* Which thread runs which droplet of an iteration (the droplets and their
+ * start positions are the same in both modes; only the timing of their
+ * atomic updates relative to each other changes):
+ *   0  (default) thread j runs droplet j, so the 64 droplets of a wave start
+ *      at unrelated places;
+ *   1  droplets sorted by the tile of their start cell (16 x 16 cells, or
+ *      n/128 for n > 2048), so a wave's droplets start close together and
+ *      share cache lines: measured 1.19x faster at 2048^2 with 2^20
+ *      droplets, but neighbouring droplets then run in lockstep and do not
+ *      see each other's changes within a step, which moves the statistics
+ *      further from the serial model (the spread of heights after 4
+ *      iterations of 8192 droplets on 256^2: 5.7 % off the serial oracle,
+ *      against 0.6-2.6 % in mode 0).
+ * Mode 1 keeps count + (number of tiles, at most 16384) words of device
+ * memory in the context.  No counterpart in the reference. */
+int hg_set_droplet_order(hg_ctx* ctx, int mode);
+int hg_get_droplet_order(const hg_ctx* ctx, int* mode);
 /* Cumulative since hg_init_fbm: mass exported over the map edge and the
  * number of droplet steps simulated. */
 int hg_particle_stats(hg_ctx* ctx, double* exported_mass, uint64_t* droplet_steps);
