@@ -293,11 +293,25 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     t0 = time.perf_counter()
     gh, gc = t.adjoint_rollout(steps, k_ckpt, g)
     t_adj = time.perf_counter() - t0
+    # the same rollout taped whole by the forward (one segment: no
+    # recompute; 11 + 9 planes per step stay in HBM, 20 GB at 1024^2 x 256)
+    t.write_state(init)
+    t.step = 0
+    t.adjoint_rollout(steps, steps, g)
+    t.write_state(init)
+    t.step = 0
+    t.sync()
+    t0 = time.perf_counter()
+    gh_full, gc_full = t.adjoint_rollout(steps, steps, g)
+    t_full = time.perf_counter() - t0
     t.close()
     return {"grid": n, "steps": steps, "k_ckpt": k_ckpt, "tile_variants": list(tiles), "loss": loss,
             "t_forward_ms": t_fwd * 1e3, "t_forward_adjoint_ms": t_adj * 1e3,
             "cost_ratio": t_adj / t_fwd, "grad_coef": [float(x) for x in gc],
-            "grad_h0_absmax": float(np.abs(gh).max()), "grad_finite": bool(np.isfinite(gh).all())}
+            "grad_h0_absmax": float(np.abs(gh).max()), "grad_finite": bool(np.isfinite(gh).all()),
+            "full_tape": {"k_ckpt": steps, "t_forward_adjoint_ms": t_full * 1e3, "cost_ratio": t_full / t_fwd,
+                          "same_gradient": bool(np.array_equal(gh_full, gh)
+                                                and np.array_equal(np.asarray(gc_full), np.asarray(gc)))}}
 
 
 def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> dict:
@@ -360,6 +374,8 @@ def main() -> int:
     ap.add_argument("--particle-oracle-iters", type=int, default=1)
     ap.add_argument("--adjoint-steps", type=int, default=256,
                     help="steps of the differentiable-rollout workload (0 = skip)")
+    ap.add_argument("--adjoint-ckpt", type=int, default=16,
+                    help="checkpoint interval of the differentiable-rollout workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy (PMC counter calibration)")
     ap.add_argument("--sweep", type=int, default=None,
@@ -431,7 +447,8 @@ def main() -> int:
         log("reference particle mode done")
     if particles:
         log("droplets done")
-    adjoint = bench_adjoint(steps=args.adjoint_steps) if args.adjoint_steps > 0 else None
+    adjoint = (bench_adjoint(steps=args.adjoint_steps, k_ckpt=args.adjoint_ckpt)
+               if args.adjoint_steps > 0 else None)
     if adjoint:
         log("adjoint done")
     copy = calibrate_copy() if args.calibrate_copy else None
