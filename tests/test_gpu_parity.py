@@ -125,6 +125,31 @@ def test_fused_equals_per_pass_long_backtrace():
         assert rel_l2(fused[p], ref[p]) < 2e-5, p
 
 
+@pytest.mark.parametrize("n", [200, 256])
+def test_fused_tile_variants_equal_per_pass(n):
+    """Every compiled tile variant of both fused kernels (including the
+    double-buffered thermal ones) gives the per-pass result bit for bit, with
+    thermal erosion on and off."""
+    st = spun_up_state(n, 20, seed=7)
+    nh, nt = len(hg.tile_variants(hg.KERNEL_HYDRO)), len(hg.tile_variants(hg.KERNEL_THERMAL))
+    for flags in (hg.STEP_DEFAULT, hg.STEP_RAIN | hg.STEP_SMOOTH):
+        with _terrain(n) as t:
+            t.write_state(st)
+            t.step = 3
+            t.dispatch_grid(3, flags | hg.STEP_PER_PASS)
+            plain = t.read_state(hg.NUM_STATE_PLANES)
+        for hv, tv in [(v, 0) for v in range(nh)] + [(0, v) for v in range(1, nt)]:
+            with _terrain(n) as t:
+                t.set_sweep(-1)
+                t.set_tiles(hv, tv)
+                t.write_state(st)
+                t.step = 3
+                t.dispatch_grid(3, flags)
+                got = t.read_state(hg.NUM_STATE_PLANES)
+            for p in range(hg.NUM_STATE_PLANES):
+                assert np.array_equal(got[p], plain[p]), (n, flags, hv, tv, p)
+
+
 @pytest.mark.parametrize("flags", [hg.STEP_RAIN, hg.STEP_RAIN | hg.STEP_THERMAL,
                                    hg.STEP_RAIN | hg.STEP_SMOOTH, hg.STEP_THERMAL | hg.STEP_SMOOTH])
 def test_fused_equals_per_pass_flag_subsets(flags):
