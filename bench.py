@@ -376,8 +376,10 @@ def main() -> int:
                     help="steps of the differentiable-rollout workload (0 = skip)")
     ap.add_argument("--adjoint-ckpt", type=int, default=16,
                     help="checkpoint interval of the differentiable-rollout workload")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a 256 MiB device copy (PMC counter calibration)")
+    ap.add_argument("--calibrate-copy", action="store_true", default=True,
+                    help="run a 256 MiB device-to-device copy: the box's copy bandwidth next to the "
+                         "8 TB/s peak, and the PMC byte counters' calibration (default)")
+    ap.add_argument("--no-calibrate-copy", dest="calibrate_copy", action="store_false")
     ap.add_argument("--sweep", type=int, default=None,
                     help="force the step kernel: -1 two kernels, v >= 0 one-kernel variant v "
                          "(default: the autotuner's choice)")
