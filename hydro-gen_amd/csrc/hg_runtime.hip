@@ -1007,8 +1007,15 @@ int hg_autotune(hg_ctx* c, uint32_t steps, uint32_t flags, int* hydro_best, int*
     void* backup = nullptr;
     HG_HIP(c, hipMalloc(&backup, pn * sizeof(float) * HG_NUM_STATE_PLANES));
     float* bk = static_cast<float*>(backup);
-    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p)
-        HG_HIP(c, hipMemcpyAsync(bk + p * pn, c->cur[p], pn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    for (int p = 0; p < HG_NUM_STATE_PLANES; ++p) {
+        const hipError_t e = hipMemcpyAsync(bk + p * pn, c->cur[p], pn * sizeof(float), hipMemcpyDeviceToDevice,
+                                            c->stream);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(backup);
+            return hgctx::hip_fail(c, e, "hg_autotune: back up the state");
+        }
+    }
     const uint64_t step0 = c->step;
     const bool saved_timing = c->timing;
     const bool rain = (flags & HG_STEP_RAIN) && (c->rain_mode == HG_RAIN_CONSTANT ||
