@@ -43,7 +43,7 @@ struct hg_ctx {
     uint32_t particle_iter = 0;
     int droplet_order = 0;           // hg_set_droplet_order
     uint32_t* drop_order = nullptr;  // slot -> droplet id (drop_cap words), then the tile offsets
-    uint32_t drop_cap = 0;
+    size_t drop_cap = 0;
     double* dev_exported = nullptr;  // cumulative droplet statistics (device)
     unsigned long long* dev_dsteps = nullptr;
     int hydro_variant = 0;           // fused kernel tile variants (see hg_autotune)

@@ -585,7 +585,7 @@ int hg_step_particle(hg_ctx* c, uint32_t iterations, uint32_t flags) {
     uint32_t* tiles = nullptr;
     if (ordered) {
         // slot -> id for `count` droplets, then the per-tile counts / offsets
-        const uint32_t need = p.count + uint32_t(hgk::droplet_tile_count(c->n));
+        const size_t need = size_t(p.count) + size_t(hgk::droplet_tile_count(c->n));
         if (need > c->drop_cap) {
             HG_HIP(c, hipStreamSynchronize(c->stream));
             if (c->drop_order) (void)hipFree(c->drop_order);
