@@ -337,7 +337,10 @@ def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> dict:
 
 def cpu_baseline(t: hg.Terrain, steps: int) -> dict:
     """FP32 oracle (CPU port of the same passes) on `steps` steps of the same
-    4096^2 state; OpenMP threads from OMP_NUM_THREADS."""
+    4096^2 state; OpenMP threads from OMP_NUM_THREADS, bound close.  Also the
+    512^2 plain configuration (fBm [0, 64], rain 0.01) for 100 steps with the
+    FP32 port and with the FP64 oracle, timed apart."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     state = np.zeros((O.NUM_PLANES, t.n, t.n), dtype=np.float32)
@@ -349,9 +352,20 @@ def cpu_baseline(t: hg.Terrain, steps: int) -> dict:
     O.run(state, s, steps, O.STEP_DEFAULT, O.RAIN_UNIFORM, 0.02, seed=1, step0=t.step + 1)
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # the 512^2 configuration, FP32 port and FP64 oracle
+    c0 = {}
+    n0, k0 = 512, 100
+    init = O.fbm_init(n0, O.fbm_params(seed=1, base_freq=1.0 / 256, hi=64.0), np.float64)
+    for name, dtype in (("fp32_port", np.float32), ("fp64_oracle", np.float64)):
+        st = O.new_state(n0).astype(dtype)
+        st[O.ROCK] = init
+        t0 = time.perf_counter()
+        O.run(st, s, k0, O.STEP_DEFAULT, O.RAIN_CONSTANT, 0.01, seed=1)
+        c0[name] = n0 * n0 * k0 / (time.perf_counter() - t0)
     return {"value": t.n * t.n * steps / dt, "unit": "cell-steps/s", "cores": cores, "kind": "port",
             "sample": f"{steps} FP32 steps of the same {t.n}x{t.n} workload (state after the "
-                      f"warm-up), oracle/liboracle.so, {cores} OpenMP threads, {dt:.1f} s"}
+                      f"warm-up), oracle/liboracle.so, {cores} OpenMP threads, {dt:.1f} s",
+            "plain_512": {"steps": k0, "unit": "cell-steps/s", **c0}}
 
 
 def log(msg: str) -> None:
