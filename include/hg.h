@@ -448,13 +448,14 @@ int hg_get_tiles(const hg_ctx* ctx, int* hydro_variant, int* thermal_variant);
  * shares with its horizontal and vertical neighbours are read through one
  * L2 (the default).  Returns HG_ERR_INVALID for other modes. */
 int hg_set_xcd_order(hg_ctx* ctx, int mode);
-/* One-kernel steps.  With a HG_KERNEL_SWEEP variant (the default is 0),
- * hg_step_grid runs each fused step as one kernel: each workgroup sweeps a
- * strip of columns row by row and runs every pass of the step a few rows
- * behind the one it depends on, so the step reads and writes its 9 state
- * planes once.  -1 runs the hydro kernel and then the thermal kernel.  Both
- * give the per-pass result bit for bit; steps with HG_STEP_KEEP_VELOCITY
- * always use the two kernels. */
+/* One-kernel steps.  With a HG_KERNEL_SWEEP variant, hg_step_grid runs
+ * each fused step as one kernel: each wave (variants 0-4) or workgroup (the
+ * rest) sweeps a strip of columns row by row and runs every pass of the step
+ * a few rows behind the one it depends on, so the step reads and writes its
+ * 9 state planes once.  -1 (the default, kept by hg_autotune unless a
+ * one-kernel variant times faster) runs the hydro kernel and then the
+ * thermal kernel.  Both give the per-pass result bit for bit; steps with
+ * HG_STEP_KEEP_VELOCITY always use the two kernels. */
 int hg_set_sweep(hg_ctx* ctx, int variant);
 int hg_get_sweep(const hg_ctx* ctx, int* variant);
 /* Times `steps` fused steps per variant (state and step counter are restored
