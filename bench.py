@@ -111,16 +111,17 @@ def bench_batched(steps: int, base_steps: int, graph_steps: int = 100) -> dict:
     # baseline: per-terrain launches of the same fused kernels (each over its
     # terrain's tiles), `graph_steps` steps captured in one hipGraph, so launch
     # overhead is off the clock for the baseline as well
-    b.set_persistent(False)
-    b.capture(graph_steps, hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN)
-    b.run()
-    b.sync()
-    t0 = time.perf_counter()
-    b.run()
-    b.sync()
-    dt = time.perf_counter() - t0
-    out["per_terrain_fused_graph"] = {"steps": graph_steps, "cell_steps_per_s": cells * graph_steps / dt,
-                                      "ms_per_step": dt / graph_steps * 1e3}
+    if graph_steps > 0:
+        b.set_persistent(False)
+        b.capture(graph_steps, hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN)
+        b.run()
+        b.sync()
+        t0 = time.perf_counter()
+        b.run()
+        b.sync()
+        dt = time.perf_counter() - t0
+        out["per_terrain_fused_graph"] = {"steps": graph_steps, "cell_steps_per_s": cells * graph_steps / dt,
+                                          "ms_per_step": dt / graph_steps * 1e3}
     # batched: one launch per pass over all terrains' tiles, `steps` steps
     # captured in one hipGraph; flat grid (a workgroup per tile) and persistent grid
     best = None
@@ -141,7 +142,8 @@ def bench_batched(steps: int, base_steps: int, graph_steps: int = 100) -> dict:
     v = best
     out["speedup_vs_per_terrain_per_pass"] = v / out["per_terrain_per_pass"]["cell_steps_per_s"]
     out["speedup_vs_per_terrain_fused"] = v / out["per_terrain_fused"]["cell_steps_per_s"]
-    out["speedup_vs_per_terrain_fused_graph"] = v / out["per_terrain_fused_graph"]["cell_steps_per_s"]
+    if "per_terrain_fused_graph" in out:
+        out["speedup_vs_per_terrain_fused_graph"] = v / out["per_terrain_fused_graph"]["cell_steps_per_s"]
     h = b.read_plane(len(sides) - 1, hg.P_ROCK)
     out["state_finite"] = bool(np.isfinite(h).all())
     b.close()
@@ -383,6 +385,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=96, help="CPU baseline sample (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=1000,
                     help="steps of the 64-terrain batched graph (0 = skip)")
+    ap.add_argument("--per-terrain-graph-steps", type=int, default=100,
+                    help="steps of the per-terrain baseline graph (128 launches per step; 0 = skip)")
     ap.add_argument("--particle-iters", type=int, default=100,
                     help="droplet iterations of the particle workload (0 = skip)")
     ap.add_argument("--particle-oracle-iters", type=int, default=1)
@@ -453,7 +457,8 @@ def main() -> int:
     if cpu:
         log(f"CPU baseline: {cpu['value']:.3e} cell-steps/s")
     t.close()
-    batched = bench_batched(args.batched_steps, 10) if args.batched_steps > 0 else None
+    batched = (bench_batched(args.batched_steps, 10, args.per_terrain_graph_steps)
+               if args.batched_steps > 0 else None)
     if batched:
         log("batched terrains done")
     particles = (bench_particles(args.particle_iters, args.particle_oracle_iters)
