@@ -129,9 +129,34 @@ __global__ void k_mm_reset(unsigned* mm) {
 // (glsl/simplex_noise.glsl:579-676): the Hugo Elias integer hash, gradient
 // noise with quintic interpolation and analytic derivatives (noised), its fBm
 // (perlfbm) and the derivative-damped fBm (erosion_perlfbm).  Every operation
-// follows the shader's order in FP32 (int32 arithmetic wraps), so the host's
-// FP32 restatement agrees bit for bit except through exp().
+// follows the shader's order in FP32 (int32 arithmetic wraps), and exp() is
+// gen_expf below, whose operations a host compiles alike, so a host FP32
+// restatement can agree bit for bit.
 // ---------------------------------------------------------------------------
+
+// exp(x) in FP32 for the exp mask (glsl/heightmap.glsl:43-44): Cody and
+// Waite's argument reduction x = k ln2 + r, with ln2 split into a 12-bit head
+// (k * head is exact for |k| < 2^12) and a tail, so |r| <= ln2 / 2 + a few
+// ulp; exp(r) by its degree-7 Taylor polynomial in Horner form (the first
+// omitted term, r^8 / 8!, is below 2^-27); then the exact scaling by 2^k.
+// Within 2 ulp of exp over the normal range.  GLSL leaves exp's precision to
+// the implementation; this one is written so that the same operations run on
+// the device and on a host (no contraction: -ffp-contract=off).
+__device__ __forceinline__ float gen_expf(float x) {
+    if (!(x < 88.5f)) return x != x ? x : __builtin_huge_valf();   // NaN, or overflow
+    if (x < -87.0f) return 0.0f;                                   // below the normal range
+    const float k = rintf(x * 1.44269504088896341f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860682030941723e-6f;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, int(k));
+}
 struct MapParams {
     float max_height, max_dirt, height_mult, seed, persistance, lacunarity, scale, uplift_scale,
         domain_warp_scale, terrace_scale;
@@ -259,7 +284,7 @@ __global__ __launch_bounds__(256) void k_gen_heightmap(float* rock, float* dirt,
     }
     if (m.mask_exp != 0) {
         hm += 2.0f;
-        val = (val * (expf(val) - 1.0f)) / 1.718f;
+        val = (val * (gen_expf(val) - 1.0f)) / 1.718f;
     }
     if (m.mask_power != 0) {
         hm += 1.25f;

@@ -48,6 +48,27 @@ using std::floor;
 using std::fabs;
 using std::exp;
 
+// exp of the reference generator's exp mask.  FP64: the library exp.  FP32:
+// the method the device kernel uses (Cody and Waite's reduction
+// x = k ln2 + r with a 12-bit head of ln2, the degree-7 Taylor polynomial of
+// exp(r), exact scaling by 2^k), restated here; pinned against FP64 exp.
+inline double gen_exp(double x) { return std::exp(x); }
+inline float gen_exp(float x) {
+    if (!(x < 88.5f)) return x != x ? x : HUGE_VALF;
+    if (x < -87.0f) return 0.0f;
+    const float k = std::rint(x * 1.44269504088896341f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860682030941723e-6f;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return std::ldexp(p, int(k));
+}
+
 // FP32 literal 999999999999.0 used by the reference for out-of-grid heights
 // (glsl/hydro_flux.glsl:45-52, glsl/thermal_erosion.glsl:20-26).
 constexpr float OOB_HEIGHT_F = 999999999999.0f;
@@ -442,7 +463,7 @@ template <class T> void gen_heightmap_cell(int x, int y, int n, const MapCfg& m,
     }
     if (m.mask_exp != 0) {
         hm = hm + T(2);
-        v = (v * (exp(v) - T(1))) / T(1.718f);
+        v = (v * (gen_exp(v) - T(1))) / T(1.718f);
     }
     if (m.mask_power != 0) {
         hm = hm + T(1.25f);

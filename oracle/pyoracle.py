@@ -242,6 +242,8 @@ def lib():
         for name, ct in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
             getattr(L, f"orc_smooth_flags_{name}").argtypes = [ctypes.POINTER(ct), ctypes.c_int, ctypes.c_int,
                                                                ctypes.POINTER(ctypes.c_uint8)]
+        L.orc_gen_exp_f32.argtypes = [ctypes.c_float]
+        L.orc_gen_exp_f32.restype = ctypes.c_float
         L.orc_ref_particle_rand_f32.argtypes = [ctypes.c_float, ctypes.c_float]
         L.orc_ref_particle_rand_f32.restype = ctypes.c_float
         L.orc_ref_particle_rand_f64.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -358,6 +360,11 @@ def run_pass(state: np.ndarray, settings: ErosionSettings, pass_id: int, rain_mo
     if rc != 0:
         raise ValueError("orc_pass failed")
     return state
+
+
+def gen_exp_f32(x: float) -> float:
+    """FP32 exp of the reference generator's exp mask (orc::gen_exp)."""
+    return lib().orc_gen_exp_f32(x)
 
 
 def smooth_flags(state: np.ndarray) -> np.ndarray:

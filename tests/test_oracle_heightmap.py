@@ -135,3 +135,67 @@ def test_generator_cells_match_python_composition(kw):
         assert rock[y, x] == pytest.approx(r, rel=1e-9, abs=1e-9), (x, y)
         assert dirt[y, x] == pytest.approx(d, rel=1e-12), (x, y)
     assert np.isfinite(rock).all() and (rock <= m.max_height).all()
+
+
+def test_generator_fp32_exp_within_two_ulp_of_exp():
+    """The FP32 exp of the exp mask (the method the device kernel runs:
+    Cody-Waite reduction, degree-7 Taylor polynomial, scaling by 2^k)
+    against FP64 exp: within 2 ulp over [-30, 30] densely and over the
+    normal range at random; exact at 0; overflow and underflow ends."""
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([np.linspace(-30, 30, 60001, dtype=np.float32),
+                         rng.uniform(-86.9, 88.4, 20000).astype(np.float32),
+                         np.float32([1e-8, -1e-8, 0.5, -0.5, 1.0, -1.0])])
+    worst = 0.0
+    for x in xs:
+        got = float(np.float32(O.gen_exp_f32(float(x))))
+        want = math.exp(float(x))
+        worst = max(worst, abs(got - want) / float(np.spacing(np.float32(want))))
+    assert worst <= 2.0, worst
+    assert O.gen_exp_f32(0.0) == 1.0
+    assert math.isinf(O.gen_exp_f32(89.0)) and O.gen_exp_f32(-90.0) == 0.0
+    # a dropped polynomial term or a wrong ln2 split is off by far more
+    assert abs(O.gen_exp_f32(0.34) - math.exp(0.34)) < 2 * float(np.spacing(np.float32(1.4)))
+
+
+def test_generator_masks_and_terrace_closed_forms():
+    """Each mask and the terrace against the plain noise height of the same
+    cells (FP64): the round mask multiplies by 5 max(0, 1 - (|uv - 0.5|^2 +
+    0.75)) (zero outside the disc), the exp mask maps v to v (e^v - 1) / 1.718
+    and adds 2 to the height multiplier, the power mask maps v to
+    v ((((v + 0.5)^3 - 0.125) / 3.25) 0.55 + 0.45) and adds 1.25, the slope
+    mask maps v to v (0.25 + 0.75 uvx uvy), and a terrace with scale 1 puts
+    every height on a multiple of max_height / terrace
+    (glsl/heightmap.glsl:94-135)."""
+    n = 48
+    base = dict(mask_round=0, mask_exp=0, mask_power=0, mask_slope=0, uplift=0, terrace=0,
+                max_height=4096.0, seed=3.5)
+    rock0, _ = O.gen_heightmap(n, O.map_settings(**base), np.float64)
+    maxh, hm = 4096.0, 1.0
+    v = rock0 / (maxh * hm)                      # unclamped: max_height is far above the noise
+    assert np.abs(v).max() < 0.9 and np.abs(v).max() > 0.05
+    y, x = np.mgrid[0:n, 0:n].astype(np.float64)
+    ux, uy = x / n, y / n
+
+    def run(**kw):
+        return O.gen_heightmap(n, O.map_settings(**{**base, **kw}), np.float64)[0]
+
+    w = 1.0 - ((ux - 0.5) ** 2 + (uy - 0.5) ** 2 + 0.75)
+    got = run(mask_round=1)
+    assert np.allclose(got, 5 * v * np.maximum(0.0, w) * maxh * hm, rtol=1e-12, atol=1e-9)
+    assert (got[w <= 0] == 0).all() and (got[w > 0] != 0).mean() > 0.9
+    got = run(mask_exp=1)
+    assert np.allclose(got, v * (np.exp(v) - 1) / np.float32(1.718) * maxh * (hm + 2), rtol=1e-12, atol=1e-9)
+    got = run(mask_power=1)
+    b = v + 0.5
+    c55, c45 = float(np.float32(0.55)), float(np.float32(0.45))   # the shader's FP32 literals
+    assert np.allclose(got, v * (((b ** 3 - 0.125) / 3.25) * c55 + c45) * maxh * (hm + 1.25),
+                       rtol=1e-12, atol=1e-9)
+    got = run(mask_slope=1)
+    assert np.allclose(got, v * (0.25 + 0.75 * ux * uy) * maxh * hm, rtol=1e-12, atol=1e-9)
+    for steps in (3, 7):
+        got = run(terrace=steps, terrace_scale=1.0)
+        levels = got * steps / maxh
+        assert np.allclose(levels, np.round(levels), atol=1e-9)
+        assert np.array_equal(np.round(levels), np.floor(v * steps * hm))
+
