@@ -29,8 +29,9 @@
 //
 // Arithmetic is FP32 in the shaders' order; division and square root are the
 // correctly rounded ones and mix(a, b, t) is a + t (b - a), so that one
-// particle on its own reproduces the FP32 oracle bit for bit.  The spawn hash
-// uses sinf, whose last bit may differ from the host's sinf.
+// particle on its own reproduces the FP32 oracle bit for bit.  The spawn
+// hash's sin is hash_sinf below, whose operations a host compiles alike, so
+// spawns reproduce it too.
 #include <cmath>
 
 #include "hg_kernels.hpp"
@@ -43,9 +44,32 @@ constexpr int kCasTries = 4096;
 
 __device__ __forceinline__ float fract_(float x) { return x - floorf(x); }
 
+// sin of an FP32 argument for the spawn hash, evaluated in FP64 and rounded
+// once: x = k pi/2 + r with pi/2 split into three parts (two of 24
+// significant bits, so k * part is exact for |k| < 2^29, i.e. |x| < 8e8), the
+// Taylor polynomials of sin r and cos r on |r| <= pi/4 to degree 15 and 16,
+// and the quadrant of k.  The hash multiplies sin by 1e4 before taking the
+// fraction, so any last-bit difference moves a spawn; GLSL leaves sin's
+// precision to the implementation, and this one runs the same operations on
+// the device and on a host (no contraction: -ffp-contract=off).
+__device__ __forceinline__ float hash_sinf(float xf) {
+    const double x = double(xf);
+    const double k = rint(x * 0.6366197723675814);
+    const double r = ((x - k * 1.570796251296997) - k * 7.549789415861596e-08) - k * 5.390302858158119e-15;
+    const double z = r * r;
+    const double sn = r * (1.0 + z * (-1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0 +
+                      z * (-1.0 / 39916800.0 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0))))))));
+    const double cs = 1.0 + z * (-0.5 + z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 +
+                      z * (-1.0 / 3628800.0 + z * (1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 +
+                      z * (1.0 / 20922789888000.0))))))));
+    const int q = int(static_cast<long long>(k) & 3);
+    const double v = q == 0 ? sn : q == 1 ? cs : q == 2 ? -sn : -cs;
+    return float(v);
+}
+
 // rand(vec2 p), glsl/particle.glsl:41-44
 __device__ __forceinline__ float rp_rand(float x, float y) {
-    return fract_(1e4f * sinf(17.0f * x + y * 0.1f) * (0.1f + fabsf(sinf(y * 13.0f + x))));
+    return fract_(1e4f * hash_sinf(17.0f * x + y * 0.1f) * (0.1f + fabsf(hash_sinf(y * 13.0f + x))));
 }
 
 // img_bilinear (glsl/img_interpolation.glsl:3-22) of one plane; texels

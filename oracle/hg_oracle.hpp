@@ -1009,10 +1009,30 @@ template <class T> struct RefParticleParams {
 
 template <class T> inline T glsl_fract_t(T x) { return x - T(std::floor(val(x))); }
 
+// sin of the spawn hash.  FP64: the library sin.  FP32: the method the
+// device kernel uses (FP64 evaluation of an FP32 argument: reduction by a
+// three-part pi/2, Taylor polynomials of sin and cos to degree 15 and 16,
+// the quadrant; one rounding to FP32), restated here; pinned against FP64 sin.
+inline double hash_sin(double x) { return std::sin(x); }
+inline float hash_sin(float xf) {
+    const double x = double(xf);
+    const double k = std::rint(x * 0.6366197723675814);
+    const double r = ((x - k * 1.570796251296997) - k * 7.549789415861596e-08) - k * 5.390302858158119e-15;
+    const double z = r * r;
+    const double sn = r * (1.0 + z * (-1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0 +
+                      z * (-1.0 / 39916800.0 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0))))))));
+    const double cs = 1.0 + z * (-0.5 + z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 +
+                      z * (-1.0 / 3628800.0 + z * (1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 +
+                      z * (1.0 / 20922789888000.0))))))));
+    const int q = int(static_cast<long long>(k) & 3);
+    const double v = q == 0 ? sn : q == 1 ? cs : q == 2 ? -sn : -cs;
+    return float(v);
+}
+
 // rand(vec2 p), glsl/particle.glsl:41-44
 template <class T> T ref_particle_rand(T x, T y) {
-    using std::sin; using std::fabs;
-    return glsl_fract_t(T(1e4) * sin(T(17.0) * x + y * T(0.1)) * (T(0.1) + fabs(sin(y * T(13.0) + x))));
+    using std::fabs;
+    return glsl_fract_t(T(1e4) * hash_sin(T(17.0) * x + y * T(0.1)) * (T(0.1) + fabs(hash_sin(y * T(13.0) + x))));
 }
 
 // img_bilinear (glsl/img_interpolation.glsl:3-22) of one plane.

@@ -463,5 +463,7 @@ int orc_ref_particle_run_f32(float* planes, int n, hg_ref_particle* parts, uint3
 float orc_ref_particle_rand_f32(float x, float y) { return orc::ref_particle_rand<float>(x, y); }
 // FP32 exp of the generator's exp mask (orc::gen_exp).
 float orc_gen_exp_f32(float x) { return orc::gen_exp(x); }
+// FP32 sin of the particle spawn hash (orc::hash_sin).
+float orc_hash_sin_f32(float x) { return orc::hash_sin(x); }
 double orc_ref_particle_rand_f64(double x, double y) { return orc::ref_particle_rand<double>(x, y); }
 }  // extern "C"

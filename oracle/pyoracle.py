@@ -244,6 +244,8 @@ def lib():
                                                                ctypes.POINTER(ctypes.c_uint8)]
         L.orc_gen_exp_f32.argtypes = [ctypes.c_float]
         L.orc_gen_exp_f32.restype = ctypes.c_float
+        L.orc_hash_sin_f32.argtypes = [ctypes.c_float]
+        L.orc_hash_sin_f32.restype = ctypes.c_float
         L.orc_ref_particle_rand_f32.argtypes = [ctypes.c_float, ctypes.c_float]
         L.orc_ref_particle_rand_f32.restype = ctypes.c_float
         L.orc_ref_particle_rand_f64.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -365,6 +367,11 @@ def run_pass(state: np.ndarray, settings: ErosionSettings, pass_id: int, rain_mo
 def gen_exp_f32(x: float) -> float:
     """FP32 exp of the reference generator's exp mask (orc::gen_exp)."""
     return lib().orc_gen_exp_f32(x)
+
+
+def hash_sin_f32(x: float) -> float:
+    """FP32 sin of the reference particle spawn hash (orc::hash_sin)."""
+    return lib().orc_hash_sin_f32(x)
 
 
 def smooth_flags(state: np.ndarray) -> np.ndarray:

@@ -6,10 +6,9 @@ settings, 10 dispatches of hg_step_particle_reference: movement, per-corner
 erosion, thermal erosion of both layers and smoothing with its particle
 branch (src/erosion.cpp:132-156).  Particles start at seeded positions and
 re-spawn through the reference's hash when they die.  Concurrent particles
-update shared pixels in an order the hardware picks, and re-spawns go
-through sinf, whose last bit may differ from the host's, so the run is
-compared by distribution: heights and water against the serial FP64
-oracle, next to the serial FP32 oracle's distance to it."""
+update shared pixels in an order the hardware picks, so the run is compared
+by distribution: heights and water against the serial FP64 oracle, next to
+the serial FP32 oracle's distance to it."""
 from __future__ import annotations
 
 import ctypes

@@ -131,10 +131,9 @@ def test_separated_particles_match_fp32_oracle_bitwise():
 
 def test_spawn_positions_follow_the_hash():
     """Unspawned particles with should_rain: positions from rand() of
-    glsl/particle.glsl:41-44.  The hash feeds sinf of arguments up to ~2e4
-    through a 1e4 gain, so the last bits in which the device's and the host's
-    sinf differ move a spawn by up to a few 1e-3 of the map (measured 0.37
-    cells of 124): about half the spawns agree exactly, all within 1e-2."""
+    glsl/particle.glsl:41-44, whose sin the device evaluates with the same
+    FP64 operations as the FP32 oracle (a 1e4 gain would show any last-bit
+    difference): every spawn agrees bit for bit."""
     n, count = 128, 2048
     s = _settings(count)
     t = _terrain(n)
@@ -152,8 +151,7 @@ def test_spawn_positions_follow_the_hash():
     err = np.abs(gx - ox).max()
     print(f"spawn: {same:.3f} identical, max |diff| {err:.3e} cells", flush=True)
     assert (gx >= 1.0).all() and (gx <= n - 2.0).all()
-    assert same >= 0.4
-    assert err <= 1e-2 * (n - 4)
+    assert same == 1.0 and err == 0.0
 
 
 @pytest.mark.parametrize("dims", [(1024, 1024), (72, 88)])

@@ -58,6 +58,25 @@ def test_rand_in_unit_interval():
             assert 0.0 <= r < 1.0
 
 
+def test_spawn_hash_fp32_sin_is_rounded_fp64_sin():
+    """The spawn hash's FP32 sin (the method the device kernel runs: FP64
+    evaluation with a three-part pi/2 reduction, one rounding) against FP64
+    sin rounded to FP32, over the arguments the hash produces (17 x + 0.1 y
+    up to ~2e4 and 13 y + x up to 13 * 2^26) and near the quadrant
+    boundaries: equal, which a missing reduction part or polynomial term
+    would break for large arguments."""
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-2e4, 2e4, 20000), rng.uniform(0.0, 13.0 * 2 ** 26, 20000),
+                         np.arange(-40, 41) * (math.pi / 2), np.linspace(-10.0, 10.0, 2001)]).astype(np.float32)
+    for x in xs:
+        assert np.float32(O.hash_sin_f32(float(x))) == np.float32(math.sin(float(x))), x
+    # and the hash in FP32 stays in [0, 1)
+    for x in np.linspace(0.0, 999.0, 37):
+        for y in (0.0, 1.0, 77.0, 1e5, 3.14, 1e6 + 3.14, 6.7e7):
+            r = O.ref_particle_rand(float(x), float(y), np.float32)
+            assert 0.0 <= r < 1.0
+
+
 def test_flat_terrain_particle_stops_and_dies():
     st = planes()
     st[O.ROCK] = 10.0
