@@ -187,6 +187,63 @@ __device__ __forceinline__ void tile_gather(int nx, int ny, Pre&& pre, Cell&& ce
 }
 
 // ---- hydro_flux ----------------------------------------------------------
+// VJP of the flux pass at an in-grid cell p = (px, py).  f: p's old outflow
+// (L, R, T, B); b: its water after rain; w, wn: its own and its L, R, T, B
+// neighbours' total height; fin: the neighbours' old outflow towards p; lf,
+// ld2, lvx/lvy/lvz: the adjoints of p's new outflow, water and velocity.
+// Adds to lfo (p's old outflow), lfin (the neighbours' old outflow towards
+// p), lwn (their total height), ld1 (p's water after rain) and lwp (p's own
+// total height), all zero on entry.  Nothing is sent off the grid.
+__device__ __forceinline__ void flux_vjp(float f[4], float b, float w, const float wn[4], const float fin[4],
+                                         int px, int py, int nx, int ny, const float lf[4], float ld2,
+                                         float lvx, float lvy, float lvz, const KParams& k, float lfo[4],
+                                         float lfin[4], float lwn[4], float& ld1, float& lwp) {
+    const unsigned wall = (px == 0 ? 1u : 0u) | (px == nx - 1 ? 2u : 0u) | (py == 0 ? 4u : 0u) |
+                          (py == ny - 1 ? 8u : 0u);
+    FluxTape tp;
+    float vx, vy, vz;
+    (void)flux_cell_t(f, b, w, wn, fin, wall, k, vx, vy, vz, tp);
+    if (vz > 0.0f) {                              // v = n * inv, inv = 1 / vz
+        const float lnx = lvx * tp.inv, lny = lvy * tp.inv;
+        lvz -= (lvx * vx + lvy * vy) * tp.inv;
+        lfin[0] += lnx; lfo[0] -= lnx; lfo[1] += lnx; lfin[1] -= lnx;
+        lfin[3] += lny; lfo[3] -= lny; lfo[2] += lny; lfin[2] -= lny;
+    }
+    ld1 += lvz;                                   // vz = d1 + d2
+    ld2 += lvz;
+    float lsum_out = 0.0f;
+    if (tp.arg > 0.0f) {                          // d2 = max(0, d1 + dt (sum_in - sum_out))
+        ld1 += ld2;
+        const float ls = k.dt * ld2;
+        for (int j = 0; j < 4; ++j) lfin[j] += ls;
+        lsum_out = -ls;
+    }
+    const float K = tp.q < 1.0f ? tp.q : 1.0f;
+    float lK = lsum_out * tp.S;                   // f = K f1, sum_out = K S
+    float lf1[4];
+    for (int j = 0; j < 4; ++j) {
+        lK += lf[j] * tp.f1[j];
+        lf1[j] = (lf[j] + lsum_out) * K;
+    }
+    if (tp.q < 1.0f) {                            // K = q = d1 / (S dt)
+        ld1 += lK * rcp(tp.S * k.dt);
+        const float lS = -lK * tp.q * rcp(tp.S);
+        for (int j = 0; j < 4; ++j) lf1[j] += lS;
+    }
+    const float c = k.dt * k.G;                   // f1 = max(0, E f + c (w - wn))
+    for (int j = 0; j < 4; ++j)
+        if ((tp.act >> j) & 1u) {
+            lfo[j] += k.E * lf1[j];
+            lwp += c * lf1[j];
+            lwn[j] -= c * lf1[j];
+        }
+    // contributions to neighbours outside the grid are dropped
+    if (px == 0) { lfin[0] = 0.0f; lwn[0] = 0.0f; }
+    if (px == nx - 1) { lfin[1] = 0.0f; lwn[1] = 0.0f; }
+    if (py == ny - 1) { lfin[2] = 0.0f; lwn[2] = 0.0f; }
+    if (py == 0) { lfin[3] = 0.0f; lwn[3] = 0.0f; }
+}
+
 // x: flux inputs (rock, dirt, water after rain, old flux); lo: adjoint of the
 // flux outputs (water, flux, velocity).  Writes li's water and flux (fresh)
 // and adds the total-height adjoint to li's rock and dirt.
@@ -231,54 +288,8 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
                 const float w = t_w[sc];
                 const float wn[4] = {t_w[sc - 1], t_w[sc + 1], t_w[sc + SRW], t_w[sc - SRW]};
                 const float fin[4] = {t_fr[sc - 1], t_fl[sc + 1], t_fb[sc + SRW], t_ft[sc - SRW]};
-                const unsigned wall = (px == 0 ? 1u : 0u) | (px == nx - 1 ? 2u : 0u) | (py == 0 ? 4u : 0u) |
-                                      (py == ny - 1 ? 8u : 0u);
-                FluxTape tp;
-                float vx, vy, vz;
-                (void)flux_cell_t(f, pb[it], w, wn, fin, wall, k, vx, vy, vz, tp);
-                const float* lf = plf[it];
-                float ld2 = pld2[it];
-                const float lvx = plv[it][0], lvy = plv[it][1];
-                float lvz = plv[it][2];
-                if (vz > 0.0f) {                              // v = n * inv, inv = 1 / vz
-                    const float lnx = lvx * tp.inv, lny = lvy * tp.inv;
-                    lvz -= (lvx * vx + lvy * vy) * tp.inv;
-                    lfin[0] += lnx; lfo[0] -= lnx; lfo[1] += lnx; lfin[1] -= lnx;
-                    lfin[3] += lny; lfo[3] -= lny; lfo[2] += lny; lfin[2] -= lny;
-                }
-                ld1 += lvz;                                   // vz = d1 + d2
-                ld2 += lvz;
-                float lsum_out = 0.0f;
-                if (tp.arg > 0.0f) {                          // d2 = max(0, d1 + dt (sum_in - sum_out))
-                    ld1 += ld2;
-                    const float ls = k.dt * ld2;
-                    for (int j = 0; j < 4; ++j) lfin[j] += ls;
-                    lsum_out = -ls;
-                }
-                const float K = tp.q < 1.0f ? tp.q : 1.0f;
-                float lK = lsum_out * tp.S;                   // f = K f1, sum_out = K S
-                float lf1[4];
-                for (int j = 0; j < 4; ++j) {
-                    lK += lf[j] * tp.f1[j];
-                    lf1[j] = (lf[j] + lsum_out) * K;
-                }
-                if (tp.q < 1.0f) {                            // K = q = d1 / (S dt)
-                    ld1 += lK * rcp(tp.S * k.dt);
-                    const float lS = -lK * tp.q * rcp(tp.S);
-                    for (int j = 0; j < 4; ++j) lf1[j] += lS;
-                }
-                const float c = k.dt * k.G;                   // f1 = max(0, E f + c (w - wn))
-                for (int j = 0; j < 4; ++j)
-                    if ((tp.act >> j) & 1u) {
-                        lfo[j] += k.E * lf1[j];
-                        lwp += c * lf1[j];
-                        lwn[j] -= c * lf1[j];
-                    }
-                // contributions to neighbours outside the grid are dropped
-                if (px == 0) { lfin[0] = 0.0f; lwn[0] = 0.0f; }
-                if (px == nx - 1) { lfin[1] = 0.0f; lwn[1] = 0.0f; }
-                if (py == ny - 1) { lfin[2] = 0.0f; lwn[2] = 0.0f; }
-                if (py == 0) { lfin[3] = 0.0f; lwn[3] = 0.0f; }
+                flux_vjp(f, pb[it], w, wn, fin, px, py, nx, ny, plf[it], pld2[it], plv[it][0], plv[it][1],
+                         plv[it][2], k, lfo, lfin, lwn, ld1, lwp);
             }
             for (int j = 0; j < 4; ++j) {
                 s_fin[j][e] = lfin[j];
@@ -307,6 +318,92 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
 }
 
 // ---- hydro_erosion -------------------------------------------------------
+// VJP of the erosion pass at an in-grid cell.  r_in .. vz: the cell's rock,
+// dirt, sediment and velocity before the pass; hl, hr, ht, hb: the total
+// heights (rock + dirt) of its x-1, x+1, y+1 and y-1 neighbours; lr .. ls1: the
+// adjoints of its rock, dirt and sediment after the pass.
+struct ErosionVjp {
+    float lr, ld, ls0, ls1;     // adjoints of the cell's rock, dirt, sediment before the pass
+    float lvx, lvy, lvz;        // and of its velocity
+    float ldx, ldz;             // slope adjoints: +ldx to x+1, -ldx to x-1, +ldz to y+1, -ldz to y-1
+    double cKc, cKs, cKd;       // coefficient terms
+};
+__device__ __forceinline__ ErosionVjp erosion_vjp(float r_in, float d_in, float s0_in, float s1_in, float vx,
+                                                  float vy, float vz, float hl, float hr, float ht, float hb,
+                                                  float lr, float ld, float ls0, float ls1, const KParams& k) {
+    ErosionVjp o;
+    float r = r_in, d = d_in, s0 = s0_in, s1 = s1_in;
+    ErosionTape tp;
+    erosion_cell_t(r, d, s0, s1, vx, vy, vz, hl, hr, ht, hb, k, tp);
+    double cKc = 0, cKs = 0, cKd = 0;
+    // conversion s1 += s0 kc, s0 -= s0 kc
+    const float kc = k.Kconv * k.dt;
+    const float ls0c = ls0 * (1.0f - kc) + ls1 * kc, ls1c = ls1;
+    // rock layer: (r, s0, c0) -> (r', s0c)
+    float lc0 = 0.0f, lcap0 = 0.0f, lcap = 0.0f;
+    float ls0_in = ls0c;
+    float lr_in = lr;
+    if (tp.case0 == 0 || tp.case0 == 2) {
+        const float a = k.dt * (tp.case0 == 0 ? k.Ks[0] : k.Kd[0]);
+        ls0_in = lr * a + ls0c * (1.0f - a);
+        lc0 = (ls0c - lr) * a;
+        if (tp.case0 == 0) cKs += double(k.dt) * double(tp.c0 - s0_in) * double(ls0c - lr);
+        else cKd += double(k.dt) * double(s0_in - tp.c0) * double(lr - ls0c);
+    } else if (tp.case0 == 1) {               // rock exhausted: r' = 0, s0c = s0 + r
+        lr_in = ls0c;
+        ls0_in = ls0c;
+    }
+    if (tp.case0 != 3 && tp.c0 > 0.0f) {      // c0 = max(0, cap0 - cap)
+        lcap0 += lc0;
+        lcap -= lc0;
+    }
+    // dirt layer: (d, s1, c1) -> (d', s1c); cap = d when exhausted
+    float lc1 = 0.0f;
+    float ls1_in = ls1c;
+    float ld_in = ld;
+    if (tp.case1 == 0 || tp.case1 == 2) {
+        const float a = k.dt * (tp.case1 == 0 ? k.Ks[1] : k.Kd[1]);
+        ls1_in = ld * a + ls1c * (1.0f - a);
+        lc1 = (ls1c - ld) * a;
+        if (tp.case1 == 0) cKs += double(k.dt) * double(tp.c1 - s1_in) * double(ls1c - ld);
+        else cKd += double(k.dt) * double(s1_in - tp.c1) * double(ld - ls1c);
+    } else {                                  // exhausted: d' = 0, s1c = s1 + d, cap = d
+        ld_in = ls1c + lcap;
+        ls1_in = ls1c;
+    }
+    if (tp.c1 > 0.0f) lcap0 += lc1;           // c1 = max(0, cap0)
+    // cap0 = Kc * max(0.02, sin_a) * ero
+    const float m = fmaxf(0.02f, tp.sin_a);
+    const float ero = tp.slow ? tp.ero0 * (1.0f - tp.sm) : tp.ero0;
+    cKc += double(lcap0) * double(m) * double(ero);
+    const float lm = lcap0 * k.Kc * ero, lero = lcap0 * k.Kc * m;
+    float ldx = 0.0f, ldz = 0.0f, lvx = 0.0f, lvy = 0.0f, lvz = 0.0f;
+    if (tp.sin_a > 0.02f && tp.sin_a > 0.0f) {  // sin_a = sqrt(g2 / (g2 + 4))
+        const float q = tp.g2 + 4.0f;
+        const float lg2 = lm * 2.0f * rcp(tp.sin_a * q * q);
+        ldx = 2.0f * tp.dx * lg2;
+        ldz = 2.0f * tp.dz * lg2;
+    }
+    float lero0 = lero;
+    if (tp.slow) {
+        lero0 = lero * (1.0f - tp.sm);
+        if (tp.t_inner) {                     // sm = t^2 (3 - 2t), t = (vz - 1e-3) / (5e-4 - 1e-3)
+            const float lsm = -lero * tp.ero0;
+            lvz = lsm * 6.0f * tp.tt * (1.0f - tp.tt) * (1.0f / (5e-4f - 1e-3f));
+        }
+    }
+    if (tp.ero0 > 0.0f) {
+        const float inv = rcp(tp.ero0);
+        lvx = lero0 * vx * inv;
+        lvy = lero0 * vy * inv;
+    }
+    o.lr = lr_in; o.ld = ld_in; o.ls0 = ls0_in; o.ls1 = ls1_in;
+    o.lvx = lvx; o.lvy = lvy; o.lvz = lvz;
+    o.ldx = ldx; o.ldz = ldz;
+    o.cKc = cKc; o.cKs = cKs; o.cKd = cKd;
+    return o;
+}
+
 // Writes li's rock, dirt and sediment (fresh) and adds to li's velocity.
 __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
                                                      double* gpart) {
@@ -338,92 +435,25 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
             pl0[it] = lo.p[HG_P_SED_ROCK][i]; pl1[it] = lo.p[HG_P_SED_DIRT][i];
         },
         [&](int it, int e, int px, int py, bool in, bool tile) {
-            float lr_in = 0.0f, ld_in = 0.0f, ldx = 0.0f, ldz = 0.0f;
-            float ls0_in = 0.0f, ls1_in = 0.0f, lvx = 0.0f, lvy = 0.0f, lvz = 0.0f;
+            ErosionVjp ev{};
             if (in) {
                 const int sc = sidx(e);
-                const float r_in = t_r[sc], d_in = t_d[sc], s0_in = p0[it], s1_in = p1[it];
-                const float vx = pvx[it], vy = pvy[it], vz = pvz[it];
-                float r = r_in, d = d_in, s0 = s0_in, s1 = s1_in;
-                ErosionTape tp;
-                erosion_cell_t(r, d, s0, s1, vx, vy, vz, rd(sc - 1), rd(sc + 1), rd(sc + SRW), rd(sc - SRW), k, tp);
-                const float lr = plr[it], ld = pld[it];
-                const float ls0 = pl0[it], ls1 = pl1[it];
-                double cKc = 0, cKs = 0, cKd = 0;
-                // conversion s1 += s0 kc, s0 -= s0 kc
-                const float kc = k.Kconv * k.dt;
-                const float ls0c = ls0 * (1.0f - kc) + ls1 * kc, ls1c = ls1;
-                // rock layer: (r, s0, c0) -> (r', s0c)
-                float lc0 = 0.0f, lcap0 = 0.0f, lcap = 0.0f;
-                ls0_in = ls0c;
-                lr_in = lr;
-                if (tp.case0 == 0 || tp.case0 == 2) {
-                    const float a = k.dt * (tp.case0 == 0 ? k.Ks[0] : k.Kd[0]);
-                    ls0_in = lr * a + ls0c * (1.0f - a);
-                    lc0 = (ls0c - lr) * a;
-                    if (tp.case0 == 0) cKs += double(k.dt) * double(tp.c0 - s0_in) * double(ls0c - lr);
-                    else cKd += double(k.dt) * double(s0_in - tp.c0) * double(lr - ls0c);
-                } else if (tp.case0 == 1) {               // rock exhausted: r' = 0, s0c = s0 + r
-                    lr_in = ls0c;
-                    ls0_in = ls0c;
-                }
-                if (tp.case0 != 3 && tp.c0 > 0.0f) {      // c0 = max(0, cap0 - cap)
-                    lcap0 += lc0;
-                    lcap -= lc0;
-                }
-                // dirt layer: (d, s1, c1) -> (d', s1c); cap = d when exhausted
-                float lc1 = 0.0f;
-                ls1_in = ls1c;
-                ld_in = ld;
-                if (tp.case1 == 0 || tp.case1 == 2) {
-                    const float a = k.dt * (tp.case1 == 0 ? k.Ks[1] : k.Kd[1]);
-                    ls1_in = ld * a + ls1c * (1.0f - a);
-                    lc1 = (ls1c - ld) * a;
-                    if (tp.case1 == 0) cKs += double(k.dt) * double(tp.c1 - s1_in) * double(ls1c - ld);
-                    else cKd += double(k.dt) * double(s1_in - tp.c1) * double(ld - ls1c);
-                } else {                                  // exhausted: d' = 0, s1c = s1 + d, cap = d
-                    ld_in = ls1c + lcap;
-                    ls1_in = ls1c;
-                }
-                if (tp.c1 > 0.0f) lcap0 += lc1;           // c1 = max(0, cap0)
-                // cap0 = Kc * max(0.02, sin_a) * ero
-                const float m = fmaxf(0.02f, tp.sin_a);
-                const float ero = tp.slow ? tp.ero0 * (1.0f - tp.sm) : tp.ero0;
-                cKc += double(lcap0) * double(m) * double(ero);
-                const float lm = lcap0 * k.Kc * ero, lero = lcap0 * k.Kc * m;
-                if (tp.sin_a > 0.02f && tp.sin_a > 0.0f) {  // sin_a = sqrt(g2 / (g2 + 4))
-                    const float q = tp.g2 + 4.0f;
-                    const float lg2 = lm * 2.0f * rcp(tp.sin_a * q * q);
-                    ldx = 2.0f * tp.dx * lg2;             // +ldx to p+1, -ldx to p-1
-                    ldz = 2.0f * tp.dz * lg2;             // +ldz to p+nx, -ldz to p-nx
-                }
-                float lero0 = lero;
-                if (tp.slow) {
-                    lero0 = lero * (1.0f - tp.sm);
-                    if (tp.t_inner) {                     // sm = t^2 (3 - 2t), t = (vz - 1e-3) / (5e-4 - 1e-3)
-                        const float lsm = -lero * tp.ero0;
-                        lvz = lsm * 6.0f * tp.tt * (1.0f - tp.tt) * (1.0f / (5e-4f - 1e-3f));
-                    }
-                }
-                if (tp.ero0 > 0.0f) {
-                    const float inv = rcp(tp.ero0);
-                    lvx = lero0 * vx * inv;
-                    lvy = lero0 * vy * inv;
-                }
+                ev = erosion_vjp(t_r[sc], t_d[sc], p0[it], p1[it], pvx[it], pvy[it], pvz[it], rd(sc - 1),
+                                 rd(sc + 1), rd(sc + SRW), rd(sc - SRW), plr[it], pld[it], pl0[it], pl1[it], k);
                 if (tile) {                               // coefficient terms counted once
-                    gKc += cKc; gKs += cKs; gKd += cKd;
+                    gKc += ev.cKc; gKs += ev.cKs; gKd += ev.cKd;
                 }
             }
-            s_dx[e] = ldx;
-            s_dz[e] = ldz;
+            s_dx[e] = ev.ldx;
+            s_dz[e] = ev.ldz;
             if (it < GTIT) {                                  // a tile cell keeps its own part
-                o_r[it] = lr_in;
-                o_d[it] = ld_in;
-                o_s0[it] = ls0_in;
-                o_s1[it] = ls1_in;
-                o_vx[it] = lvx;
-                o_vy[it] = lvy;
-                o_vz[it] = lvz;
+                o_r[it] = ev.lr;
+                o_d[it] = ev.ld;
+                o_s0[it] = ev.ls0;
+                o_s1[it] = ev.ls1;
+                o_vx[it] = ev.lvx;
+                o_vy[it] = ev.lvy;
+                o_vz[it] = ev.lvz;
             }
         },
         [&](int it, int e, int px, int py) {
@@ -436,6 +466,116 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
             li.p[HG_P_VEL_U][i] += o_vx[it];
             li.p[HG_P_VEL_V][i] += o_vy[it];
             li.p[HG_P_VEL_DEPTH][i] += o_vz[it];
+        });
+    reduce_coef<GNT>(gpart, gKc, gKs, gKd, 0.0);
+}
+
+// ---- hydro_erosion and hydro_flux in one launch ----------------------------
+// k_adj_erosion followed by k_adj_flux, with the same arithmetic in the same
+// order.  Both passes read the step's start rock and dirt, and the erosion's
+// velocity adjoint is the cell's own, so each ring and tile cell evaluates the
+// erosion VJP, completes its velocity adjoint in registers (the transport's
+// part plus the erosion's) and evaluates the flux VJP; the gather then sums
+// what both passes send.  x: rock, dirt, water after rain, old flux, sediment
+// and velocity of the step; lo: the adjoints after erosion.  Writes li's rock,
+// dirt, sediment, water and flux (fresh).  The velocity adjoint ends here: it
+// is not written, and the transport VJP of the step before writes it anew.
+__global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, Planes li, int nx, int ny,
+                                                          KParams k, double* gpart) {
+    // sent to neighbours: erosion's slope adjoints, flux's inflow (4) and
+    // total-height (4) adjoints
+    __shared__ float s_dx[GNR], s_dz[GNR], s_fin[4][GNR], s_wn[4][GNR];
+    // staged on the tile + 2-cell region: rock and dirt (0 off the grid), total
+    // height (HG_OOB_HEIGHT off the grid) and old flux (0)
+    __shared__ float t_r[SRN], t_d[SRN], t_w[SRN], t_fl[SRN], t_fr[SRN], t_ft[SRN], t_fb[SRN];
+    float o_r[GTIT], o_d[GTIT], o_s0[GTIT], o_s1[GTIT], o_fo[GTIT][4], o_wp[GTIT], o_d1[GTIT];
+    const float* R = x.p[HG_P_ROCK];
+    const float* D = x.p[HG_P_DIRT];
+    const float* B = x.p[HG_P_WATER];
+    {
+        float a[SIT], b[SIT], w[SIT], fl[SIT], fr[SIT], ft[SIT], fb[SIT];
+        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
+            const float r = R[i], d = D[i];
+            const float t = (r + d) + B[i];
+            const float f0 = x.p[HG_P_FLUX_L][i], f1 = x.p[HG_P_FLUX_R][i], f2 = x.p[HG_P_FLUX_T][i],
+                        f3 = x.p[HG_P_FLUX_B][i];
+            a[it] = in ? r : 0.0f;
+            b[it] = in ? d : 0.0f;
+            w[it] = in ? t : HG_OOB_HEIGHT;
+            fl[it] = in ? f0 : 0.0f; fr[it] = in ? f1 : 0.0f; ft[it] = in ? f2 : 0.0f; fb[it] = in ? f3 : 0.0f;
+        });
+        stage_walk(nx, ny, [&](int it, int s, size_t, bool) {
+            t_r[s] = a[it]; t_d[s] = b[it]; t_w[s] = w[it];
+            t_fl[s] = fl[it]; t_fr[s] = fr[it]; t_ft[s] = ft[it]; t_fb[s] = fb[it];
+        });
+    }
+    auto rd = [&](int sc) -> float { return t_r[sc] + t_d[sc]; };
+    float p0[GIT], p1[GIT], pvx[GIT], pvy[GIT], pvz[GIT], plr[GIT], pld[GIT], pl0[GIT], pl1[GIT];
+    float pb[GIT], plf[GIT][4], pld2[GIT], plv[GIT][3];
+    double gKc = 0, gKs = 0, gKd = 0;
+    tile_gather(nx, ny,
+        [&](int it, int, size_t i) {
+            p0[it] = x.p[HG_P_SED_ROCK][i]; p1[it] = x.p[HG_P_SED_DIRT][i];
+            pvx[it] = x.p[HG_P_VEL_U][i]; pvy[it] = x.p[HG_P_VEL_V][i]; pvz[it] = x.p[HG_P_VEL_DEPTH][i];
+            plr[it] = lo.p[HG_P_ROCK][i]; pld[it] = lo.p[HG_P_DIRT][i];
+            pl0[it] = lo.p[HG_P_SED_ROCK][i]; pl1[it] = lo.p[HG_P_SED_DIRT][i];
+            pb[it] = B[i];
+            plf[it][0] = lo.p[HG_P_FLUX_L][i]; plf[it][1] = lo.p[HG_P_FLUX_R][i];
+            plf[it][2] = lo.p[HG_P_FLUX_T][i]; plf[it][3] = lo.p[HG_P_FLUX_B][i];
+            pld2[it] = lo.p[HG_P_WATER][i];
+            plv[it][0] = lo.p[HG_P_VEL_U][i]; plv[it][1] = lo.p[HG_P_VEL_V][i]; plv[it][2] = lo.p[HG_P_VEL_DEPTH][i];
+        },
+        [&](int it, int e, int px, int py, bool in, bool tile) {
+            ErosionVjp ev{};
+            float lfo[4] = {0, 0, 0, 0}, lfin[4] = {0, 0, 0, 0}, lwn[4] = {0, 0, 0, 0};
+            float ld1 = 0.0f, lwp = 0.0f;
+            if (in) {
+                const int sc = sidx(e);
+                ev = erosion_vjp(t_r[sc], t_d[sc], p0[it], p1[it], pvx[it], pvy[it], pvz[it], rd(sc - 1),
+                                 rd(sc + 1), rd(sc + SRW), rd(sc - SRW), plr[it], pld[it], pl0[it], pl1[it], k);
+                if (tile) {                               // coefficient terms counted once
+                    gKc += ev.cKc; gKs += ev.cKs; gKd += ev.cKd;
+                }
+                // the velocity adjoint after the flux pass: the transport's part
+                // plus the erosion's (k_adj_erosion adds the latter in memory)
+                const float lvx = plv[it][0] + ev.lvx, lvy = plv[it][1] + ev.lvy, lvz = plv[it][2] + ev.lvz;
+                float f[4] = {t_fl[sc], t_fr[sc], t_ft[sc], t_fb[sc]};
+                const float wn[4] = {t_w[sc - 1], t_w[sc + 1], t_w[sc + SRW], t_w[sc - SRW]};
+                const float fin[4] = {t_fr[sc - 1], t_fl[sc + 1], t_fb[sc + SRW], t_ft[sc - SRW]};
+                flux_vjp(f, pb[it], t_w[sc], wn, fin, px, py, nx, ny, plf[it], pld2[it], lvx, lvy, lvz, k, lfo,
+                         lfin, lwn, ld1, lwp);
+            }
+            s_dx[e] = ev.ldx;
+            s_dz[e] = ev.ldz;
+            for (int j = 0; j < 4; ++j) {
+                s_fin[j][e] = lfin[j];
+                s_wn[j][e] = lwn[j];
+            }
+            if (it < GTIT) {                                  // a tile cell keeps its own part
+                o_r[it] = ev.lr;
+                o_d[it] = ev.ld;
+                o_s0[it] = ev.ls0;
+                o_s1[it] = ev.ls1;
+                for (int j = 0; j < 4; ++j) o_fo[it][j] = lfo[j];
+                o_wp[it] = lwp;
+                o_d1[it] = ld1;
+            }
+        },
+        [&](int it, int e, int px, int py) {
+            const size_t i = at(px, py, nx);
+            // rock and dirt after the erosion VJP, as k_adj_erosion writes them
+            const float nb = (s_dx[e - 1] - s_dx[e + 1]) + (s_dz[e - GRW] - s_dz[e + GRW]);
+            const float lr = o_r[it] + nb, ld = o_d[it] + nb;
+            li.p[HG_P_SED_ROCK][i] = o_s0[it];
+            li.p[HG_P_SED_DIRT][i] = o_s1[it];
+            li.p[HG_P_FLUX_L][i] = o_fo[it][0] + s_fin[1][e - 1];
+            li.p[HG_P_FLUX_R][i] = o_fo[it][1] + s_fin[0][e + 1];
+            li.p[HG_P_FLUX_T][i] = o_fo[it][2] + s_fin[3][e + GRW];
+            li.p[HG_P_FLUX_B][i] = o_fo[it][3] + s_fin[2][e - GRW];
+            const float lw = o_wp[it] + s_wn[0][e + 1] + s_wn[1][e - 1] + s_wn[2][e - GRW] + s_wn[3][e + GRW];
+            li.p[HG_P_WATER][i] = o_d1[it] + lw;               // w = rock + dirt + water
+            li.p[HG_P_ROCK][i] = lr + lw;
+            li.p[HG_P_DIRT][i] = ld + lw;
         });
     reduce_coef<GNT>(gpart, gKc, gKs, gKd, 0.0);
 }
@@ -774,9 +914,13 @@ __global__ __launch_bounds__(256) void k_zero_planes(PlaneList l, int count, siz
 
 namespace {
 
+// The erosion and flux VJPs as one reverse unit (k_adj_erosion_flux).
+constexpr int UNIT_EROSION_FLUX = 1000;
+
 // Planes each reverse unit produces: their adjoint before the unit is written
 // in full into a fresh buffer (the thermal units stand for outflow + transport
-// of a layer).
+// of a layer; the velocity planes of the flux units are left unwritten, since
+// the transport VJP of the step before writes them first).
 std::vector<int> produced(int pass) {
     switch (pass) {
         case HG_PASS_RAIN: return {HG_P_WATER};
@@ -787,6 +931,9 @@ std::vector<int> produced(int pass) {
         case HG_PASS_THERMAL_TRANSPORT_0: return {HG_P_ROCK};
         case HG_PASS_THERMAL_TRANSPORT_1: return {HG_P_DIRT};
         case HG_PASS_SMOOTH: return {HG_P_ROCK, HG_P_DIRT};
+        case UNIT_EROSION_FLUX: return {HG_P_ROCK, HG_P_DIRT, HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER,
+                                        HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
+                                        HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH};
         default: return {};
     }
 }
@@ -986,8 +1133,12 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     // reverse units: a thermal layer (outflow then transport) is one unit,
     // named by its transport pass
     std::vector<int> order;
-    order.push_back(HG_PASS_FLUX);
-    order.push_back(HG_PASS_EROSION);
+    if (c->adj_fuse) {
+        order.push_back(UNIT_EROSION_FLUX);
+    } else {
+        order.push_back(HG_PASS_FLUX);
+        order.push_back(HG_PASS_EROSION);
+    }
     order.push_back(HG_PASS_TRANSPORT);
     if (thermal) {
         order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
@@ -1067,6 +1218,12 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         X.p[HG_P_SED_ROCK] = in[HG_P_SED_ROCK]; X.p[HG_P_SED_DIRT] = in[HG_P_SED_DIRT];
                         X.p[HG_P_VEL_U] = tp[T_VX]; X.p[HG_P_VEL_V] = tp[T_VY]; X.p[HG_P_VEL_DEPTH] = tp[T_VZ];
                         break;
+                    case UNIT_EROSION_FLUX:
+                        X.p[HG_P_ROCK] = in[HG_P_ROCK]; X.p[HG_P_DIRT] = in[HG_P_DIRT]; X.p[HG_P_WATER] = tp[T_B1];
+                        for (int p = HG_P_FLUX_L; p <= HG_P_FLUX_B; ++p) X.p[p] = in[p];
+                        X.p[HG_P_SED_ROCK] = in[HG_P_SED_ROCK]; X.p[HG_P_SED_DIRT] = in[HG_P_SED_DIRT];
+                        X.p[HG_P_VEL_U] = tp[T_VX]; X.p[HG_P_VEL_V] = tp[T_VY]; X.p[HG_P_VEL_DEPTH] = tp[T_VZ];
+                        break;
                     case HG_PASS_TRANSPORT:
                         X.p[HG_P_SED_ROCK] = tp[T_S0]; X.p[HG_P_SED_DIRT] = tp[T_S1]; X.p[HG_P_WATER] = tp[T_B2];
                         X.p[HG_P_VEL_U] = tp[T_VX]; X.p[HG_P_VEL_V] = tp[T_VY]; X.p[HG_P_VEL_DEPTH] = tp[T_VZ];
@@ -1105,6 +1262,9 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         break;
                     case HG_PASS_EROSION:
                         k_adj_erosion<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gpart);
+                        break;
+                    case UNIT_EROSION_FLUX:
+                        k_adj_erosion_flux<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gpart);
                         break;
                     case HG_PASS_TRANSPORT:
                         k_adj_transport<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp, w.gpart);

@@ -62,6 +62,7 @@ struct hg_ctx {
     size_t adj_bytes = 0;
     hipStream_t adj_stream = nullptr;
     hipEvent_t adj_ev[5] = {};
+    int adj_fuse = 1;                // hg_set_adjoint_fusion: erosion + flux VJPs in one launch
     std::string arch;
     std::string err;
     // timing

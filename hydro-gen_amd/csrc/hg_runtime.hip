@@ -569,6 +569,19 @@ int hg_get_droplet_order(const hg_ctx* c, int* mode) {
     return HG_OK;
 }
 
+int hg_set_adjoint_fusion(hg_ctx* c, int on) {
+    if (!c) return HG_ERR_INVALID;
+    if (on < 0 || on > 1) return fail(c, HG_ERR_INVALID, "hg_set_adjoint_fusion: on must be 0 or 1");
+    c->adj_fuse = on;
+    return HG_OK;
+}
+
+int hg_get_adjoint_fusion(const hg_ctx* c, int* on) {
+    if (!c || !on) return HG_ERR_INVALID;
+    *on = c->adj_fuse;
+    return HG_OK;
+}
+
 int hg_step_particle(hg_ctx* c, uint32_t iterations, uint32_t flags) {
     if (!c) return HG_ERR_INVALID;
     if (flags & ~(HG_STEP_THERMAL | HG_STEP_SMOOTH)) return fail(c, HG_ERR_INVALID, "hg_step_particle: only thermal/smooth flags");

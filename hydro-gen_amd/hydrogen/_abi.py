@@ -38,7 +38,7 @@ EXPORTS = [
     "hg_set_step", "hg_read", "hg_write", "hg_save_height", "hg_load_height", "hg_read_plane", "hg_write_plane", "hg_plane_device_ptr",
     "hg_sync", "hg_get_stream", "hg_set_timing", "hg_kernel_stats", "hg_reset_timing",
     "hg_set_droplets", "hg_step_particle", "hg_particle_stats", "hg_droplets_noalloc", "hg_adjoint_rollout",
-    "hg_set_droplet_order", "hg_get_droplet_order",
+    "hg_set_droplet_order", "hg_get_droplet_order", "hg_set_adjoint_fusion", "hg_get_adjoint_fusion",
     "hg_ref_particles_init", "hg_ref_particles_read", "hg_ref_particles_write", "hg_step_particle_reference",
     "hg_ref_particle_stats", "hg_set_ref_particle_dims",
     "hg_tile_variant_count", "hg_tile_variant_info", "hg_set_tiles", "hg_get_tiles", "hg_set_xcd_order", "hg_autotune",
@@ -267,6 +267,8 @@ def load() -> ctypes.CDLL:
     L.hg_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hg_set_droplet_order.argtypes = [vp, ctypes.c_int]
     L.hg_get_droplet_order.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.hg_set_adjoint_fusion.argtypes = [vp, ctypes.c_int]
+    L.hg_get_adjoint_fusion.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.hg_set_sweep.argtypes = [vp, ctypes.c_int]
     L.hg_get_sweep.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.hg_get_tiles.argtypes = [vp, ip, ip]
@@ -396,6 +398,16 @@ class Terrain:
     def get_droplet_order(self) -> int:
         v = ctypes.c_int()
         self._check(self.lib.hg_get_droplet_order(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_adjoint_fusion(self, on: int):
+        """hg_set_adjoint_fusion: 1 the erosion and flux adjoints of a step in
+        one kernel (default), 0 one kernel each; same gradients."""
+        self._check(self.lib.hg_set_adjoint_fusion(self.h, on))
+
+    def get_adjoint_fusion(self) -> int:
+        v = ctypes.c_int()
+        self._check(self.lib.hg_get_adjoint_fusion(self.h, ctypes.byref(v)))
         return v.value
 
     def dispatch_particle(self, iterations: int = 1, flags: int = 0):

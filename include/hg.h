@@ -421,6 +421,12 @@ int hg_ref_particle_stats(hg_ctx* ctx, uint64_t* cas_giveups);
  * k = min(k_ckpt, nsteps) and s = 2, or 1 for a single segment) and a second
  * stream stay with the context for the next call, which reuses them when they
  * are large enough, until hg_destroy. */
+/* Reverse-kernel grouping of hg_adjoint_rollout (no counterpart in the
+ * reference).  1 (the default): the erosion and flux adjoints of a step run
+ * as one kernel; 0: one kernel each.  Both give the same gradients bit for
+ * bit. */
+int hg_set_adjoint_fusion(hg_ctx* ctx, int on);
+int hg_get_adjoint_fusion(const hg_ctx* ctx, int* on);
 int hg_adjoint_rollout(hg_ctx* ctx, uint32_t nsteps, uint32_t k_ckpt, uint32_t flags,
                        const float* dL_dH_final, float* dL_dH0, double* dL_dcoef);
 

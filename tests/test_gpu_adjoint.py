@@ -156,3 +156,65 @@ def test_adjoint_with_back_traces_beyond_one_cell():
         assert abs(gc[j] - dc[j]) <= 2e-3 * max(abs(dc[j]), 1e-9), (j, gc[j], dc[j])
     err = np.abs(gh.flat[cells] - dh)
     assert err.max() <= 2e-3 * np.abs(gh).max()
+
+
+def _grad_with_fusion(n, st, g, steps, k_ckpt, fuse, flags=hg.STEP_DEFAULT):
+    with hg.Terrain(n) as t:
+        t.set_erosion(hg.erosion_settings(Kc=COEF[0], Ks=COEF[1], Kd=COEF[2], Ke=COEF[3]))
+        t.set_rain(hg.RAIN_CONSTANT, amount=0.01)
+        assert t.get_adjoint_fusion() == 1          # the default
+        t.set_adjoint_fusion(fuse)
+        assert t.get_adjoint_fusion() == fuse
+        t.write_state(st)
+        t.step = 0
+        return t.adjoint_rollout(steps, k_ckpt, g, flags)
+
+
+@pytest.mark.parametrize("n,flags", [(40, hg.STEP_DEFAULT), (104, hg.STEP_DEFAULT), (136, hg.STEP_RAIN)])
+def test_fused_erosion_flux_adjoint_equals_separate_kernels(n, flags):
+    """The erosion + flux VJPs in one kernel give the gradients of the two
+    separate kernels bit for bit (sides that fill the 64 x 8 gather tiles
+    partly, several segments)."""
+    h0, tgt = setup_case(n, seed=9)
+    st = np.zeros((hg.NUM_STATE_PLANES, n, n), np.float32)
+    st[hg.P_ROCK] = h0
+    rng = np.random.default_rng(3)
+    st[hg.P_WATER] = rng.uniform(0.0, 0.5, (n, n))
+    st[hg.P_SED_ROCK] = rng.uniform(0.0, 0.05, (n, n))
+    g = rng.standard_normal((n, n)).astype(np.float32)
+    print("case", n, flags, flush=True)
+    gh1, gc1 = _grad_with_fusion(n, st, g, 7, 3, 1, flags)
+    gh0, gc0 = _grad_with_fusion(n, st, g, 7, 3, 0, flags)
+    assert np.abs(gh1).max() > 0
+    assert np.array_equal(gh1, gh0)
+    assert np.array_equal(gc1, gc0)
+
+
+def test_fused_erosion_flux_adjoint_with_long_back_traces():
+    """Back-traces of several cells (the transport's atomic scatter runs
+    before the fused unit): both groupings agree."""
+    n = 64
+    h0, tgt = setup_case(n, seed=4)
+    rng = np.random.default_rng(8)
+    st = np.zeros((hg.NUM_STATE_PLANES, n, n), np.float32)
+    st[hg.P_ROCK] = h0
+    st[hg.P_WATER] = rng.uniform(0.0, 0.02, (n, n))
+    st[hg.P_FLUX_R] = rng.uniform(20.0, 60.0, (n, n))
+    st[hg.P_FLUX_T] = rng.uniform(0.0, 5.0, (n, n))
+    st[hg.P_SED_ROCK] = rng.uniform(0, 1, (n, n))
+    st[hg.P_SED_DIRT] = rng.uniform(0, 1, (n, n))
+    g = rng.standard_normal((n, n)).astype(np.float32)
+    gh1, gc1 = _grad_with_fusion(n, st, g, 2, 1, 1)
+    gh0, gc0 = _grad_with_fusion(n, st, g, 2, 1, 0)
+    # far taps are summed with float atomics, whose order is not fixed
+    assert np.abs(gh1 - gh0).max() <= 1e-6 * np.abs(gh0).max()
+    assert np.allclose(gc1, gc0, rtol=1e-6, atol=1e-12)
+
+
+def test_adjoint_fusion_argument_rejected():
+    with hg.Terrain(32) as t:
+        with pytest.raises(hg.HGError):
+            t.set_adjoint_fusion(2)
+        with pytest.raises(hg.HGError):
+            t.set_adjoint_fusion(-1)
+        assert t.get_adjoint_fusion() == 1
