@@ -53,6 +53,17 @@ __device__ __forceinline__ void aadd(float* p, float v) {
     if (v != 0.0f) unsafeAtomicAdd(p, v);
 }
 
+// The tile a workgroup of a gather kernel works on: tile (x, y) of the
+// GTX x GTY tiling, and its slot of the coefficient partial sums (the tile's
+// row-major index).  block_tile(): the workgroup's own tile, one per
+// workgroup of a (tiles in x) x (tiles in y) grid.
+struct TileId {
+    int x, y, slot;
+};
+__device__ __forceinline__ TileId block_tile() {
+    return {int(blockIdx.x), int(blockIdx.y), int(blockIdx.y * gridDim.x + blockIdx.x)};
+}
+
 // Block reduction of up to 4 doubles, added by thread 0 to the block's own 4
 // slots of `gpart` (one slot set per workgroup of the launch, so no atomics:
 // same-address atomics from every workgroup serialise in one L2 channel).
@@ -60,7 +71,7 @@ __device__ __forceinline__ void aadd(float* p, float v) {
 // sums in wave order, so the result does not depend on scheduling.  The host
 // sums the slots in workgroup order at the end of the rollout.
 template <int NT>
-__device__ void reduce_coef(double* gpart, double a, double b, double c, double d) {
+__device__ void reduce_coef(double* gpart, int slot, double a, double b, double c, double d) {
     static_assert(NT % 64 == 0, "whole waves");
     constexpr int NW = NT / 64;
     __shared__ double sh[NW][4];
@@ -75,7 +86,7 @@ __device__ void reduce_coef(double* gpart, double a, double b, double c, double 
         for (int j = 0; j < 4; ++j) sh[t >> 6][j] = v[j];
     __syncthreads();
     if (t == 0) {
-        double* q = gpart + 4 * (size_t(blockIdx.y) * gridDim.x + blockIdx.x);
+        double* q = gpart + 4 * size_t(slot);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             double s = sh[0][j];
@@ -114,8 +125,8 @@ __device__ __forceinline__ int sidx(int e) {
 // the staged arrays, i the (clamped) global index, in whether it is on the grid.
 // Threads past the end redo the last cell (same values, same LDS slot).
 template <class F>
-__device__ __forceinline__ void stage_walk(int nx, int ny, F&& fn) {
-    const int x0 = blockIdx.x * GTX - 2, y0 = blockIdx.y * GTY - 2;
+__device__ __forceinline__ void stage_walk(const TileId& T, int nx, int ny, F&& fn) {
+    const int x0 = T.x * GTX - 2, y0 = T.y * GTY - 2;
 #pragma unroll
     for (int it = 0; it < SIT; ++it) {
         const int s = min(int(threadIdx.x) + it * GNT, SRN - 1);
@@ -157,8 +168,9 @@ __device__ __forceinline__ void ring_cell(int it, int& ex, int& ey) {
 // can stay in registers indexed by `it` (it < GTIT); only what it sends to
 // its neighbours goes through LDS.
 template <class Pre, class Cell, class Gather>
-__device__ __forceinline__ void tile_gather(int nx, int ny, Pre&& pre, Cell&& cell, Gather&& gather) {
-    const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
+__device__ __forceinline__ void tile_gather(const TileId& T, int nx, int ny, Pre&& pre, Cell&& cell,
+                                            Gather&& gather) {
+    const int x0 = T.x * GTX, y0 = T.y * GTY;
 #pragma unroll
     for (int it = 0; it < GIT; ++it) {
         int ex, ey;
@@ -248,6 +260,7 @@ __device__ __forceinline__ void flux_vjp(float f[4], float b, float w, const flo
 // flux outputs (water, flux, velocity).  Writes li's water and flux (fresh)
 // and adds the total-height adjoint to li's rock and dirt.
 __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
+    const TileId T = block_tile();
     // sent to neighbours, per ring/tile cell: inflow adj (4), total-height adj (4);
     // kept per thread: outflow adj (4), own total-height adj, water adj
     __shared__ float s_fin[4][GNR], s_wn[4][GNR];
@@ -259,19 +272,19 @@ __global__ __launch_bounds__(GNT) void k_adj_flux(Planes x, Planes lo, Planes li
     {
         // total height (out of grid: HG_OOB_HEIGHT) and old flux (0) of the region
         float w[SIT], fl[SIT], fr[SIT], ft[SIT], fb[SIT];
-        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
+        stage_walk(T, nx, ny, [&](int it, int, size_t i, bool in) {
             const float t = (R[i] + D[i]) + B[i];
             const float a = x.p[HG_P_FLUX_L][i], b = x.p[HG_P_FLUX_R][i], c = x.p[HG_P_FLUX_T][i],
                         d = x.p[HG_P_FLUX_B][i];
             w[it] = in ? t : HG_OOB_HEIGHT;
             fl[it] = in ? a : 0.0f; fr[it] = in ? b : 0.0f; ft[it] = in ? c : 0.0f; fb[it] = in ? d : 0.0f;
         });
-        stage_walk(nx, ny, [&](int it, int s, size_t, bool) {
+        stage_walk(T, nx, ny, [&](int it, int s, size_t, bool) {
             t_w[s] = w[it]; t_fl[s] = fl[it]; t_fr[s] = fr[it]; t_ft[s] = ft[it]; t_fb[s] = fb[it];
         });
     }
     float pb[GIT], plf[GIT][4], pld2[GIT], plv[GIT][3];
-    tile_gather(nx, ny,
+    tile_gather(T, nx, ny,
         [&](int it, int, size_t i) {
             pb[it] = B[i];
             plf[it][0] = lo.p[HG_P_FLUX_L][i]; plf[it][1] = lo.p[HG_P_FLUX_R][i];
@@ -407,6 +420,7 @@ __device__ __forceinline__ ErosionVjp erosion_vjp(float r_in, float d_in, float 
 // Writes li's rock, dirt and sediment (fresh) and adds to li's velocity.
 __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
                                                      double* gpart) {
+    const TileId T = block_tile();
     // sent to neighbours: the slope adjoints; kept per thread: the cell's own
     // rock, dirt, sediment and velocity adjoints
     __shared__ float s_dx[GNR], s_dz[GNR];
@@ -417,17 +431,17 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
     {
         // rock and dirt of the region (0 out of the grid)
         float a[SIT], b[SIT];
-        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
+        stage_walk(T, nx, ny, [&](int it, int, size_t i, bool in) {
             const float r = R[i], d = D[i];
             a[it] = in ? r : 0.0f;
             b[it] = in ? d : 0.0f;
         });
-        stage_walk(nx, ny, [&](int it, int s, size_t, bool) { t_r[s] = a[it]; t_d[s] = b[it]; });
+        stage_walk(T, nx, ny, [&](int it, int s, size_t, bool) { t_r[s] = a[it]; t_d[s] = b[it]; });
     }
     auto rd = [&](int sc) -> float { return t_r[sc] + t_d[sc]; };
     float p0[GIT], p1[GIT], pvx[GIT], pvy[GIT], pvz[GIT], plr[GIT], pld[GIT], pl0[GIT], pl1[GIT];
     double gKc = 0, gKs = 0, gKd = 0;
-    tile_gather(nx, ny,
+    tile_gather(T, nx, ny,
         [&](int it, int, size_t i) {
             p0[it] = x.p[HG_P_SED_ROCK][i]; p1[it] = x.p[HG_P_SED_DIRT][i];
             pvx[it] = x.p[HG_P_VEL_U][i]; pvy[it] = x.p[HG_P_VEL_V][i]; pvz[it] = x.p[HG_P_VEL_DEPTH][i];
@@ -467,7 +481,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
             li.p[HG_P_VEL_V][i] += o_vy[it];
             li.p[HG_P_VEL_DEPTH][i] += o_vz[it];
         });
-    reduce_coef<GNT>(gpart, gKc, gKs, gKd, 0.0);
+    reduce_coef<GNT>(gpart, T.slot, gKc, gKs, gKd, 0.0);
 }
 
 // ---- hydro_erosion and hydro_flux in one launch ----------------------------
@@ -482,6 +496,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
 // is not written, and the transport VJP of the step before writes it anew.
 __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, Planes li, int nx, int ny,
                                                           KParams k, double* gpart) {
+    const TileId T = block_tile();
     // sent to neighbours: erosion's slope adjoints, flux's inflow (4) and
     // total-height (4) adjoints
     __shared__ float s_dx[GNR], s_dz[GNR], s_fin[4][GNR], s_wn[4][GNR];
@@ -494,7 +509,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, P
     const float* B = x.p[HG_P_WATER];
     {
         float a[SIT], b[SIT], w[SIT], fl[SIT], fr[SIT], ft[SIT], fb[SIT];
-        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
+        stage_walk(T, nx, ny, [&](int it, int, size_t i, bool in) {
             const float r = R[i], d = D[i];
             const float t = (r + d) + B[i];
             const float f0 = x.p[HG_P_FLUX_L][i], f1 = x.p[HG_P_FLUX_R][i], f2 = x.p[HG_P_FLUX_T][i],
@@ -504,7 +519,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, P
             w[it] = in ? t : HG_OOB_HEIGHT;
             fl[it] = in ? f0 : 0.0f; fr[it] = in ? f1 : 0.0f; ft[it] = in ? f2 : 0.0f; fb[it] = in ? f3 : 0.0f;
         });
-        stage_walk(nx, ny, [&](int it, int s, size_t, bool) {
+        stage_walk(T, nx, ny, [&](int it, int s, size_t, bool) {
             t_r[s] = a[it]; t_d[s] = b[it]; t_w[s] = w[it];
             t_fl[s] = fl[it]; t_fr[s] = fr[it]; t_ft[s] = ft[it]; t_fb[s] = fb[it];
         });
@@ -513,7 +528,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, P
     float p0[GIT], p1[GIT], pvx[GIT], pvy[GIT], pvz[GIT], plr[GIT], pld[GIT], pl0[GIT], pl1[GIT];
     float pb[GIT], plf[GIT][4], pld2[GIT], plv[GIT][3];
     double gKc = 0, gKs = 0, gKd = 0;
-    tile_gather(nx, ny,
+    tile_gather(T, nx, ny,
         [&](int it, int, size_t i) {
             p0[it] = x.p[HG_P_SED_ROCK][i]; p1[it] = x.p[HG_P_SED_DIRT][i];
             pvx[it] = x.p[HG_P_VEL_U][i]; pvy[it] = x.p[HG_P_VEL_V][i]; pvz[it] = x.p[HG_P_VEL_DEPTH][i];
@@ -577,7 +592,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, P
             li.p[HG_P_ROCK][i] = lr + lw;
             li.p[HG_P_DIRT][i] = ld + lw;
         });
-    reduce_coef<GNT>(gpart, gKc, gKs, gKd, 0.0);
+    reduce_coef<GNT>(gpart, T.slot, gKc, gKs, gKd, 0.0);
 }
 
 // ---- sediment_transport --------------------------------------------------
@@ -602,19 +617,26 @@ __device__ __forceinline__ Taps taps_of(int qx, int qy, float vx, float vy, int 
     return t;
 }
 
-__global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
-                                                       double* gpart) {
+// LDS of one transport tile, in floats.
+constexpr int TRANSPORT_LDS = 6 * GNR + (GNR + 3) / 4;
+
+// The transport VJP's near gather on tile T, with `lds` (TRANSPORT_LDS
+// floats) as its LDS.
+__device__ __forceinline__ void adj_transport_tile(const TileId& T, float* lds, Planes x, Planes lo, Planes li,
+                                                   int nx, int ny, const KParams& k, double* gpart) {
     // per tile/ring cell q: its 4 bilinear weights, its taps as offsets from
     // q (bits 0-1: ix - qx + 1, 2-3: ix1 - qx + 1, 4-5: iy - qy + 1, 6-7:
     // iy1 - qy + 1; 0 when q is off the grid or its taps are not near), and the
     // sediment adjoints it spreads over them
-    __shared__ float s_w[4][GNR], s_l0[GNR], s_l1[GNR];
-    __shared__ unsigned char s_tap[GNR];
+    float (*s_w)[GNR] = reinterpret_cast<float (*)[GNR]>(lds);
+    float* s_l0 = lds + 4 * GNR;
+    float* s_l1 = lds + 5 * GNR;
+    unsigned char* s_tap = reinterpret_cast<unsigned char*>(lds + 6 * GNR);
     const float* VX = x.p[HG_P_VEL_U];
     const float* VY = x.p[HG_P_VEL_V];
     double gKe = 0;
     float pvx[GIT], pvy[GIT], pl0[GIT], pl1[GIT];
-    tile_gather(nx, ny,
+    tile_gather(T, nx, ny,
         [&](int it, int, size_t i) {
             pvx[it] = VX[i]; pvy[it] = VY[i];
             pl0[it] = lo.p[HG_P_SED_ROCK][i]; pl1[it] = lo.p[HG_P_SED_DIRT][i];
@@ -696,7 +718,13 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
             li.p[HG_P_WATER][i] = lb * k.evap;                   // b' = b (1 - Ke dt)
             gKe += double(lb) * double(x.p[HG_P_WATER][i]) * double(-k.dt);
         });
-    reduce_coef<GNT>(gpart, 0.0, 0.0, 0.0, gKe);
+    reduce_coef<GNT>(gpart, T.slot, 0.0, 0.0, 0.0, gKe);
+}
+
+__global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Planes li, int nx, int ny, KParams k,
+                                                       double* gpart) {
+    __shared__ float lds[TRANSPORT_LDS];
+    adj_transport_tile(block_tile(), lds, x, lo, li, nx, ny, k, gpart);
 }
 
 __global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, Planes li, int nx, int ny,
@@ -725,12 +753,63 @@ __global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, 
 // Adjoint: dL/dT_k(p) = lh(p+k) - lh(p) (lh = dL/dh_out of the layer), then the
 // outflow's vector-Jacobian product sends -dL/dD_k to neighbour p+k and the
 // sum to p (both rock and dirt for layer 1, whose D_k spans both layers).
-// x: the layer's input rock and dirt.  Writes li's layer plane (fresh) and,
-// for layer 1, adds the rock part to li's rock in place.
+//
+// thermal_vjp: one in-grid cell p.  Dk: p's height differences to its 8
+// neighbours (thermal_g's order), h_layer: p's height of the layer, lh and
+// lhn: the layer's adjoint at p and at its neighbours, ok: whether each
+// neighbour is on the grid.  Gives lD (what p sends to each neighbour, 0 off
+// the grid), lself (p's own part) and lh_layer (through H = min(h_layer, .)).
 template <int layer>
-__global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
-    __shared__ float s_lD[8][GNR];                // sent to the 8 neighbours
-    __shared__ float t_r[SRN], t_d[SRN], t_lh[SRN];
+__device__ __forceinline__ void thermal_vjp(const float Dk[8], float h_layer, float lh, const float lhn[8],
+                                            const bool ok[8], const KParams& k, float lD[8], float& lself,
+                                            float& lh_layer) {
+    ThermalTape tp;
+    const float g = thermal_g_t(Dk, h_layer, layer, k, tp);
+    if (tp.bk > 0.0f) {
+        float lg = 0.0f;
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+            const bool m = talus_marked(Dk[kk], kk, layer, k);
+            const float lout = lhn[kk] - lh;          // dL/dT_k
+            lD[kk] = m ? lout * g : 0.0f;
+            if (m) lg += lout * Dk[kk];
+        }
+        const float rb = rcp(tp.bk);              // g = S / bk
+        const float lS = lg * rb, lbk = -lg * g * rb;
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk)
+            if (talus_marked(Dk[kk], kk, layer, k)) lD[kk] += lbk;
+        const float A = k.dt * k.Kspeed[layer] * 0.5f;  // S = A sharp^3 H
+        const float sh3 = tp.sharp * (tp.sharp * tp.sharp);
+        const float lsh3 = lS * A * tp.H, lH = lS * A * sh3;
+        if (tp.sharp_active) {                    // sharp = 1 + atan(rmax) - Kalpha
+            const float lrmax = lsh3 * 3.0f * tp.sharp * tp.sharp * rcp(1.0f + tp.rmax * tp.rmax);
+            if (tp.mc >= tp.md * 0.70710678118654752f) lD[tp.kc] += lrmax;
+            else lD[tp.kd] += lrmax * 0.70710678118654752f;
+        }
+        if (tp.h_is_layer) lh_layer = lH;         // H = min(h_l, max(0, mc, md))
+        else if (fmaxf(tp.mc, tp.md) > 0.0f) lD[tp.mc >= tp.md ? tp.kc : tp.kd] += lH;
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+            lself += lD[kk];
+            if (!ok[kk]) lD[kk] = 0.0f;           // nothing is sent off the grid
+        }
+    }
+}
+
+// LDS of one thermal tile, in floats.
+constexpr int THERMAL_LDS = 8 * GNR + 3 * SRN;
+
+// A thermal layer's VJP on tile T, with `lds` (THERMAL_LDS floats) as its
+// LDS.  x: the layer's input rock and dirt.  Writes li's layer plane (fresh)
+// and, for layer 1, adds the rock part to li's rock in place.
+template <int layer>
+__device__ __forceinline__ void adj_thermal_tile(const TileId& T, float* lds, Planes x, Planes lo, Planes li,
+                                                 int nx, int ny, const KParams& k) {
+    float (*s_lD)[GNR] = reinterpret_cast<float (*)[GNR]>(lds);   // sent to the 8 neighbours
+    float* t_r = lds + 8 * GNR;
+    float* t_d = t_r + SRN;
+    float* t_lh = t_d + SRN;
     float o_self[GTIT], o_hl[GTIT];                 // kept per thread
     constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
@@ -739,23 +818,31 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
     const float* LH = lo.p[layer == 0 ? HG_P_ROCK : HG_P_DIRT];
     {
         // rock, dirt (out of grid: HG_OOB_HEIGHT) and the layer's adjoint (0)
+        // (layer 0 reads no dirt)
         float a[SIT], b[SIT], c[SIT];
-        stage_walk(nx, ny, [&](int it, int, size_t i, bool in) {
-            const float r = R[i], d = Dt[i], l = LH[i];
+        stage_walk(T, nx, ny, [&](int it, int, size_t i, bool in) {
+            const float r = R[i], l = LH[i];
             a[it] = in ? r : HG_OOB_HEIGHT;
-            b[it] = in ? d : HG_OOB_HEIGHT;
+            if constexpr (layer == 1) {
+                const float d = Dt[i];
+                b[it] = in ? d : HG_OOB_HEIGHT;
+            }
             c[it] = in ? l : 0.0f;
         });
-        stage_walk(nx, ny, [&](int it, int s, size_t, bool) { t_r[s] = a[it]; t_d[s] = b[it]; t_lh[s] = c[it]; });
+        stage_walk(T, nx, ny, [&](int it, int s, size_t, bool) {
+            t_r[s] = a[it];
+            if constexpr (layer == 1) t_d[s] = b[it];
+            t_lh[s] = c[it];
+        });
     }
-    tile_gather(nx, ny,
+    tile_gather(T, nx, ny,
         [&](int, int, size_t) {},
         [&](int it, int e, int px, int py, bool in, bool) {
             float lD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             float lself = 0.0f, lh_layer = 0.0f;
             if (in) {
                 const int sc = sidx(e);
-                const float r = t_r[sc], d = t_d[sc];
+                const float r = t_r[sc], d = layer == 1 ? t_d[sc] : 0.0f;
                 float Dk[8], lhn[8];
                 bool ok[8];
 #pragma unroll
@@ -768,39 +855,7 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
                     Dk[kk] = v;
                     lhn[kk] = t_lh[o];
                 }
-                ThermalTape tp;
-                const float g = thermal_g_t(Dk, layer == 0 ? r : d, layer, k, tp);
-                if (tp.bk > 0.0f) {
-                    const float lh = t_lh[sc];
-                    float lg = 0.0f;
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) {
-                        const bool m = talus_marked(Dk[kk], kk, layer, k);
-                        const float lout = lhn[kk] - lh;          // dL/dT_k
-                        lD[kk] = m ? lout * g : 0.0f;
-                        if (m) lg += lout * Dk[kk];
-                    }
-                    const float rb = rcp(tp.bk);              // g = S / bk
-                    const float lS = lg * rb, lbk = -lg * g * rb;
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk)
-                        if (talus_marked(Dk[kk], kk, layer, k)) lD[kk] += lbk;
-                    const float A = k.dt * k.Kspeed[layer] * 0.5f;  // S = A sharp^3 H
-                    const float sh3 = tp.sharp * (tp.sharp * tp.sharp);
-                    const float lsh3 = lS * A * tp.H, lH = lS * A * sh3;
-                    if (tp.sharp_active) {                    // sharp = 1 + atan(rmax) - Kalpha
-                        const float lrmax = lsh3 * 3.0f * tp.sharp * tp.sharp * rcp(1.0f + tp.rmax * tp.rmax);
-                        if (tp.mc >= tp.md * 0.70710678118654752f) lD[tp.kc] += lrmax;
-                        else lD[tp.kd] += lrmax * 0.70710678118654752f;
-                    }
-                    if (tp.h_is_layer) lh_layer = lH;         // H = min(h_l, max(0, mc, md))
-                    else if (fmaxf(tp.mc, tp.md) > 0.0f) lD[tp.mc >= tp.md ? tp.kc : tp.kd] += lH;
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) {
-                        lself += lD[kk];
-                        if (!ok[kk]) lD[kk] = 0.0f;           // nothing is sent off the grid
-                    }
-                }
+                thermal_vjp<layer>(Dk, layer == 0 ? r : d, t_lh[sc], lhn, ok, k, lD, lself, lh_layer);
             }
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = lD[kk];
@@ -825,12 +880,19 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes
         });
 }
 
+template <int layer>
+__global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
+    __shared__ float lds[THERMAL_LDS];
+    adj_thermal_tile<layer>(block_tile(), lds, x, lo, li, nx, ny, k);
+}
+
 // ---- smoothing -----------------------------------------------------------
 // y = x + m (mean5 - x) at an extremum.  Gather: p keeps its own share and
 // receives 0.2 m dL/dy(q) from each interior neighbour q that is an extremum.
 // The extremum flags of the tile + 1-cell ring are evaluated once into LDS.
 // Writes li's rock and dirt (fresh).
 __global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
+    const TileId T = block_tile();
     // rock, dirt and their adjoints on the tile + 2-cell region; cells off the
     // grid hold the clamped cell's values, which only feed extremum flags of
     // non-interior cells, and those are not used
@@ -839,17 +901,17 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes 
     __shared__ unsigned char s_f[GNR];
     {
         float a[SIT], b[SIT], c[SIT], d[SIT];
-        stage_walk(nx, ny, [&](int it, int, size_t i, bool) {
+        stage_walk(T, nx, ny, [&](int it, int, size_t i, bool) {
             a[it] = x.p[HG_P_ROCK][i]; b[it] = x.p[HG_P_DIRT][i];
             c[it] = lo.p[HG_P_ROCK][i]; d[it] = lo.p[HG_P_DIRT][i];
         });
-        stage_walk(nx, ny, [&](int it, int s, size_t, bool) {
+        stage_walk(T, nx, ny, [&](int it, int s, size_t, bool) {
             t_r[s] = a[it]; t_d[s] = b[it]; t_lr[s] = c[it]; t_ld[s] = d[it];
         });
     }
     const float m = k.smooth_m;
     float o_lr[GTIT], o_ld[GTIT];
-    tile_gather(nx, ny,
+    tile_gather(T, nx, ny,
         [&](int, int, size_t) {},
         [&](int it, int e, int px, int py, bool, bool) {
             const int sc = sidx(e);
@@ -884,6 +946,175 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes 
         });
 }
 
+// ---- smoothing and thermal layer 1 in one launch --------------------------
+// k_adj_smooth followed by k_adj_thermal<1>, with the same arithmetic in the
+// same order.  The layer-1 VJP of a ring cell reads the dirt adjoint of its
+// neighbours, so the smoothing VJP is evaluated on the tile + 2-cell region
+// (into LDS), from extremum flags on the tile + 3 region and rock and dirt
+// staged on the tile + 4 region; the layer-1 VJP then runs as in
+// k_adj_thermal<1>.  x: rock after layer 0 (x.p[HG_P_ROCK]) and dirt after
+// layer 1 (x.p[HG_P_DIRT]), the smoothing's inputs; d3: dirt after erosion,
+// layer 1's input dirt.  lo: the adjoints after smoothing.  Writes li's rock
+// and dirt (fresh).
+
+// fn(it, s, i, in, gx, gy) for the cells of the tile + H region: s its index
+// in a (GTX + 2H) x (GTY + 2H) array, i the (clamped) global index.  Threads
+// past the end redo the last cell.
+template <int H, class F>
+__device__ __forceinline__ void region_walk(const TileId& T, int nx, int ny, F&& fn) {
+    constexpr int W = GTX + 2 * H, N = W * (GTY + 2 * H), IT = (N + GNT - 1) / GNT;
+    const int x0 = T.x * GTX - H, y0 = T.y * GTY - H;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int s = min(int(threadIdx.x) + it * GNT, N - 1);
+        const int sy = s / W, sx = s - sy * W;
+        const int gx = x0 + sx, gy = y0 + sy;
+        const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+        fn(it, s, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx), in, gx, gy);
+    }
+}
+
+__global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo, Planes li, const float* d3,
+                                                             int nx, int ny, KParams k) {
+    const TileId T = block_tile();
+    constexpr int W4 = GTX + 8, N4 = W4 * (GTY + 8), IT4 = (N4 + GNT - 1) / GNT;
+    constexpr int W3 = GTX + 6, N3 = W3 * (GTY + 6), IT3 = (N3 + GNT - 1) / GNT;
+    // LDS shared by the two halves: the smoothing's inputs and flags, then
+    // what layer 1 sends to the 8 neighbours (30.9 KB in all: 5 workgroups
+    // per CU, as k_adj_thermal<1>)
+    constexpr int NU0 = 2 * N4 + 2 * N3 + (N3 + 3) / 4, NU1 = 8 * GNR;
+    constexpr int NU = NU0 > NU1 ? NU0 : NU1;
+    __shared__ float u[NU];
+    // rock after layer 0 and dirt after layer 1 on tile + 4 (HG_OOB_HEIGHT
+    // off the grid; the flags of cells off or on the edge of the grid are
+    // not evaluated)
+    float* t_r5 = u;
+    float* t_d6 = u + N4;
+    float* t_lr = u + 2 * N4;                    // adjoints after smoothing on tile + 3 (0 off the grid)
+    float* t_ld = t_lr + N3;
+    unsigned char* s_f = reinterpret_cast<unsigned char*>(t_ld + N3);   // extremum flags, tile + 3
+    float (*s_lD)[GNR] = reinterpret_cast<float (*)[GNR]>(u);          // layer 1's sends
+    // tile + 2: rock after layer 0 and dirt after erosion (HG_OOB_HEIGHT off
+    // the grid), the dirt adjoint before smoothing (0 off the grid)
+    __shared__ float t_r[SRN], t_d[SRN], t_md[SRN];
+    float o_mr[GTIT], o_self[GTIT], o_hl[GTIT];
+    constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+    constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+    const float m = k.smooth_m;
+    {
+        float a[IT4], b[IT4], c[IT3], d[IT3], e[SIT];
+        region_walk<4>(T, nx, ny, [&](int it, int, size_t i, bool in, int, int) {
+            const float r = x.p[HG_P_ROCK][i], dd = x.p[HG_P_DIRT][i];
+            a[it] = in ? r : HG_OOB_HEIGHT;
+            b[it] = in ? dd : HG_OOB_HEIGHT;
+        });
+        region_walk<3>(T, nx, ny, [&](int it, int, size_t i, bool in, int, int) {
+            const float lr = lo.p[HG_P_ROCK][i], ld = lo.p[HG_P_DIRT][i];
+            c[it] = in ? lr : 0.0f;
+            d[it] = in ? ld : 0.0f;
+        });
+        region_walk<2>(T, nx, ny, [&](int it, int, size_t i, bool in, int, int) {
+            const float v = d3[i];
+            e[it] = in ? v : HG_OOB_HEIGHT;
+        });
+        region_walk<4>(T, nx, ny, [&](int it, int s, size_t, bool, int, int) { t_r5[s] = a[it]; t_d6[s] = b[it]; });
+        region_walk<3>(T, nx, ny, [&](int it, int s, size_t, bool, int, int) { t_lr[s] = c[it]; t_ld[s] = d[it]; });
+        region_walk<2>(T, nx, ny, [&](int it, int s, size_t, bool, int, int) { t_d[s] = e[it]; });
+    }
+    __syncthreads();
+    // extremum flags on tile + 3, as k_adj_smooth evaluates them
+    region_walk<3>(T, nx, ny, [&](int, int s, size_t, bool, int gx, int gy) {
+        const int sy = s / W3, sx = s - sy * W3;
+        const int c = (sy + 1) * W4 + sx + 1;
+        unsigned f = 0;
+        if (gx > 0 && gy > 0 && gx < nx - 1 && gy < ny - 1) {
+            bool er, eg;
+            smooth_flags(t_r5[c], t_d6[c], t_r5[c - 1], t_d6[c - 1], t_r5[c + 1], t_d6[c + 1], t_r5[c + W4],
+                         t_d6[c + W4], t_r5[c - W4], t_d6[c - W4], er, eg);
+            f = (er ? 1u : 0u) | (eg ? 2u : 0u);
+        }
+        s_f[s] = (unsigned char)f;
+    });
+    __syncthreads();
+    // the smoothing VJP: its own share, then what each extremum neighbour
+    // passes back, in k_adj_smooth's order; c is a cell's index on tile + 3
+    constexpr int NXo[4] = {-1, 1, 0, 0}, NYo[4] = {0, 0, 1, -1};
+    auto smooth_r = [&](int c) {
+        const unsigned f = s_f[c];
+        const float lr = t_lr[c];
+        float ar = (f & 1u) ? lr * (1.0f - 0.8f * m) : lr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = c + NXo[j] + NYo[j] * W3;
+            if (s_f[q] & 1u) ar += t_lr[q] * 0.2f * m;
+        }
+        return ar;
+    };
+    auto smooth_d = [&](int c) {
+        const unsigned f = s_f[c];
+        const float ld = t_ld[c];
+        float ad = (f & 2u) ? ld * (1.0f - 0.8f * m) : ld;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = c + NXo[j] + NYo[j] * W3;
+            if (s_f[q] & 2u) ad += t_ld[q] * 0.2f * m;
+        }
+        return ad;
+    };
+    // the dirt adjoint on tile + 2 (layer 1's input adjoint) and the rock
+    // on tile + 2 (layer 1's heights) into their own arrays
+    region_walk<2>(T, nx, ny, [&](int, int s, size_t, bool in, int, int) {
+        const int sy = s / SRW, sx = s - sy * SRW;
+        t_md[s] = in ? smooth_d((sy + 1) * W3 + sx + 1) : 0.0f;
+        t_r[s] = t_r5[(sy + 2) * W4 + sx + 2];
+    });
+    // the rock adjoint of each tile cell, kept by the thread that writes it
+#pragma unroll
+    for (int it = 0; it < GTIT; ++it) {
+        int ex, ey;
+        ring_cell(it, ex, ey);
+        o_mr[it] = smooth_r((ey + 2) * W3 + ex + 2);
+    }
+    // layer 1, as k_adj_thermal<1> with t_lh = t_md (tile_gather's first
+    // barrier ends the reads of the shared LDS above)
+    tile_gather(T, nx, ny,
+        [&](int, int, size_t) {},
+        [&](int it, int e, int px, int py, bool in, bool) {
+            float lD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            float lself = 0.0f, lh_layer = 0.0f;
+            if (in) {
+                const int sc = sidx(e);
+                const float r = t_r[sc], d = t_d[sc];
+                float Dk[8], lhn[8];
+                bool ok[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const int xx = px + DX[kk], yy = py + DY[kk];
+                    const int o = sc + DX[kk] + DY[kk] * SRW;
+                    ok[kk] = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
+                    Dk[kk] = (r - t_r[o]) + (d - t_d[o]);
+                    lhn[kk] = t_md[o];
+                }
+                thermal_vjp<1>(Dk, d, t_md[sc], lhn, ok, k, lD, lself, lh_layer);
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = lD[kk];
+            if (it < GTIT) {
+                o_self[it] = lself;
+                o_hl[it] = lh_layer;
+            }
+        },
+        [&](int it, int e, int px, int py) {
+            const size_t i = at(px, py, nx);
+            float recv = 0.0f;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) recv -= s_lD[kk][e - (DX[kk] + DY[kk] * GRW)];
+            const float shared = o_self[it] + recv;
+            li.p[HG_P_DIRT][i] = t_md[sidx(e)] + shared + o_hl[it];
+            li.p[HG_P_ROCK][i] = o_mr[it] + shared;
+        });
+}
+
 // Copies the 9 state planes of n floats (n = side^2, side % 8 == 0) in one launch.
 struct StateCopy {
     const float* s[HG_NUM_STATE_PLANES];
@@ -914,8 +1145,10 @@ __global__ __launch_bounds__(256) void k_zero_planes(PlaneList l, int count, siz
 
 namespace {
 
-// The erosion and flux VJPs as one reverse unit (k_adj_erosion_flux).
+// The erosion and flux VJPs as one reverse unit (k_adj_erosion_flux), and
+// the smoothing and thermal layer-1 VJPs (k_adj_smooth_thermal1).
 constexpr int UNIT_EROSION_FLUX = 1000;
+constexpr int UNIT_SMOOTH_THERMAL1 = 1001;
 
 // Planes each reverse unit produces: their adjoint before the unit is written
 // in full into a fresh buffer (the thermal units stand for outflow + transport
@@ -931,6 +1164,7 @@ std::vector<int> produced(int pass) {
         case HG_PASS_THERMAL_TRANSPORT_0: return {HG_P_ROCK};
         case HG_PASS_THERMAL_TRANSPORT_1: return {HG_P_DIRT};
         case HG_PASS_SMOOTH: return {HG_P_ROCK, HG_P_DIRT};
+        case UNIT_SMOOTH_THERMAL1: return {HG_P_ROCK, HG_P_DIRT};
         case UNIT_EROSION_FLUX: return {HG_P_ROCK, HG_P_DIRT, HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER,
                                         HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
                                         HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH};
@@ -1133,18 +1367,20 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     // reverse units: a thermal layer (outflow then transport) is one unit,
     // named by its transport pass
     std::vector<int> order;
-    if (c->adj_fuse) {
+    if (c->adj_fuse & HG_ADJ_FUSE_EROSION_FLUX) {
         order.push_back(UNIT_EROSION_FLUX);
     } else {
         order.push_back(HG_PASS_FLUX);
         order.push_back(HG_PASS_EROSION);
     }
     order.push_back(HG_PASS_TRANSPORT);
-    if (thermal) {
-        order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
-        order.push_back(HG_PASS_THERMAL_TRANSPORT_1);
+    if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
+    if (thermal && smooth && (c->adj_fuse & HG_ADJ_FUSE_SMOOTH_THERMAL1)) {
+        order.push_back(UNIT_SMOOTH_THERMAL1);
+    } else {
+        if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_1);
+        if (smooth) order.push_back(HG_PASS_SMOOTH);
     }
-    if (smooth) order.push_back(HG_PASS_SMOOTH);
 
     const dim3 blk(BX, BY), grd((n + BX - 1) / BX, (n + BY - 1) / BY);
     const dim3 gtile((n + GTX - 1) / GTX, (n + GTY - 1) / GTY);
@@ -1235,6 +1471,7 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         X.p[HG_P_ROCK] = tp[T_R5]; X.p[HG_P_DIRT] = tp[T_D3];
                         break;
                     case HG_PASS_SMOOTH:
+                    case UNIT_SMOOTH_THERMAL1:             // and tp[T_D3], passed on its own
                         X.p[HG_P_ROCK] = tp[T_R5]; X.p[HG_P_DIRT] = tp[T_D6];
                         break;
                     default: break;
@@ -1280,6 +1517,9 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         break;
                     case HG_PASS_SMOOTH:
                         k_adj_smooth<<<gtile, GNT, 0, c->stream>>>(X, lo, li, n, n, c->kp);
+                        break;
+                    case UNIT_SMOOTH_THERMAL1:
+                        k_adj_smooth_thermal1<<<gtile, GNT, 0, c->stream>>>(X, lo, li, tp[T_D3], n, n, c->kp);
                         break;
                     default: break;
                 }

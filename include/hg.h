@@ -422,9 +422,13 @@ int hg_ref_particle_stats(hg_ctx* ctx, uint64_t* cas_giveups);
  * stream stay with the context for the next call, which reuses them when they
  * are large enough, until hg_destroy. */
 /* Reverse-kernel grouping of hg_adjoint_rollout (no counterpart in the
- * reference).  1 (the default): the erosion and flux adjoints of a step run
- * as one kernel; 0: one kernel each.  Both give the same gradients bit for
- * bit. */
+ * reference): a combination of the bits below, all set by default.  With a
+ * bit set, the adjoints it names run as one kernel per step, otherwise one
+ * kernel each; every combination gives the same gradients bit for bit. */
+enum {
+    HG_ADJ_FUSE_EROSION_FLUX = 1,      /* hydro_erosion + hydro_flux */
+    HG_ADJ_FUSE_SMOOTH_THERMAL1 = 2    /* smoothing + thermal layer 1 (with both flags) */
+};
 int hg_set_adjoint_fusion(hg_ctx* ctx, int on);
 int hg_get_adjoint_fusion(const hg_ctx* ctx, int* on);
 int hg_adjoint_rollout(hg_ctx* ctx, uint32_t nsteps, uint32_t k_ckpt, uint32_t flags,
