@@ -198,6 +198,49 @@ __device__ __forceinline__ void tile_gather(const TileId& T, int nx, int ny, Pre
     }
 }
 
+// tile_gather for kernels whose staged inputs and sends share LDS: cell(it,
+// e, px, py, in, tile) keeps what a cell sends in registers; after a barrier
+// (every read of the staged regions done) send(it, e) writes it to LDS, and
+// after another the gather runs.
+template <class Pre, class Cell, class Send, class Gather>
+__device__ __forceinline__ void tile_gather_late(const TileId& T, int nx, int ny, Pre&& pre, Cell&& cell,
+                                                 Send&& send, Gather&& gather) {
+    const int x0 = T.x * GTX, y0 = T.y * GTY;
+#pragma unroll
+    for (int it = 0; it < GIT; ++it) {
+        int ex, ey;
+        ring_cell(it, ex, ey);
+        const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
+        pre(it, ey * GRW + ex, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx));
+    }
+    __syncthreads();   // staged LDS regions are complete
+#pragma unroll
+    for (int it = 0; it < GIT; ++it) {
+        if (it == GTIT && int(threadIdx.x) >= GNR - GTX * GTY) break;   // past the end of the ring
+        int ex, ey;
+        ring_cell(it, ex, ey);
+        const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
+        const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+        cell(it, ey * GRW + ex, gx, gy, in, it < GTIT);
+    }
+    __syncthreads();   // the staged regions are no longer read
+#pragma unroll
+    for (int it = 0; it < GIT; ++it) {
+        if (it == GTIT && int(threadIdx.x) >= GNR - GTX * GTY) break;
+        int ex, ey;
+        ring_cell(it, ex, ey);
+        send(it, ey * GRW + ex);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < GTIT; ++it) {
+        int ex, ey;
+        ring_cell(it, ex, ey);
+        const int gx = x0 - 1 + ex, gy = y0 - 1 + ey;
+        if (gx < nx && gy < ny) gather(it, ey * GRW + ex, gx, gy);
+    }
+}
+
 // ---- hydro_flux ----------------------------------------------------------
 // VJP of the flux pass at an in-grid cell p = (px, py).  f: p's old outflow
 // (L, R, T, B); b: its water after rain; w, wn: its own and its L, R, T, B
@@ -497,13 +540,26 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion(Planes x, Planes lo, Planes
 __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, Planes li, int nx, int ny,
                                                           KParams k, double* gpart) {
     const TileId T = block_tile();
-    // sent to neighbours: erosion's slope adjoints, flux's inflow (4) and
-    // total-height (4) adjoints
-    __shared__ float s_dx[GNR], s_dz[GNR], s_fin[4][GNR], s_wn[4][GNR];
-    // staged on the tile + 2-cell region: rock and dirt (0 off the grid), total
-    // height (HG_OOB_HEIGHT off the grid) and old flux (0)
-    __shared__ float t_r[SRN], t_d[SRN], t_w[SRN], t_fl[SRN], t_fr[SRN], t_ft[SRN], t_fb[SRN];
+    // One LDS array (26.4 KB), first for the inputs staged on the tile + 2-cell
+    // region: rock and dirt (0 off the grid), total height (HG_OOB_HEIGHT off
+    // the grid) and old flux (0); then, once every cell is evaluated, for what
+    // the cells send their neighbours: erosion's slope adjoints, flux's inflow
+    // (4) and total-height (4) adjoints.
+    constexpr int NU = 7 * SRN > 10 * GNR ? 7 * SRN : 10 * GNR;
+    __shared__ float u[NU];
+    float* t_r = u;
+    float* t_d = u + SRN;
+    float* t_w = u + 2 * SRN;
+    float* t_fl = u + 3 * SRN;
+    float* t_fr = u + 4 * SRN;
+    float* t_ft = u + 5 * SRN;
+    float* t_fb = u + 6 * SRN;
+    float* s_dx = u;
+    float* s_dz = u + GNR;
+    float (*s_fin)[GNR] = reinterpret_cast<float (*)[GNR]>(u + 2 * GNR);
+    float (*s_wn)[GNR] = reinterpret_cast<float (*)[GNR]>(u + 6 * GNR);
     float o_r[GTIT], o_d[GTIT], o_s0[GTIT], o_s1[GTIT], o_fo[GTIT][4], o_wp[GTIT], o_d1[GTIT];
+    float snd[GIT][10];                          // what each of a thread's cells sends
     const float* R = x.p[HG_P_ROCK];
     const float* D = x.p[HG_P_DIRT];
     const float* B = x.p[HG_P_WATER];
@@ -528,7 +584,7 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, P
     float p0[GIT], p1[GIT], pvx[GIT], pvy[GIT], pvz[GIT], plr[GIT], pld[GIT], pl0[GIT], pl1[GIT];
     float pb[GIT], plf[GIT][4], pld2[GIT], plv[GIT][3];
     double gKc = 0, gKs = 0, gKd = 0;
-    tile_gather(T, nx, ny,
+    tile_gather_late(T, nx, ny,
         [&](int it, int, size_t i) {
             p0[it] = x.p[HG_P_SED_ROCK][i]; p1[it] = x.p[HG_P_SED_DIRT][i];
             pvx[it] = x.p[HG_P_VEL_U][i]; pvy[it] = x.p[HG_P_VEL_V][i]; pvz[it] = x.p[HG_P_VEL_DEPTH][i];
@@ -560,11 +616,11 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, P
                 flux_vjp(f, pb[it], t_w[sc], wn, fin, px, py, nx, ny, plf[it], pld2[it], lvx, lvy, lvz, k, lfo,
                          lfin, lwn, ld1, lwp);
             }
-            s_dx[e] = ev.ldx;
-            s_dz[e] = ev.ldz;
+            snd[it][0] = ev.ldx;
+            snd[it][1] = ev.ldz;
             for (int j = 0; j < 4; ++j) {
-                s_fin[j][e] = lfin[j];
-                s_wn[j][e] = lwn[j];
+                snd[it][2 + j] = lfin[j];
+                snd[it][6 + j] = lwn[j];
             }
             if (it < GTIT) {                                  // a tile cell keeps its own part
                 o_r[it] = ev.lr;
@@ -574,6 +630,14 @@ __global__ __launch_bounds__(GNT) void k_adj_erosion_flux(Planes x, Planes lo, P
                 for (int j = 0; j < 4; ++j) o_fo[it][j] = lfo[j];
                 o_wp[it] = lwp;
                 o_d1[it] = ld1;
+            }
+        },
+        [&](int it, int e) {
+            s_dx[e] = snd[it][0];
+            s_dz[e] = snd[it][1];
+            for (int j = 0; j < 4; ++j) {
+                s_fin[j][e] = snd[it][2 + j];
+                s_wn[j][e] = snd[it][6 + j];
             }
         },
         [&](int it, int e, int px, int py) {
@@ -636,10 +700,18 @@ __device__ __forceinline__ void adj_transport_tile(const TileId& T, float* lds, 
     const float* VY = x.p[HG_P_VEL_V];
     double gKe = 0;
     float pvx[GIT], pvy[GIT], pl0[GIT], pl1[GIT];
+    // a tile cell's own water and its adjoint, and the sediment at its own
+    // back-trace taps (both layers: s00, s10, s01, s11), loaded before the
+    // gather phase needs them
+    float plb[GTIT], pb[GTIT], ps[GTIT][8];
     tile_gather(T, nx, ny,
         [&](int it, int, size_t i) {
             pvx[it] = VX[i]; pvy[it] = VY[i];
             pl0[it] = lo.p[HG_P_SED_ROCK][i]; pl1[it] = lo.p[HG_P_SED_DIRT][i];
+            if (it < GTIT) {
+                plb[it] = lo.p[HG_P_WATER][i];
+                pb[it] = x.p[HG_P_WATER][i];
+            }
         },
         [&](int it, int e, int qx, int qy, bool in, bool) {
             unsigned code = 0;
@@ -650,6 +722,16 @@ __device__ __forceinline__ void adj_transport_tile(const TileId& T, float* lds, 
                     code = unsigned(t.ix - qx + 1) | (unsigned(t.ix1 - qx + 1) << 2) | (unsigned(t.iy - qy + 1) << 4) |
                            (unsigned(t.iy1 - qy + 1) << 6);
                     w[0] = t.w00; w[1] = t.w10; w[2] = t.w01; w[3] = t.w11;
+                }
+                if (it < GTIT) {
+                    const size_t j00 = at(t.ix, t.iy, nx), j10 = at(t.ix1, t.iy, nx), j01 = at(t.ix, t.iy1, nx),
+                                 j11 = at(t.ix1, t.iy1, nx);
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const float* S = x.p[c == 0 ? HG_P_SED_ROCK : HG_P_SED_DIRT];
+                        ps[it][4 * c + 0] = S[j00]; ps[it][4 * c + 1] = S[j10];
+                        ps[it][4 * c + 2] = S[j01]; ps[it][4 * c + 3] = S[j11];
+                    }
                 }
             }
 #pragma unroll
@@ -695,16 +777,15 @@ __device__ __forceinline__ void adj_transport_tile(const TileId& T, float* lds, 
             const float bxr = float(px) - vx * k.dt;
             const float byr = float(py) - vy * k.dt;
             const Taps t = taps_of(px, py, vx, vy, nx, ny, k.dt);
-            const size_t j00 = at(t.ix, t.iy, nx), j10 = at(t.ix1, t.iy, nx), j01 = at(t.ix, t.iy1, nx),
-                         j11 = at(t.ix1, t.iy1, nx);
             float bx, by;
             back_trace(px, py, vx, vy, k.dt, nx, ny, bx, by);
             const float fx = bx - float(t.ix), fy = by - float(t.iy);
             const float l0 = pl0[it], l1 = pl1[it];
             float lbx = 0.0f, lby = 0.0f;
+#pragma unroll
             for (int c = 0; c < 2; ++c) {
-                const float* S = x.p[c == 0 ? HG_P_SED_ROCK : HG_P_SED_DIRT];
-                const float s00 = S[j00], s10 = S[j10], s01 = S[j01], s11 = S[j11];
+                const float s00 = ps[it][4 * c + 0], s10 = ps[it][4 * c + 1], s01 = ps[it][4 * c + 2],
+                            s11 = ps[it][4 * c + 3];
                 const float l = c == 0 ? l0 : l1;
                 lbx += l * ((1.0f - fy) * (s10 - s00) + fy * (s11 - s01));
                 lby += l * (lerp1(s01, s11, fx) - lerp1(s00, s10, fx));
@@ -714,9 +795,9 @@ __device__ __forceinline__ void adj_transport_tile(const TileId& T, float* lds, 
             li.p[HG_P_VEL_U][i] = (bxr > 0.0f && bxr < float(nx - 1)) ? -k.dt * lbx : 0.0f;
             li.p[HG_P_VEL_V][i] = (byr > 0.0f && byr < float(ny - 1)) ? -k.dt * lby : 0.0f;
             li.p[HG_P_VEL_DEPTH][i] = 0.0f;
-            const float lb = lo.p[HG_P_WATER][i];
+            const float lb = plb[it];
             li.p[HG_P_WATER][i] = lb * k.evap;                   // b' = b (1 - Ke dt)
-            gKe += double(lb) * double(x.p[HG_P_WATER][i]) * double(-k.dt);
+            gKe += double(lb) * double(pb[it]) * double(-k.dt);
         });
     reduce_coef<GNT>(gpart, T.slot, 0.0, 0.0, 0.0, gKe);
 }
@@ -797,8 +878,9 @@ __device__ __forceinline__ void thermal_vjp(const float Dk[8], float h_layer, fl
     }
 }
 
-// LDS of one thermal tile, in floats.
-constexpr int THERMAL_LDS = 8 * GNR + 3 * SRN;
+// LDS of one thermal tile, in floats: the staged inputs, then (once every
+// cell is evaluated) what the cells send their 8 neighbours.
+constexpr int THERMAL_LDS = 8 * GNR > 3 * SRN ? 8 * GNR : 3 * SRN;
 
 // A thermal layer's VJP on tile T, with `lds` (THERMAL_LDS floats) as its
 // LDS.  x: the layer's input rock and dirt.  Writes li's layer plane (fresh)
@@ -806,11 +888,12 @@ constexpr int THERMAL_LDS = 8 * GNR + 3 * SRN;
 template <int layer>
 __device__ __forceinline__ void adj_thermal_tile(const TileId& T, float* lds, Planes x, Planes lo, Planes li,
                                                  int nx, int ny, const KParams& k) {
+    float* t_r = lds;
+    float* t_d = lds + SRN;
+    float* t_lh = lds + 2 * SRN;
     float (*s_lD)[GNR] = reinterpret_cast<float (*)[GNR]>(lds);   // sent to the 8 neighbours
-    float* t_r = lds + 8 * GNR;
-    float* t_d = t_r + SRN;
-    float* t_lh = t_d + SRN;
-    float o_self[GTIT], o_hl[GTIT];                 // kept per thread
+    float o_self[GTIT], o_hl[GTIT], o_lh[GTIT];     // kept per thread
+    float snd[GIT][8];
     constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
     const float* R = x.p[HG_P_ROCK];
@@ -835,13 +918,14 @@ __device__ __forceinline__ void adj_thermal_tile(const TileId& T, float* lds, Pl
             t_lh[s] = c[it];
         });
     }
-    tile_gather(T, nx, ny,
+    tile_gather_late(T, nx, ny,
         [&](int, int, size_t) {},
         [&](int it, int e, int px, int py, bool in, bool) {
             float lD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             float lself = 0.0f, lh_layer = 0.0f;
+            const int sc = sidx(e);
+            if (it < GTIT) o_lh[it] = t_lh[sc];
             if (in) {
-                const int sc = sidx(e);
                 const float r = t_r[sc], d = layer == 1 ? t_d[sc] : 0.0f;
                 float Dk[8], lhn[8];
                 bool ok[8];
@@ -858,11 +942,15 @@ __device__ __forceinline__ void adj_thermal_tile(const TileId& T, float* lds, Pl
                 thermal_vjp<layer>(Dk, layer == 0 ? r : d, t_lh[sc], lhn, ok, k, lD, lself, lh_layer);
             }
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = lD[kk];
+            for (int kk = 0; kk < 8; ++kk) snd[it][kk] = lD[kk];
             if (it < GTIT) {                                  // a tile cell keeps its own part
                 o_self[it] = lself;
                 o_hl[it] = lh_layer;
             }
+        },
+        [&](int it, int e) {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = snd[it][kk];
         },
         [&](int it, int e, int px, int py) {
             const size_t i = at(px, py, nx);
@@ -870,7 +958,7 @@ __device__ __forceinline__ void adj_thermal_tile(const TileId& T, float* lds, Pl
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) recv -= s_lD[kk][e - (DX[kk] + DY[kk] * GRW)];
             const float shared = o_self[it] + recv;
-            const float lh = t_lh[sidx(e)];
+            const float lh = o_lh[it];
             if constexpr (layer == 0) {
                 li.p[HG_P_ROCK][i] = lh + shared + o_hl[it];
             } else {
@@ -979,30 +1067,37 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
     const TileId T = block_tile();
     constexpr int W4 = GTX + 8, N4 = W4 * (GTY + 8), IT4 = (N4 + GNT - 1) / GNT;
     constexpr int W3 = GTX + 6, N3 = W3 * (GTY + 6), IT3 = (N3 + GNT - 1) / GNT;
-    // LDS shared by the two halves: the smoothing's inputs and flags, then
-    // what layer 1 sends to the 8 neighbours (30.9 KB in all: 5 workgroups
-    // per CU, as k_adj_thermal<1>)
+    // One LDS array (21.1 KB), in turn:
+    //  * rock after layer 0 (t_r5) and dirt after layer 1 (t_d6) on tile + 4,
+    //    HG_OOB_HEIGHT off the grid (the flags of cells off or on the edge of
+    //    the grid are not evaluated, and layer 1 reads the rock as
+    //    k_adj_thermal<1> stages it); the adjoints after smoothing on tile + 3
+    //    (0 off the grid) and their extremum flags;
+    //  * once the flags are set, the dirt adjoint before smoothing on tile + 2
+    //    (t_md, 0 off the grid) in t_d6's place; once the smoothing VJP is
+    //    done, dirt after erosion on tile + 2 (t_d) in the adjoints' place;
+    //  * once every cell's layer-1 VJP is evaluated, what the cells send
+    //    their 8 neighbours.
     constexpr int NU0 = 2 * N4 + 2 * N3 + (N3 + 3) / 4, NU1 = 8 * GNR;
     constexpr int NU = NU0 > NU1 ? NU0 : NU1;
+    static_assert(SRN <= N4 && SRN <= 2 * N3, "the tile + 2 arrays fit the places they take");
     __shared__ float u[NU];
-    // rock after layer 0 and dirt after layer 1 on tile + 4 (HG_OOB_HEIGHT
-    // off the grid; the flags of cells off or on the edge of the grid are
-    // not evaluated)
     float* t_r5 = u;
     float* t_d6 = u + N4;
-    float* t_lr = u + 2 * N4;                    // adjoints after smoothing on tile + 3 (0 off the grid)
+    float* t_lr = u + 2 * N4;
     float* t_ld = t_lr + N3;
-    unsigned char* s_f = reinterpret_cast<unsigned char*>(t_ld + N3);   // extremum flags, tile + 3
-    float (*s_lD)[GNR] = reinterpret_cast<float (*)[GNR]>(u);          // layer 1's sends
-    // tile + 2: rock after layer 0 and dirt after erosion (HG_OOB_HEIGHT off
-    // the grid), the dirt adjoint before smoothing (0 off the grid)
-    __shared__ float t_r[SRN], t_d[SRN], t_md[SRN];
-    float o_mr[GTIT], o_self[GTIT], o_hl[GTIT];
+    unsigned char* s_f = reinterpret_cast<unsigned char*>(t_ld + N3);
+    float* t_md = t_d6;
+    float* t_d = t_lr;
+    float (*s_lD)[GNR] = reinterpret_cast<float (*)[GNR]>(u);
+    float o_mr[GTIT], o_lh[GTIT], o_self[GTIT], o_hl[GTIT];
+    float snd[GIT][8];
     constexpr int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
     constexpr int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
     const float m = k.smooth_m;
+    float e[SIT];                                // dirt after erosion, tile + 2, until its place is free
     {
-        float a[IT4], b[IT4], c[IT3], d[IT3], e[SIT];
+        float a[IT4], b[IT4], c[IT3], d[IT3];
         region_walk<4>(T, nx, ny, [&](int it, int, size_t i, bool in, int, int) {
             const float r = x.p[HG_P_ROCK][i], dd = x.p[HG_P_DIRT][i];
             a[it] = in ? r : HG_OOB_HEIGHT;
@@ -1019,7 +1114,6 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
         });
         region_walk<4>(T, nx, ny, [&](int it, int s, size_t, bool, int, int) { t_r5[s] = a[it]; t_d6[s] = b[it]; });
         region_walk<3>(T, nx, ny, [&](int it, int s, size_t, bool, int, int) { t_lr[s] = c[it]; t_ld[s] = d[it]; });
-        region_walk<2>(T, nx, ny, [&](int it, int s, size_t, bool, int, int) { t_d[s] = e[it]; });
     }
     __syncthreads();
     // extremum flags on tile + 3, as k_adj_smooth evaluates them
@@ -1061,12 +1155,10 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
         }
         return ad;
     };
-    // the dirt adjoint on tile + 2 (layer 1's input adjoint) and the rock
-    // on tile + 2 (layer 1's heights) into their own arrays
+    // the dirt adjoint on tile + 2 (layer 1's input adjoint)
     region_walk<2>(T, nx, ny, [&](int, int s, size_t, bool in, int, int) {
         const int sy = s / SRW, sx = s - sy * SRW;
         t_md[s] = in ? smooth_d((sy + 1) * W3 + sx + 1) : 0.0f;
-        t_r[s] = t_r5[(sy + 2) * W4 + sx + 2];
     });
     // the rock adjoint of each tile cell, kept by the thread that writes it
 #pragma unroll
@@ -1075,16 +1167,21 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
         ring_cell(it, ex, ey);
         o_mr[it] = smooth_r((ey + 2) * W3 + ex + 2);
     }
-    // layer 1, as k_adj_thermal<1> with t_lh = t_md (tile_gather's first
-    // barrier ends the reads of the shared LDS above)
-    tile_gather(T, nx, ny,
+    __syncthreads();                             // the adjoints after smoothing are no longer read
+    region_walk<2>(T, nx, ny, [&](int it, int s, size_t, bool, int, int) { t_d[s] = e[it]; });
+    // layer 1, as k_adj_thermal<1> with t_lh = t_md (tile_gather_late's first
+    // barrier completes t_d)
+    tile_gather_late(T, nx, ny,
         [&](int, int, size_t) {},
         [&](int it, int e, int px, int py, bool in, bool) {
             float lD[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             float lself = 0.0f, lh_layer = 0.0f;
+            const int sc = sidx(e);
+            if (it < GTIT) o_lh[it] = t_md[sc];
             if (in) {
-                const int sc = sidx(e);
-                const float r = t_r[sc], d = t_d[sc];
+                const int sy = sc / SRW, sx = sc - sy * SRW;
+                const int c4 = (sy + 2) * W4 + sx + 2;
+                const float r = t_r5[c4], d = t_d[sc];
                 float Dk[8], lhn[8];
                 bool ok[8];
 #pragma unroll
@@ -1092,17 +1189,21 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
                     const int xx = px + DX[kk], yy = py + DY[kk];
                     const int o = sc + DX[kk] + DY[kk] * SRW;
                     ok[kk] = xx >= 0 && yy >= 0 && xx < nx && yy < ny;
-                    Dk[kk] = (r - t_r[o]) + (d - t_d[o]);
+                    Dk[kk] = (r - t_r5[c4 + DX[kk] + DY[kk] * W4]) + (d - t_d[o]);
                     lhn[kk] = t_md[o];
                 }
                 thermal_vjp<1>(Dk, d, t_md[sc], lhn, ok, k, lD, lself, lh_layer);
             }
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = lD[kk];
+            for (int kk = 0; kk < 8; ++kk) snd[it][kk] = lD[kk];
             if (it < GTIT) {
                 o_self[it] = lself;
                 o_hl[it] = lh_layer;
             }
+        },
+        [&](int it, int e) {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) s_lD[kk][e] = snd[it][kk];
         },
         [&](int it, int e, int px, int py) {
             const size_t i = at(px, py, nx);
@@ -1110,7 +1211,7 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) recv -= s_lD[kk][e - (DX[kk] + DY[kk] * GRW)];
             const float shared = o_self[it] + recv;
-            li.p[HG_P_DIRT][i] = t_md[sidx(e)] + shared + o_hl[it];
+            li.p[HG_P_DIRT][i] = o_lh[it] + shared + o_hl[it];
             li.p[HG_P_ROCK][i] = o_mr[it] + shared;
         });
 }
