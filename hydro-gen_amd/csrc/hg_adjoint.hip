@@ -22,9 +22,13 @@
 //    clamps, max/min, the talus test and the erosion cases contribute the
 //    derivative of the branch taken (a subgradient at switching points).
 //  * contributions a cell sends to its neighbours are gathered, not
-//    scattered: each tile evaluates its cells and a one-cell ring into LDS and
-//    every output is written once.  Only semi-Lagrangian back-traces longer
-//    than one cell scatter, with float atomic adds, after the gather.
+//    scattered: each tile evaluates its cells and a one-cell ring, passes
+//    what they send through LDS and writes every output once.  Only
+//    semi-Lagrangian back-traces longer than one cell scatter, with float
+//    atomic adds, after the gather.
+//  * by default two pairs of pass adjoints run as one kernel each (erosion +
+//    flux, smoothing + thermal layer 1), with the same arithmetic as the
+//    kernels of the single passes.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -41,14 +45,6 @@ constexpr int BX = 64, BY = 4;
 
 __device__ __forceinline__ size_t at(int x, int y, int nx) { return size_t(y) * size_t(nx) + size_t(x); }
 
-// p[x, y], or `fill` off the grid.  The load itself is unconditional (from the
-// clamped cell), so a kernel's neighbour loads issue together instead of each
-// waiting behind a branch.
-__device__ __forceinline__ float ldc(const float* p, int x, int y, int nx, int ny, float fill) {
-    const int cx = min(max(x, 0), nx - 1), cy = min(max(y, 0), ny - 1);
-    const float v = p[at(cx, cy, nx)];
-    return (cx == x && cy == y) ? v : fill;
-}
 __device__ __forceinline__ void aadd(float* p, float v) {
     if (v != 0.0f) unsafeAtomicAdd(p, v);
 }
@@ -121,20 +117,30 @@ __device__ __forceinline__ int sidx(int e) {
     return (ey + 1) * SRW + ex + 1;
 }
 
-// fn(it, s, i, in) for the cells of the tile + 2-cell region: s its index in
-// the staged arrays, i the (clamped) global index, in whether it is on the grid.
-// Threads past the end redo the last cell (same values, same LDS slot).
-template <class F>
-__device__ __forceinline__ void stage_walk(const TileId& T, int nx, int ny, F&& fn) {
-    const int x0 = T.x * GTX - 2, y0 = T.y * GTY - 2;
+// fn(it, s, i, in, gx, gy) for the cells of the tile + H region: s its index
+// in a (GTX + 2H) x (GTY + 2H) array, i the (clamped) global index.  Threads
+// past the end redo the last cell.
+template <int H, class F>
+__device__ __forceinline__ void region_walk(const TileId& T, int nx, int ny, F&& fn) {
+    constexpr int W = GTX + 2 * H, N = W * (GTY + 2 * H), IT = (N + GNT - 1) / GNT;
+    const int x0 = T.x * GTX - H, y0 = T.y * GTY - H;
 #pragma unroll
-    for (int it = 0; it < SIT; ++it) {
-        const int s = min(int(threadIdx.x) + it * GNT, SRN - 1);
-        const int sy = s / SRW, sx = s - sy * SRW;
+    for (int it = 0; it < IT; ++it) {
+        const int s = min(int(threadIdx.x) + it * GNT, N - 1);
+        const int sy = s / W, sx = s - sy * W;
         const int gx = x0 + sx, gy = y0 + sy;
         const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
-        fn(it, s, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx), in);
+        fn(it, s, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx), in, gx, gy);
     }
+}
+
+// fn(it, s, i, in) for the cells of the tile + 2-cell region: s its index in
+// the staged (SRW x SRH) arrays, i the (clamped) global index, in whether it is
+// on the grid.
+template <class F>
+__device__ __forceinline__ void stage_walk(const TileId& T, int nx, int ny, F&& fn) {
+    static_assert((SRN + GNT - 1) / GNT == SIT, "one iteration per staged cell of a thread");
+    region_walk<2>(T, nx, ny, [&](int it, int s, size_t i, bool in, int, int) { fn(it, s, i, in); });
 }
 
 // A thread's cells of the tile + ring, numbered j = threadIdx.x + it * GNT:
@@ -1044,23 +1050,6 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes 
 // layer 1 (x.p[HG_P_DIRT]), the smoothing's inputs; d3: dirt after erosion,
 // layer 1's input dirt.  lo: the adjoints after smoothing.  Writes li's rock
 // and dirt (fresh).
-
-// fn(it, s, i, in, gx, gy) for the cells of the tile + H region: s its index
-// in a (GTX + 2H) x (GTY + 2H) array, i the (clamped) global index.  Threads
-// past the end redo the last cell.
-template <int H, class F>
-__device__ __forceinline__ void region_walk(const TileId& T, int nx, int ny, F&& fn) {
-    constexpr int W = GTX + 2 * H, N = W * (GTY + 2 * H), IT = (N + GNT - 1) / GNT;
-    const int x0 = T.x * GTX - H, y0 = T.y * GTY - H;
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        const int s = min(int(threadIdx.x) + it * GNT, N - 1);
-        const int sy = s / W, sx = s - sy * W;
-        const int gx = x0 + sx, gy = y0 + sy;
-        const bool in = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
-        fn(it, s, at(min(max(gx, 0), nx - 1), min(max(gy, 0), ny - 1), nx), in, gx, gy);
-    }
-}
 
 __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo, Planes li, const float* d3,
                                                              int nx, int ny, KParams k) {
