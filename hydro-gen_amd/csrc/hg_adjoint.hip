@@ -814,10 +814,10 @@ __global__ __launch_bounds__(GNT) void k_adj_transport(Planes x, Planes lo, Plan
     adj_transport_tile(block_tile(), lds, x, lo, li, nx, ny, k, gpart);
 }
 
-__global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, Planes li, int nx, int ny,
-                                                           KParams k) {
-    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
-    if (px >= nx || py >= ny) return;
+// The transport VJP's scatter for cell (px, py) when its back-trace taps are
+// not all within one cell (after the near gather has written li).
+__device__ __forceinline__ void transport_far_cell(const Planes& x, const Planes& lo, const Planes& li, int px,
+                                                   int py, int nx, int ny, const KParams& k) {
     const size_t i = at(px, py, nx);
     const Taps t = taps_of(px, py, x.p[HG_P_VEL_U][i], x.p[HG_P_VEL_V][i], nx, ny, k.dt);
     if (t.near) return;
@@ -832,6 +832,13 @@ __global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, 
     aadd(li.p[HG_P_SED_DIRT] + j10, l1 * t.w10);
     aadd(li.p[HG_P_SED_DIRT] + j01, l1 * t.w01);
     aadd(li.p[HG_P_SED_DIRT] + j11, l1 * t.w11);
+}
+
+__global__ __launch_bounds__(256) void k_adj_transport_far(Planes x, Planes lo, Planes li, int nx, int ny,
+                                                           KParams k) {
+    const int px = blockIdx.x * BX + threadIdx.x, py = blockIdx.y * BY + threadIdx.y;
+    if (px >= nx || py >= ny) return;
+    transport_far_cell(x, lo, li, px, py, nx, ny, k);
 }
 
 
@@ -978,6 +985,27 @@ template <int layer>
 __global__ __launch_bounds__(GNT) void k_adj_thermal(Planes x, Planes lo, Planes li, int nx, int ny, KParams k) {
     __shared__ float lds[THERMAL_LDS];
     adj_thermal_tile<layer>(block_tile(), lds, x, lo, li, nx, ny, k);
+}
+
+// ---- thermal layer 0 and the transport's far taps in one launch ----------
+// Run after the transport's near gather (the two passes' adjoints touch
+// disjoint planes, so their order does not matter): workgroups [0, tiles)
+// take the layer-0 VJP's tiles in row-major order (as k_adj_thermal<0>'s
+// grid does), the rest scatter the far taps of 256 cells each (as
+// k_adj_transport_far), filling the thermal tiles' tail instead of a launch
+// of their own.  x0: layer 0's inputs; xt, lot, lit: the transport's.
+__global__ __launch_bounds__(GNT) void k_adj_thermal0_far(Planes x0, Planes lo0, Planes li0, Planes xt, Planes lot,
+                                                          Planes lit, int nx, int ny, KParams k) {
+    __shared__ float lds[THERMAL_LDS];
+    const int ntx = (nx + GTX - 1) / GTX, nty = (ny + GTY - 1) / GTY;
+    const int w = int(blockIdx.x);
+    if (w < ntx * nty) {
+        adj_thermal_tile<0>(TileId{w % ntx, w / ntx, w}, lds, x0, lo0, li0, nx, ny, k);
+        return;
+    }
+    const size_t cell = size_t(w - ntx * nty) * GNT + threadIdx.x;
+    if (cell >= size_t(nx) * size_t(ny)) return;
+    transport_far_cell(xt, lot, lit, int(cell % size_t(nx)), int(cell / size_t(nx)), nx, ny, k);
 }
 
 // ---- smoothing -----------------------------------------------------------
@@ -1239,6 +1267,7 @@ namespace {
 // the smoothing and thermal layer-1 VJPs (k_adj_smooth_thermal1).
 constexpr int UNIT_EROSION_FLUX = 1000;
 constexpr int UNIT_SMOOTH_THERMAL1 = 1001;
+constexpr int UNIT_TRANSPORT_THERMAL0 = 1002;  // k_adj_transport, then k_adj_thermal0_far
 
 // Planes each reverse unit produces: their adjoint before the unit is written
 // in full into a fresh buffer (the thermal units stand for outflow + transport
@@ -1255,6 +1284,7 @@ std::vector<int> produced(int pass) {
         case HG_PASS_THERMAL_TRANSPORT_1: return {HG_P_DIRT};
         case HG_PASS_SMOOTH: return {HG_P_ROCK, HG_P_DIRT};
         case UNIT_SMOOTH_THERMAL1: return {HG_P_ROCK, HG_P_DIRT};
+        case UNIT_TRANSPORT_THERMAL0: return {HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER, HG_P_ROCK};
         case UNIT_EROSION_FLUX: return {HG_P_ROCK, HG_P_DIRT, HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER,
                                         HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
                                         HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH};
@@ -1463,8 +1493,12 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
         order.push_back(HG_PASS_FLUX);
         order.push_back(HG_PASS_EROSION);
     }
-    order.push_back(HG_PASS_TRANSPORT);
-    if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
+    if (thermal && (c->adj_fuse & HG_ADJ_FUSE_FAR_THERMAL0)) {
+        order.push_back(UNIT_TRANSPORT_THERMAL0);
+    } else {
+        order.push_back(HG_PASS_TRANSPORT);
+        if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
+    }
     if (thermal && smooth && (c->adj_fuse & HG_ADJ_FUSE_SMOOTH_THERMAL1)) {
         order.push_back(UNIT_SMOOTH_THERMAL1);
     } else {
@@ -1598,6 +1632,15 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         HG_HIP(c, hipGetLastError());
                         k_adj_transport_far<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
+                    case UNIT_TRANSPORT_THERMAL0: {
+                        const Planes Xt = x_of(HG_PASS_TRANSPORT), X0 = x_of(HG_PASS_THERMAL_TRANSPORT_0);
+                        k_adj_transport<<<gtile, GNT, 0, c->stream>>>(Xt, lo, li, n, n, c->kp, w.gpart);
+                        HG_HIP(c, hipGetLastError());
+                        const unsigned nfar = unsigned((size_t(n) * size_t(n) + GNT - 1) / GNT);
+                        k_adj_thermal0_far<<<gtile.x * gtile.y + nfar, GNT, 0, c->stream>>>(X0, lo, li, Xt, lo, li,
+                                                                                            n, n, c->kp);
+                        break;
+                    }
                     case HG_PASS_THERMAL_TRANSPORT_0:
                     case HG_PASS_THERMAL_TRANSPORT_1:
                         if (pass == HG_PASS_THERMAL_TRANSPORT_0)
