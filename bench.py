@@ -306,9 +306,8 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     t0 = time.perf_counter()
     gh_full, gc_full = t.adjoint_rollout(steps, steps, g)
     t_full = time.perf_counter() - t0
-    # the checkpointed rollout again with one launch per pass adjoint (the
-    # erosion + flux, smoothing + thermal layer-1 and far-tap + thermal
-    # layer-0 launches split)
+    # the checkpointed rollout again with one launch per pass adjoint (no
+    # fused reverse launches)
     t.set_adjoint_fusion(0)
     t.write_state(init)
     t.step = 0
@@ -319,7 +318,7 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     t0 = time.perf_counter()
     gh_sep, gc_sep = t.adjoint_rollout(steps, k_ckpt, g)
     t_sep = time.perf_counter() - t0
-    t.set_adjoint_fusion(7)
+    t.set_adjoint_fusion(15)
     t.close()
     return {"grid": n, "steps": steps, "k_ckpt": k_ckpt, "tile_variants": list(tiles), "loss": loss,
             "t_forward_ms": t_fwd * 1e3, "t_forward_adjoint_ms": t_adj * 1e3,

@@ -1079,11 +1079,17 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth(Planes x, Planes lo, Planes 
 // layer 1's input dirt.  lo: the adjoints after smoothing.  Writes li's rock
 // and dirt (fresh).
 
-__global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo, Planes li, const float* d3,
-                                                             int nx, int ny, KParams k) {
-    const TileId T = block_tile();
-    constexpr int W4 = GTX + 8, N4 = W4 * (GTY + 8), IT4 = (N4 + GNT - 1) / GNT;
-    constexpr int W3 = GTX + 6, N3 = W3 * (GTY + 6), IT3 = (N3 + GNT - 1) / GNT;
+// LDS of one smoothing + layer-1 tile, in floats (see adj_smooth_thermal1_tile).
+constexpr int ST1_N4 = (GTX + 8) * (GTY + 8), ST1_N3 = (GTX + 6) * (GTY + 6);
+constexpr int ST1_LDS = 2 * ST1_N4 + 2 * ST1_N3 + (ST1_N3 + 3) / 4 > 8 * GNR
+                            ? 2 * ST1_N4 + 2 * ST1_N3 + (ST1_N3 + 3) / 4
+                            : 8 * GNR;
+
+__device__ __forceinline__ void adj_smooth_thermal1_tile(const TileId& T, float* u, Planes x, Planes lo,
+                                                         Planes li, const float* d3, int nx, int ny,
+                                                         const KParams& k) {
+    constexpr int W4 = GTX + 8, N4 = ST1_N4, IT4 = (N4 + GNT - 1) / GNT;
+    constexpr int W3 = GTX + 6, N3 = ST1_N3, IT3 = (N3 + GNT - 1) / GNT;
     // One LDS array (21.1 KB), in turn:
     //  * rock after layer 0 (t_r5) and dirt after layer 1 (t_d6) on tile + 4,
     //    HG_OOB_HEIGHT off the grid (the flags of cells off or on the edge of
@@ -1095,10 +1101,7 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
     //    done, dirt after erosion on tile + 2 (t_d) in the adjoints' place;
     //  * once every cell's layer-1 VJP is evaluated, what the cells send
     //    their 8 neighbours.
-    constexpr int NU0 = 2 * N4 + 2 * N3 + (N3 + 3) / 4, NU1 = 8 * GNR;
-    constexpr int NU = NU0 > NU1 ? NU0 : NU1;
     static_assert(SRN <= N4 && SRN <= 2 * N3, "the tile + 2 arrays fit the places they take");
-    __shared__ float u[NU];
     float* t_r5 = u;
     float* t_d6 = u + N4;
     float* t_lr = u + 2 * N4;
@@ -1233,6 +1236,35 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo
         });
 }
 
+__global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1(Planes x, Planes lo, Planes li, const float* d3,
+                                                             int nx, int ny, KParams k) {
+    __shared__ float u[ST1_LDS];
+    adj_smooth_thermal1_tile(block_tile(), u, x, lo, li, d3, nx, ny, k);
+}
+
+// ---- smoothing + thermal layer 1 and the transport's near gather ----------
+// The two touch disjoint planes (rock and dirt; sediment, water and
+// velocity), so one launch runs both: workgroups [0, tiles) take the
+// smoothing + layer-1 tiles in row-major order, the next `tiles` workgroups
+// the transport's tiles (each on the XCD its own kernel would give it, with
+// the tile's coefficient slot), starting as the first ones finish.  xs, los,
+// lis: the smoothing + layer-1 planes; xt, lot, lit: the transport's.
+__global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1_transport(Planes xs, Planes los, Planes lis,
+                                                                       const float* d3, Planes xt, Planes lot,
+                                                                       Planes lit, int nx, int ny, KParams k,
+                                                                       double* gpart) {
+    constexpr int NL = ST1_LDS > TRANSPORT_LDS ? ST1_LDS : TRANSPORT_LDS;
+    __shared__ float u[NL];
+    const int ntx = (nx + GTX - 1) / GTX, nty = (ny + GTY - 1) / GTY, nt = ntx * nty;
+    const int w = int(blockIdx.x);
+    if (w < nt) {
+        adj_smooth_thermal1_tile(TileId{w % ntx, w / ntx, w}, u, xs, los, lis, d3, nx, ny, k);
+    } else {
+        const int t = w - nt;
+        adj_transport_tile(TileId{t % ntx, t / ntx, t}, u, xt, lot, lit, nx, ny, k, gpart);
+    }
+}
+
 // Copies the 9 state planes of n floats (n = side^2, side % 8 == 0) in one launch.
 struct StateCopy {
     const float* s[HG_NUM_STATE_PLANES];
@@ -1268,6 +1300,10 @@ namespace {
 constexpr int UNIT_EROSION_FLUX = 1000;
 constexpr int UNIT_SMOOTH_THERMAL1 = 1001;
 constexpr int UNIT_TRANSPORT_THERMAL0 = 1002;  // k_adj_transport, then k_adj_thermal0_far
+// With all four fusions: the smoothing + layer-1 and transport tiles in one
+// launch (k_adj_smooth_thermal1_transport), then layer 0 and the far taps.
+constexpr int UNIT_ST1_TRANSPORT = 1003;
+constexpr int UNIT_THERMAL0_FAR = 1004;
 
 // Planes each reverse unit produces: their adjoint before the unit is written
 // in full into a fresh buffer (the thermal units stand for outflow + transport
@@ -1285,6 +1321,8 @@ std::vector<int> produced(int pass) {
         case HG_PASS_SMOOTH: return {HG_P_ROCK, HG_P_DIRT};
         case UNIT_SMOOTH_THERMAL1: return {HG_P_ROCK, HG_P_DIRT};
         case UNIT_TRANSPORT_THERMAL0: return {HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER, HG_P_ROCK};
+        case UNIT_ST1_TRANSPORT: return {HG_P_ROCK, HG_P_DIRT, HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER};
+        case UNIT_THERMAL0_FAR: return {HG_P_ROCK};
         case UNIT_EROSION_FLUX: return {HG_P_ROCK, HG_P_DIRT, HG_P_SED_ROCK, HG_P_SED_DIRT, HG_P_WATER,
                                         HG_P_FLUX_L, HG_P_FLUX_R, HG_P_FLUX_T, HG_P_FLUX_B,
                                         HG_P_VEL_U, HG_P_VEL_V, HG_P_VEL_DEPTH};
@@ -1493,17 +1531,24 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
         order.push_back(HG_PASS_FLUX);
         order.push_back(HG_PASS_EROSION);
     }
-    if (thermal && (c->adj_fuse & HG_ADJ_FUSE_FAR_THERMAL0)) {
-        order.push_back(UNIT_TRANSPORT_THERMAL0);
+    const bool fuse_st1 = thermal && smooth && (c->adj_fuse & HG_ADJ_FUSE_SMOOTH_THERMAL1);
+    const bool fuse_far0 = thermal && (c->adj_fuse & HG_ADJ_FUSE_FAR_THERMAL0);
+    if (fuse_st1 && fuse_far0 && (c->adj_fuse & HG_ADJ_FUSE_TRANSPORT_BESIDE)) {
+        order.push_back(UNIT_THERMAL0_FAR);
+        order.push_back(UNIT_ST1_TRANSPORT);
     } else {
-        order.push_back(HG_PASS_TRANSPORT);
-        if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
-    }
-    if (thermal && smooth && (c->adj_fuse & HG_ADJ_FUSE_SMOOTH_THERMAL1)) {
-        order.push_back(UNIT_SMOOTH_THERMAL1);
-    } else {
-        if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_1);
-        if (smooth) order.push_back(HG_PASS_SMOOTH);
+        if (fuse_far0) {
+            order.push_back(UNIT_TRANSPORT_THERMAL0);
+        } else {
+            order.push_back(HG_PASS_TRANSPORT);
+            if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_0);
+        }
+        if (fuse_st1) {
+            order.push_back(UNIT_SMOOTH_THERMAL1);
+        } else {
+            if (thermal) order.push_back(HG_PASS_THERMAL_TRANSPORT_1);
+            if (smooth) order.push_back(HG_PASS_SMOOTH);
+        }
     }
 
     const dim3 blk(BX, BY), grd((n + BX - 1) / BX, (n + BY - 1) / BY);
@@ -1607,6 +1652,9 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
             std::vector<std::vector<float*>> fresh_of(order.size());
             for (size_t oi = 0; oi < order.size(); ++oi)
                 for (size_t q = 0; q < produced(order[oi]).size(); ++q) fresh_of[oi].push_back(take(w.pool));
+            // the transport's adjoint planes (before, after) for the far taps
+            // of UNIT_THERMAL0_FAR, set by UNIT_ST1_TRANSPORT
+            Planes far_lo{}, far_li{};
             // pass adjoints in reverse order (rain, water += rain, is the identity)
             for (int oi = int(order.size()) - 1; oi >= 0; --oi) {
                 const int pass = order[oi];
@@ -1632,6 +1680,21 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
                         HG_HIP(c, hipGetLastError());
                         k_adj_transport_far<<<grd, blk, 0, c->stream>>>(X, lo, li, n, n, c->kp);
                         break;
+                    case UNIT_ST1_TRANSPORT: {
+                        const Planes Xs = x_of(UNIT_SMOOTH_THERMAL1), Xt = x_of(HG_PASS_TRANSPORT);
+                        k_adj_smooth_thermal1_transport<<<2 * gtile.x * gtile.y, GNT, 0, c->stream>>>(
+                            Xs, lo, li, tp[T_D3], Xt, lo, li, n, n, c->kp, w.gpart);
+                        far_lo = lo;
+                        far_li = li;
+                        break;
+                    }
+                    case UNIT_THERMAL0_FAR: {
+                        const Planes Xt = x_of(HG_PASS_TRANSPORT), X0 = x_of(HG_PASS_THERMAL_TRANSPORT_0);
+                        const unsigned nfar = unsigned((size_t(n) * size_t(n) + GNT - 1) / GNT);
+                        k_adj_thermal0_far<<<gtile.x * gtile.y + nfar, GNT, 0, c->stream>>>(X0, lo, li, Xt, far_lo,
+                                                                                            far_li, n, n, c->kp);
+                        break;
+                    }
                     case UNIT_TRANSPORT_THERMAL0: {
                         const Planes Xt = x_of(HG_PASS_TRANSPORT), X0 = x_of(HG_PASS_THERMAL_TRANSPORT_0);
                         k_adj_transport<<<gtile, GNT, 0, c->stream>>>(Xt, lo, li, n, n, c->kp, w.gpart);

@@ -62,8 +62,8 @@ struct hg_ctx {
     size_t adj_bytes = 0;
     hipStream_t adj_stream = nullptr;
     hipEvent_t adj_ev[5] = {};
-    int adj_fuse = HG_ADJ_FUSE_EROSION_FLUX | HG_ADJ_FUSE_SMOOTH_THERMAL1 |
-                   HG_ADJ_FUSE_FAR_THERMAL0;   // hg_set_adjoint_fusion
+    int adj_fuse = HG_ADJ_FUSE_EROSION_FLUX | HG_ADJ_FUSE_SMOOTH_THERMAL1 | HG_ADJ_FUSE_FAR_THERMAL0 |
+                   HG_ADJ_FUSE_TRANSPORT_BESIDE;   // hg_set_adjoint_fusion
     std::string arch;
     std::string err;
     // timing

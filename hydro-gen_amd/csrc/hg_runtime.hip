@@ -571,7 +571,8 @@ int hg_get_droplet_order(const hg_ctx* c, int* mode) {
 
 int hg_set_adjoint_fusion(hg_ctx* c, int on) {
     if (!c) return HG_ERR_INVALID;
-    if (on < 0 || on > (HG_ADJ_FUSE_EROSION_FLUX | HG_ADJ_FUSE_SMOOTH_THERMAL1 | HG_ADJ_FUSE_FAR_THERMAL0))
+    if (on < 0 || on > (HG_ADJ_FUSE_EROSION_FLUX | HG_ADJ_FUSE_SMOOTH_THERMAL1 | HG_ADJ_FUSE_FAR_THERMAL0 |
+                        HG_ADJ_FUSE_TRANSPORT_BESIDE))
         return fail(c, HG_ERR_INVALID, "hg_set_adjoint_fusion: a combination of the HG_ADJ_FUSE_* bits");
     c->adj_fuse = on;
     return HG_OK;

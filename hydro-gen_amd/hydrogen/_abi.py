@@ -403,9 +403,10 @@ class Terrain:
     def set_adjoint_fusion(self, on: int):
         """hg_set_adjoint_fusion: bit 0 the erosion and flux adjoints of a step
         in one kernel, bit 1 the smoothing and thermal layer-1 adjoints, bit 2
-        the transport's far-tap scatter in the thermal layer-0 launch (all set
-        by default); a clear bit runs them one launch each.  Every combination
-        gives the same gradients."""
+        the transport's far-tap scatter in the thermal layer-0 launch, bit 3
+        the transport's gather in bit 1's launch (all set by default); a clear
+        bit runs them one launch each.  Every combination gives the same
+        gradients."""
         self._check(self.lib.hg_set_adjoint_fusion(self.h, on))
 
     def get_adjoint_fusion(self) -> int:

@@ -428,8 +428,10 @@ int hg_ref_particle_stats(hg_ctx* ctx, uint64_t* cas_giveups);
 enum {
     HG_ADJ_FUSE_EROSION_FLUX = 1,      /* hydro_erosion + hydro_flux */
     HG_ADJ_FUSE_SMOOTH_THERMAL1 = 2,   /* smoothing + thermal layer 1 (with both flags) */
-    HG_ADJ_FUSE_FAR_THERMAL0 = 4       /* sediment_transport's scatter of back-trace taps beyond one
+    HG_ADJ_FUSE_FAR_THERMAL0 = 4,      /* sediment_transport's scatter of back-trace taps beyond one
                                           cell + thermal layer 0 (with the thermal flag) */
+    HG_ADJ_FUSE_TRANSPORT_BESIDE = 8   /* sediment_transport's gather in the launch of bit 2's kernel
+                                          (with bits 2 and 4 and both flags) */
 };
 int hg_set_adjoint_fusion(hg_ctx* ctx, int on);
 int hg_get_adjoint_fusion(const hg_ctx* ctx, int* on);

@@ -162,7 +162,7 @@ def _grad_with_fusion(n, st, g, steps, k_ckpt, fuse, flags=hg.STEP_DEFAULT):
     with hg.Terrain(n) as t:
         t.set_erosion(hg.erosion_settings(Kc=COEF[0], Ks=COEF[1], Kd=COEF[2], Ke=COEF[3]))
         t.set_rain(hg.RAIN_CONSTANT, amount=0.01)
-        assert t.get_adjoint_fusion() == 7          # the default: every grouping
+        assert t.get_adjoint_fusion() == 15         # the default: every grouping
         t.set_adjoint_fusion(fuse)
         assert t.get_adjoint_fusion() == fuse
         t.write_state(st)
@@ -175,9 +175,9 @@ def _grad_with_fusion(n, st, g, steps, k_ckpt, fuse, flags=hg.STEP_DEFAULT):
                                      (64, hg.STEP_RAIN | hg.STEP_SMOOTH)])
 def test_fused_adjoint_kernels_equal_separate_kernels(n, flags):
     """The erosion + flux VJPs in one kernel, the smoothing + thermal layer-1
-    VJPs in one kernel and the transport's far taps in the thermal layer-0
-    launch give the gradients of the separate kernels bit for bit, in every
-    combination (sides that fill the 64 x 8 gather tiles partly, several
+    VJPs in one kernel, the transport's far taps in the thermal layer-0
+    launch and the transport's gather in the smoothing + layer-1 launch give
+    the gradients of the separate kernels bit for bit, in every combination (sides that fill the 64 x 8 gather tiles partly, several
     segments, flag subsets)."""
     h0, tgt = setup_case(n, seed=9)
     st = np.zeros((hg.NUM_STATE_PLANES, n, n), np.float32)
@@ -189,7 +189,7 @@ def test_fused_adjoint_kernels_equal_separate_kernels(n, flags):
     print("case", n, flags, flush=True)
     gh0, gc0 = _grad_with_fusion(n, st, g, 7, 3, 0, flags)
     assert np.abs(gh0).max() > 0
-    for fuse in range(1, 8):
+    for fuse in range(1, 16):
         gh, gc = _grad_with_fusion(n, st, g, 7, 3, fuse, flags)
         assert np.array_equal(gh, gh0), fuse
         assert np.array_equal(gc, gc0), fuse
@@ -209,7 +209,7 @@ def test_fused_erosion_flux_adjoint_with_long_back_traces():
     st[hg.P_SED_ROCK] = rng.uniform(0, 1, (n, n))
     st[hg.P_SED_DIRT] = rng.uniform(0, 1, (n, n))
     g = rng.standard_normal((n, n)).astype(np.float32)
-    gh1, gc1 = _grad_with_fusion(n, st, g, 2, 1, 7)
+    gh1, gc1 = _grad_with_fusion(n, st, g, 2, 1, 15)
     gh0, gc0 = _grad_with_fusion(n, st, g, 2, 1, 0)
     # far taps are summed with float atomics, whose order is not fixed
     assert np.abs(gh1 - gh0).max() <= 1e-6 * np.abs(gh0).max()
@@ -219,7 +219,7 @@ def test_fused_erosion_flux_adjoint_with_long_back_traces():
 def test_adjoint_fusion_argument_rejected():
     with hg.Terrain(32) as t:
         with pytest.raises(hg.HGError):
-            t.set_adjoint_fusion(8)
+            t.set_adjoint_fusion(16)
         with pytest.raises(hg.HGError):
             t.set_adjoint_fusion(-1)
-        assert t.get_adjoint_fusion() == 7
+        assert t.get_adjoint_fusion() == 15
