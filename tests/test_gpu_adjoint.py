@@ -170,7 +170,8 @@ def _grad_with_fusion(n, st, g, steps, k_ckpt, fuse, flags=hg.STEP_DEFAULT):
         return t.adjoint_rollout(steps, k_ckpt, g, flags)
 
 
-@pytest.mark.parametrize("n,flags", [(40, hg.STEP_DEFAULT), (104, hg.STEP_DEFAULT), (136, hg.STEP_RAIN),
+@pytest.mark.parametrize("n,flags", [(8, hg.STEP_DEFAULT), (16, hg.STEP_DEFAULT), (40, hg.STEP_DEFAULT),
+                                     (104, hg.STEP_DEFAULT), (136, hg.STEP_RAIN),
                                      (72, hg.STEP_RAIN | hg.STEP_THERMAL),
                                      (64, hg.STEP_RAIN | hg.STEP_SMOOTH)])
 def test_fused_adjoint_kernels_equal_separate_kernels(n, flags):
