@@ -1520,7 +1520,7 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
         if (rz != HG_OK) return rz;
     }
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_ROCK], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HG_HIP(c, hipMemcpyAsync(lam[HG_P_DIRT], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HG_HIP(c, hipMemcpyAsync(lam[HG_P_DIRT], lam[HG_P_ROCK], pn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
 
     // reverse units: a thermal layer (outflow then transport) is one unit,
     // named by its transport pass
