@@ -1295,11 +1295,13 @@ __global__ __launch_bounds__(256) void k_zero_planes(PlaneList l, int count, siz
 
 namespace {
 
-// The erosion and flux VJPs as one reverse unit (k_adj_erosion_flux), and
-// the smoothing and thermal layer-1 VJPs (k_adj_smooth_thermal1).
+// Grouped reverse units (hg_set_adjoint_fusion): the erosion and flux VJPs
+// (k_adj_erosion_flux), the smoothing and thermal layer-1 VJPs
+// (k_adj_smooth_thermal1), the transport's near gather then thermal layer 0
+// with the transport's far taps (k_adj_transport, k_adj_thermal0_far).
 constexpr int UNIT_EROSION_FLUX = 1000;
 constexpr int UNIT_SMOOTH_THERMAL1 = 1001;
-constexpr int UNIT_TRANSPORT_THERMAL0 = 1002;  // k_adj_transport, then k_adj_thermal0_far
+constexpr int UNIT_TRANSPORT_THERMAL0 = 1002;
 // With all four fusions: the smoothing + layer-1 and transport tiles in one
 // launch (k_adj_smooth_thermal1_transport), then layer 0 and the far taps.
 constexpr int UNIT_ST1_TRANSPORT = 1003;
@@ -1522,8 +1524,11 @@ extern "C" int hg_adjoint_rollout(hg_ctx* c, uint32_t nsteps, uint32_t k_ckpt, u
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_ROCK], dL_dH_final, pn * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HG_HIP(c, hipMemcpyAsync(lam[HG_P_DIRT], lam[HG_P_ROCK], pn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
 
-    // reverse units: a thermal layer (outflow then transport) is one unit,
-    // named by its transport pass
+    // reverse units in forward order (they run from the last): a thermal
+    // layer (outflow then transport) is one unit, named by its transport
+    // pass.  The sediment transport and thermal layer 0 touch disjoint
+    // planes, so the grouped units reverse the transport first and layer 0
+    // after it.
     std::vector<int> order;
     if (c->adj_fuse & HG_ADJ_FUSE_EROSION_FLUX) {
         order.push_back(UNIT_EROSION_FLUX);
