@@ -421,10 +421,12 @@ int hg_ref_particle_stats(hg_ctx* ctx, uint64_t* cas_giveups);
  * k = min(k_ckpt, nsteps) and s = 2, or 1 for a single segment) and a second
  * stream stay with the context for the next call, which reuses them when they
  * are large enough, until hg_destroy. */
-/* Reverse-kernel grouping of hg_adjoint_rollout (no counterpart in the
+/* Reverse-launch grouping of hg_adjoint_rollout (no counterpart in the
  * reference): a combination of the bits below, all set by default.  With a
- * bit set, the adjoints it names run as one kernel per step, otherwise one
- * kernel each; every combination gives the same gradients bit for bit. */
+ * bit set, the adjoints it names share one launch per step, otherwise each has
+ * its own; every combination gives the same gradients bit for bit (the float
+ * atomics that sum back-traces longer than one cell run in no fixed order in
+ * any grouping). */
 enum {
     HG_ADJ_FUSE_EROSION_FLUX = 1,      /* hydro_erosion + hydro_flux */
     HG_ADJ_FUSE_SMOOTH_THERMAL1 = 2,   /* smoothing + thermal layer 1 (with both flags) */
