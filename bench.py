@@ -252,7 +252,8 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     a target made by the forward model from another seed with coefficients
     scaled by U(0.8, 1.2); gradients w.r.t. the initial height and
     (Kc, Ks, Kd, Ke) through the adjoint with a checkpoint every k_ckpt steps.
-    Reports t(forward + adjoint) / t(forward)."""
+    Reports t(forward + adjoint) / t(forward), each the median of 3 warm runs
+    (forward and adjoint alternating)."""
     coef = dict(Kc=0.05, Ks=0.02, Kd=0.02, Ke=0.01)
     rng = np.random.default_rng(9)
     pert = {k: v * float(rng.uniform(0.8, 1.2)) for k, v in coef.items()}
@@ -271,58 +272,67 @@ def bench_adjoint(n: int = 1024, steps: int = 256, k_ckpt: int = 16) -> dict:
     t.set_erosion(hg.erosion_settings(**coef))
     t.init_fbm(hg.fbm_params(seed=11, base_freq=1.0 / 256, hi=64.0))
     init = t.read_state(hg.NUM_STATE_PLANES)
-    # forward alone
+    reps = 3
+
+    def time_forward() -> float:
+        t.write_state(init)
+        t.step = 0
+        t.sync()
+        t0 = time.perf_counter()
+        t.dispatch_grid(steps)
+        t.sync()
+        return time.perf_counter() - t0
+
+    def time_adjoint(k: int):
+        t.write_state(init)
+        t.step = 0
+        t.sync()
+        t0 = time.perf_counter()
+        out = t.adjoint_rollout(steps, k, g)
+        return time.perf_counter() - t0, out
+
+    # forward alone (one untimed run first), then the loss gradient at its end
     t.dispatch_grid(steps)
     t.sync()
-    t.write_state(init)
-    t.step = 0
-    t.sync()
-    t0 = time.perf_counter()
-    t.dispatch_grid(steps)
-    t.sync()
-    t_fwd = time.perf_counter() - t0
+    time_forward()
     h = t.read_plane(hg.P_ROCK).astype(np.float64) + t.read_plane(hg.P_DIRT)
     loss = float(((h - target) ** 2).mean())
     g = (2.0 * (h - target) / h.size).astype(np.float32)
-    # forward + adjoint (the adjoint call runs its own checkpointed forward);
-    # one untimed call first, as the forward above is timed warm
-    t.write_state(init)
-    t.step = 0
-    t.adjoint_rollout(steps, k_ckpt, g)
-    t.write_state(init)
-    t.step = 0
-    t.sync()
-    t0 = time.perf_counter()
-    gh, gc = t.adjoint_rollout(steps, k_ckpt, g)
-    t_adj = time.perf_counter() - t0
+    # forward + adjoint (the adjoint call runs its own checkpointed forward),
+    # one untimed call first; then `reps` forwards and adjoints, alternating,
+    # and the median of each
+    time_adjoint(k_ckpt)
+    fwd_s, adj_s = [], []
+    for _ in range(reps):
+        fwd_s.append(time_forward())
+        dt, (gh, gc) = time_adjoint(k_ckpt)
+        adj_s.append(dt)
+    t_fwd, t_adj = float(np.median(fwd_s)), float(np.median(adj_s))
     # the same rollout taped whole by the forward (one segment: no
     # recompute; 11 + 9 planes per step stay in HBM, 20 GB at 1024^2 x 256)
-    t.write_state(init)
-    t.step = 0
-    t.adjoint_rollout(steps, steps, g)
-    t.write_state(init)
-    t.step = 0
-    t.sync()
-    t0 = time.perf_counter()
-    gh_full, gc_full = t.adjoint_rollout(steps, steps, g)
-    t_full = time.perf_counter() - t0
+    time_adjoint(steps)
+    full_s = []
+    for _ in range(reps):
+        dt, (gh_full, gc_full) = time_adjoint(steps)
+        full_s.append(dt)
+    t_full = float(np.median(full_s))
     # the checkpointed rollout again with one launch per pass adjoint (no
     # fused reverse launches)
     t.set_adjoint_fusion(0)
-    t.write_state(init)
-    t.step = 0
-    t.adjoint_rollout(steps, k_ckpt, g)
-    t.write_state(init)
-    t.step = 0
-    t.sync()
-    t0 = time.perf_counter()
-    gh_sep, gc_sep = t.adjoint_rollout(steps, k_ckpt, g)
-    t_sep = time.perf_counter() - t0
+    time_adjoint(k_ckpt)
+    sep_s = []
+    for _ in range(reps):
+        dt, (gh_sep, gc_sep) = time_adjoint(k_ckpt)
+        sep_s.append(dt)
+    t_sep = float(np.median(sep_s))
     t.set_adjoint_fusion(15)
     t.close()
     return {"grid": n, "steps": steps, "k_ckpt": k_ckpt, "tile_variants": list(tiles), "loss": loss,
             "t_forward_ms": t_fwd * 1e3, "t_forward_adjoint_ms": t_adj * 1e3,
-            "cost_ratio": t_adj / t_fwd, "grad_coef": [float(x) for x in gc],
+            "cost_ratio": t_adj / t_fwd, "timing": f"median of {reps} runs each, forward and adjoint alternating",
+            "runs_ms": {"forward": [round(x * 1e3, 3) for x in fwd_s],
+                        "forward_adjoint": [round(x * 1e3, 3) for x in adj_s]},
+            "grad_coef": [float(x) for x in gc],
             "grad_h0_absmax": float(np.abs(gh).max()), "grad_finite": bool(np.isfinite(gh).all()),
             "full_tape": {"k_ckpt": steps, "t_forward_adjoint_ms": t_full * 1e3, "cost_ratio": t_full / t_fwd,
                           "same_gradient": bool(np.array_equal(gh_full, gh)
