@@ -44,7 +44,7 @@ struct hg_batch {
     int hydro_variant = 0, thermal_variant = 1;
     size_t total_floats = 0;            // size of `block`
     bool persistent = false;            // false: one workgroup per work-list entry
-    bool xcd_order = false;             // interleave the work-list for the 8 XCDs
+    bool xcd_order = true;              // interleave the work-list for the 8 XCDs
     uint32_t* dev_tiles_h = nullptr;
     uint32_t* dev_tiles_t = nullptr;
     int ntiles_h = 0, ntiles_t = 0;

@@ -505,8 +505,9 @@ int hg_batch_autotune(hg_batch* b, uint32_t steps, uint32_t flags, int* hydro_be
 /* 0 (default): each batched launch has one workgroup per tile of the work-list;
  * 1: a persistent grid of resident workgroups strides over the work-list. */
 int hg_batch_set_persistent(hg_batch* b, int persistent);
-/* 1: the work-list is cut into 8 contiguous chunks interleaved so that each
- * XCD works through one chunk; 0 (default): natural (terrain, row, column). */
+/* 1 (default): the work-list is cut into 8 contiguous chunks interleaved so
+ * that each XCD works through one chunk (its L2 then holds the halo rows its
+ * tiles share); 0: natural (terrain, row, column) order. */
 int hg_batch_set_xcd_order(hg_batch* b, int on);
 /* nsteps steps with persistent launches.  HG_STEP_PER_TERRAIN instead launches
  * the single-terrain fused kernels terrain by terrain; with HG_STEP_PER_PASS as

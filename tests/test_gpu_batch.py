@@ -51,6 +51,20 @@ def test_persistent_grid_equals_flat(states):
             assert all(np.array_equal(st[p], ref[i][p]) for p in range(hg.NUM_STATE_PLANES)), i
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_xcd_chunked_order_equals_natural_order(states, persistent):
+    """The XCD-chunked work-list (the default) and the natural one give the
+    same terrains bit for bit: tiles are independent within a launch."""
+    ref = run_mode(states, 4, hg.STEP_DEFAULT)
+    with make_batch(states) as b:
+        b.set_xcd_order(False)
+        b.set_persistent(persistent)
+        b.step(4)
+        for i in range(len(SIZES)):
+            st = b.read_state(i)
+            assert all(np.array_equal(st[p], ref[i][p]) for p in range(hg.NUM_STATE_PLANES)), i
+
+
 def test_batched_equals_per_terrain_and_per_pass(states):
     a = run_mode(states, 5, hg.STEP_DEFAULT)
     b = run_mode(states, 5, hg.STEP_DEFAULT | hg.STEP_PER_TERRAIN)
