@@ -51,13 +51,31 @@ __device__ __forceinline__ void aadd(float* p, float v) {
 
 // The tile a workgroup of a gather kernel works on: tile (x, y) of the
 // GTX x GTY tiling, and its slot of the coefficient partial sums (the tile's
-// row-major index).  block_tile(): the workgroup's own tile, one per
-// workgroup of a (tiles in x) x (tiles in y) grid.
+// row-major index, whatever order the tiles run in).
 struct TileId {
     int x, y, slot;
 };
+// Tile of the b-th of nb workgroups over a tiling gx tiles wide.  Workgroups
+// go to the 8 XCDs in turn; when gx and nb are multiples of 8, XCD x takes
+// the column strip of tiles [x*gx/8, (x+1)*gx/8) row by row, so its L2 holds
+// the tiles' left, right, upper and lower neighbours' staged rows.
+// Otherwise row-major.
+__device__ __forceinline__ TileId tile_at(int b, int nb, int gx) {
+    int x, y;
+    if ((gx & 7) == 0 && (nb & 7) == 0) {
+        const int sw = gx >> 3, j = b >> 3;
+        y = j / sw;
+        x = (b & 7) * sw + (j - y * sw);
+    } else {
+        y = b / gx;
+        x = b - y * gx;
+    }
+    return {x, y, y * gx + x};
+}
+// The workgroup's own tile, one per workgroup of a (tiles in x) x (tiles in
+// y) grid.
 __device__ __forceinline__ TileId block_tile() {
-    return {int(blockIdx.x), int(blockIdx.y), int(blockIdx.y * gridDim.x + blockIdx.x)};
+    return tile_at(int(blockIdx.y * gridDim.x + blockIdx.x), int(gridDim.x * gridDim.y), int(gridDim.x));
 }
 
 // Block reduction of up to 4 doubles, added by thread 0 to the block's own 4
@@ -1000,7 +1018,7 @@ __global__ __launch_bounds__(GNT) void k_adj_thermal0_far(Planes x0, Planes lo0,
     const int ntx = (nx + GTX - 1) / GTX, nty = (ny + GTY - 1) / GTY;
     const int w = int(blockIdx.x);
     if (w < ntx * nty) {
-        adj_thermal_tile<0>(TileId{w % ntx, w / ntx, w}, lds, x0, lo0, li0, nx, ny, k);
+        adj_thermal_tile<0>(tile_at(w, ntx * nty, ntx), lds, x0, lo0, li0, nx, ny, k);
         return;
     }
     const size_t cell = size_t(w - ntx * nty) * GNT + threadIdx.x;
@@ -1258,10 +1276,9 @@ __global__ __launch_bounds__(GNT) void k_adj_smooth_thermal1_transport(Planes xs
     const int ntx = (nx + GTX - 1) / GTX, nty = (ny + GTY - 1) / GTY, nt = ntx * nty;
     const int w = int(blockIdx.x);
     if (w < nt) {
-        adj_smooth_thermal1_tile(TileId{w % ntx, w / ntx, w}, u, xs, los, lis, d3, nx, ny, k);
+        adj_smooth_thermal1_tile(tile_at(w, nt, ntx), u, xs, los, lis, d3, nx, ny, k);
     } else {
-        const int t = w - nt;
-        adj_transport_tile(TileId{t % ntx, t / ntx, t}, u, xt, lot, lit, nx, ny, k, gpart);
+        adj_transport_tile(tile_at(w - nt, nt, ntx), u, xt, lot, lit, nx, ny, k, gpart);
     }
 }
 
