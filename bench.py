@@ -11,7 +11,8 @@ smoothing every step.  W warm-up steps, then K timed steps (default: steps
 Prints one JSON line.  `value` is cell-steps/s over the timed steps with the
 state resident in HBM.  `roofline` is the dominant kernel's algorithmic HBM
 bytes per launch over its average launch time, measured with HIP events on
-the library's stream inside the timed region.  `cpu_baseline` is the FP32
+the library's stream inside the timed region; its `step_frac` is the whole
+step's rate at 72 B per cell-step over the same peak.  `cpu_baseline` is the FP32
 oracle (a CPU port of the same passes, OpenMP over rows) on a bounded sample.
 """
 from __future__ import annotations
@@ -474,7 +475,9 @@ def main() -> int:
         roofline = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": ach / HBM_PEAK,
                     "traffic": profiled_traffic(dom, n, variant),
-                    "algo_bytes_per_launch": per_cell * n * n}
+                    "algo_bytes_per_launch": per_cell * n * n,
+                    # the whole step (every kernel of it) at 72 B per cell-step
+                    "step_frac": value * STEP_ALGO_BYTES / HBM_PEAK}
 
     log(f"{args.steps} GRID steps at {n}^2: {wall / args.steps * 1e3:.3f} ms/step")
     h = t.read_plane(hg.P_ROCK)
