@@ -432,8 +432,8 @@ def main() -> int:
     ap.add_argument("--no-tune", dest="tune", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on exactly one GPU: the erosion path is measured per GPU and "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: the erosion path is measured per GPU; "
+              "the multi-GPU sharding is designed but not implemented", file=sys.stderr)
         return 2
 
     n = args.n
